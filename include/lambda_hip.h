@@ -1,0 +1,262 @@
+/*
+ * lambda_hip.h - C ABI of the MI355X (gfx950) spectral path-tracing integrator.
+ *
+ * This is the drop-in boundary for Lambda's render call.  In the reference the
+ * host parses a scene (core/parser.h:29), uploads the flat arrays
+ * (src/main.cpp:84-109), launches renderKernel / Renderer::renderImage
+ * (src/main.cpp:113-127, render/renderer.h:42-55,200-212), polls a progress
+ * counter (src/main.cpp:129-146) and copies back a W*H*3 float buffer of
+ * linear sRGB averaged over the samples (src/main.cpp:148-156,
+ * render/renderer.h:87-91).  lambda_hip_render() replaces exactly that block.
+ *
+ * Everything crossing this boundary is plain old data: counts, pointers to
+ * host arrays and sizes.  The arrays describe the scene after the host has
+ * parsed it and built its BVH, in flat layouts of this library's own (no C++
+ * unions).  The caller owns every array; the library only reads them.
+ *
+ * Return value of every int-returning entry point: 0 on success, otherwise a
+ * LAMBDA_E* code, with a NUL-terminated message written to err[0..err_len).
+ * Nothing throws across this boundary.
+ */
+#ifndef LAMBDA_HIP_H
+#define LAMBDA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Wavelength grid of every dense spectrum: 1 nm bins over 360..830 nm
+ * (core/constants.h:20-22). */
+#define LAMBDA_CIE_LAMBDA_MIN 360
+#define LAMBDA_CIE_LAMBDA_MAX 830
+#define LAMBDA_CIE_BINS 471
+
+/* Deepest BVH the traversal kernels accept (bvh.h:221 caps leaves at depth 20,
+ * which bounds a depth-first stack at 20 entries, scene.h:52-57). */
+#define LAMBDA_MAX_STACK 20
+
+enum lambda_error {
+    LAMBDA_OK = 0,
+    LAMBDA_EINVAL = 1,    /* malformed scene or parameters */
+    LAMBDA_EHIP = 2,      /* HIP runtime error */
+    LAMBDA_ENOMEM = 3,    /* device or host allocation failed */
+    LAMBDA_ENODEV = 4,    /* no gfx950 device visible */
+    LAMBDA_EIO = 5,       /* file could not be read or written */
+    LAMBDA_EPARSE = 6,    /* scene file rejected by the parser */
+    LAMBDA_EBUSY = 7      /* batch queue full */
+};
+
+/* Object kinds (scene/object.h:13). */
+enum lambda_object_type { LAMBDA_OBJECT_SPHERE = 0, LAMBDA_OBJECT_QUAD = 1, LAMBDA_OBJECT_TRI = 2 };
+
+/* Material kinds (scene/material.h:15). */
+enum lambda_material_type {
+    LAMBDA_MATERIAL_LAMBERTIAN = 0,
+    LAMBDA_MATERIAL_MIRROR = 1,
+    LAMBDA_MATERIAL_DIELECTRIC = 2,
+    LAMBDA_MATERIAL_EMISSIVE = 3,
+    LAMBDA_MATERIAL_THINFILM = 4
+};
+
+/* Scalar texture kinds (scene/texture.h:8). */
+enum lambda_scalar_texture_type {
+    LAMBDA_SCALAR_CONSTANT = 0,
+    LAMBDA_SCALAR_PERLIN = 1,
+    LAMBDA_SCALAR_WORLEY = 2
+};
+
+/* Spectrum texture kinds (scene/texture.h:9). */
+enum lambda_spectrum_texture_type {
+    LAMBDA_SPECTRUM_CONSTANT = 0,
+    LAMBDA_SPECTRUM_CHECKER = 1,
+    LAMBDA_SPECTRUM_SCALAR = 2
+};
+
+/*
+ * A parsed scene with its BVH, as flat arrays.  Field groups mirror the
+ * arguments of Renderer::setScene (render/renderer.h:38) and Camera
+ * (render/camera.h:10):
+ *
+ *  spectra            [num_spectra][471] float      DenseSpectrum<Float>
+ *  complex_spectra    [num_complex_spectra][471][2] double (re, im)
+ *  scalar_texture_*   type[n]; params[n][3] = {value,0,0} | {min,max,frequency}
+ *  spectrum_texture_* type[n]; index[n][2] = {spectrum,0} | {spectrum1,spectrum2}
+ *                     | {scalar texture,0}; scale[n] (checker only)
+ *  material_*         type[n]; params[n][3]:
+ *                       lambertian/mirror {albedo spectrum texture}
+ *                       dielectric {n0 spectrum, n1 spectrum}
+ *                       emissive {emission spectrum texture}
+ *                       thinfilm {numLayers, offset of n, offset of d} into
+ *                       material_properties (parser.h:183-210)
+ *  object_*           type[n]; material[n]; geometry[n][12]:
+ *                       sphere {cx,cy,cz, r, 0...}
+ *                       quad/tri {corner[3], horizontal[3], vertical[3],
+ *                                 normal[3] = cross(horizontal, vertical)}
+ *                     shading[n][15] (triangles): {n0[3], n1[3], n2[3],
+ *                                 u0,u1,u2, v0,v1,v2}
+ *                     Objects are in BVH order (bvh.h:98-105).
+ *  instance_*         ints[n][4] = {first object, object count, BLAS root,
+ *                     material override or -1}; matrices[n][32] = object to
+ *                     world 4x4 row-major, then its inverse (instance.h:114-115).
+ *                     Instances are in BVH order (bvh.h:74-81).
+ *  node_*             bounds[n][6] = {min xyz, max xyz}; ints[n][2] = {index,
+ *                     count}: count > 0 is a leaf over [index, index+count);
+ *                     count == 0 is an inner node with left child at i+1 and
+ *                     right child at index (bvh.h:14-28).  Node 0 is the root of
+ *                     the instance level; each instance's BLAS root is in
+ *                     instance ints[.][2].
+ *  light_*            num_lights entries; the last one is the environment with
+ *                     instance and object -1 (parser.h:335-378).
+ */
+typedef struct lambda_scene_view {
+    int32_t num_spectra;
+    const float *spectra;
+    int32_t num_complex_spectra;
+    const double *complex_spectra;
+
+    int32_t num_scalar_textures;
+    const int32_t *scalar_texture_type;
+    const float *scalar_texture_params;
+
+    int32_t num_spectrum_textures;
+    const int32_t *spectrum_texture_type;
+    const int32_t *spectrum_texture_index;
+    const float *spectrum_texture_scale;
+
+    int32_t num_materials;
+    const int32_t *material_type;
+    const int32_t *material_params;
+    int32_t num_material_properties;
+    const int32_t *material_properties;
+
+    int32_t num_objects;
+    const int32_t *object_type;
+    const int32_t *object_material;
+    const float *object_geometry;
+    const float *object_shading;
+
+    int32_t num_instances;
+    const int32_t *instance_ints;
+    const float *instance_matrices;
+
+    int32_t num_nodes;
+    const float *node_bounds;
+    const int32_t *node_ints;
+
+    int32_t num_lights;
+    const int32_t *light_instances;
+    const int32_t *light_objects;
+    const float *light_powers;
+    float total_light_power;
+
+    int32_t background_texture;
+    float camera[12]; /* position, corner, horizontal, vertical */
+} lambda_scene_view;
+
+/* Random-number stream layouts. */
+enum lambda_rng_mode {
+    /* Each pixel owns PCG32 stream (seed, py*W+px) and its samples run one
+     * after another, consuming the stream draw for draw as the reference does
+     * (render/renderer.h:49,69-91).  Images match the reference CPU render up
+     * to libm rounding. */
+    LAMBDA_RNG_REFERENCE = 0,
+    /* Each (pixel, sample) pair owns its own PCG32 stream, so every sample of
+     * every pixel can be in flight at once.  Same estimator, different
+     * random numbers. */
+    LAMBDA_RNG_THROUGHPUT = 1
+};
+
+/* Render settings (Render command, parser.h:75-100; seed, main.cpp:122). */
+typedef struct lambda_render_params {
+    int32_t width;
+    int32_t height;
+    int32_t depth;         /* last bounce index; depth+1 segments */
+    int32_t samples;       /* perfect square */
+    int32_t sqrt_samples;
+    float lambda_min;      /* nm, >= 360 */
+    float lambda_max;      /* nm, <= 830 */
+    uint64_t seed;         /* the reference always passes 42 */
+    int32_t rng_mode;      /* enum lambda_rng_mode */
+    int32_t path_slots;    /* paths in flight (throughput mode); 0 = automatic */
+} lambda_render_params;
+
+/*
+ * One-shot render: uploads the scene, renders, writes out_rgb[W*H*3] (linear
+ * sRGB, mean over samples, row-major from the top image row) and frees every
+ * device allocation before returning.  progress, when non-NULL, receives the
+ * number of finished pixels (monotone, readable from another thread while the
+ * call runs; W*H on return).
+ * Replaces src/main.cpp:105-156 (renderKernel launch, poll loop, copy-back).
+ */
+int lambda_hip_render(const lambda_scene_view *scene, const lambda_render_params *params,
+                      float *out_rgb, volatile int32_t *progress, char *err, size_t err_len);
+
+/* A scene resident in device memory (the uploads of src/main.cpp:84-103). */
+typedef struct lambda_hip_scene lambda_hip_scene;
+
+/* Validates the scene view and copies it into device memory. */
+int lambda_hip_scene_create(const lambda_scene_view *scene, lambda_hip_scene **out,
+                            char *err, size_t err_len);
+
+/* Releases every device allocation of the scene. */
+void lambda_hip_scene_destroy(lambda_hip_scene *scene);
+
+/*
+ * Renders an uploaded scene; the same semantics as lambda_hip_render without
+ * the upload.  Replaces src/main.cpp:113-156.
+ */
+int lambda_hip_scene_render(lambda_hip_scene *scene, const lambda_render_params *params,
+                            float *out_rgb, volatile int32_t *progress, char *err, size_t err_len);
+
+/* Closest-hit record for one ray (math/intersection.h:8-17). */
+typedef struct lambda_hit {
+    int32_t hit;        /* 1 if the ray hit an object */
+    int32_t instance;   /* index into the (BVH-ordered) instance array */
+    int32_t object;     /* index into the (BVH-ordered) object array */
+    float t;
+    float point[3];     /* world space */
+    float normal[3];    /* world space, facing the ray */
+    float u, v;
+    int32_t front_facing;
+} lambda_hit;
+
+/*
+ * Closest-hit queries for rays[n][6] = {origin xyz, direction xyz} in world
+ * space: Scene::hit in closest-hit mode (scene/scene.h:51-124 with
+ * occludedObject = -2).
+ */
+int lambda_hip_closest_hit(lambda_hip_scene *scene, const float *rays, int32_t n,
+                           lambda_hit *hits, char *err, size_t err_len);
+
+/*
+ * Shadow queries: Scene::hit in occlusion mode (scene/scene.h:61-70,99,106)
+ * for rays[n][6] toward light (light_instance[i], light_object[i]); -1/-1 is
+ * the environment.  occluded[i] = 1 if Scene::hit returned true.
+ */
+int lambda_hip_occluded(lambda_hip_scene *scene, const float *rays, const int32_t *light_instance,
+                        const int32_t *light_object, int32_t n, int32_t *occluded,
+                        char *err, size_t err_len);
+
+/*
+ * Thin-film reflectance (scene/material.h:263-309) of material `material` for
+ * n queries {cos_theta, lambda_nm, front_facing (0/1), thickness scale}: the
+ * layer thicknesses come from the material's scalar textures evaluated at the
+ * query point point[i][3].  Computed by the same FP32 device code the
+ * integrator uses.  out[i] = reflectance.
+ */
+int lambda_hip_thinfilm_reflectance(lambda_hip_scene *scene, int32_t material, const float *queries,
+                                    const float *points, int32_t n, float *out,
+                                    char *err, size_t err_len);
+
+/* Device name and compute-unit count of the current HIP device. */
+int lambda_hip_device_info(char *name, size_t name_len, int32_t *compute_units,
+                           char *err, size_t err_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LAMBDA_HIP_H */

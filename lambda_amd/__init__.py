@@ -1,0 +1,133 @@
+"""lambda_amd: MI355X-native spectral path-tracing integrator for Lambda scenes.
+
+Python mirror of the reference's host interface:
+
+* ``Scene.load(path)`` is ``Parser::parseLRD`` (core/parser.h:29): it parses a
+  .lrd file with the C++ front end (liblambda_host.so) and builds the BVH.
+* ``render(scene, ...)`` is the render call of src/main.cpp:113-156: it calls
+  ``lambda_hip_render`` in liblambda_hip.so and returns the W x H x 3 float
+  linear-sRGB image averaged over samples.
+* ``write_ppm(path, image)`` is ``Writer::write`` (core/writer.h:17).
+
+Errors from the C ABI are raised as ``LambdaError`` with the library's
+message, the way the reference prints its exceptions (src/main.cpp:76-79).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import LAMBDA_RNG_REFERENCE, LAMBDA_RNG_THROUGHPUT, RenderParams, SceneView
+
+__all__ = ["LambdaError", "Scene", "render", "write_ppm", "LAMBDA_RNG_REFERENCE", "LAMBDA_RNG_THROUGHPUT"]
+
+
+class LambdaError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+def _check(code: int, buf) -> None:
+    if code != 0:
+        raise LambdaError(code, buf.value.decode(errors="replace"))
+
+
+class Scene:
+    """A parsed scene (host arrays) with its BVH; owns the C++ object."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+        self.view = SceneView()
+        _abi.host().lambda_scene_get_view(self._h, C.byref(self.view))
+        self.params = RenderParams()
+        _abi.host().lambda_scene_get_params(self._h, C.byref(self.params))
+
+    @classmethod
+    def load(cls, path: str) -> "Scene":
+        h = C.c_void_p()
+        err = C.create_string_buffer(4096)
+        _check(_abi.host().lambda_scene_load(path.encode(), C.byref(h), err, 4096), err)
+        return cls(h.value)
+
+    @classmethod
+    def parse(cls, text: str, name: str = "<string>") -> "Scene":
+        h = C.c_void_p()
+        err = C.create_string_buffer(4096)
+        _check(_abi.host().lambda_scene_parse_string(text.encode(), name.encode(), C.byref(h), err, 4096), err)
+        return cls(h.value)
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            _abi.host().lambda_scene_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- numpy copies of the flat arrays (for tests and tools) ----
+    def arrays(self) -> dict:
+        v = self.view
+
+        def arr(p, n, dtype, shape):
+            if n == 0:
+                return np.zeros((0,) + shape[1:], dtype)
+            ct = np.ctypeslib.as_array(p, shape=(int(np.prod(shape)),))
+            return np.array(ct, dtype=dtype).reshape(shape)
+
+        B = _abi.CIE_BINS
+        return {
+            "spectra": arr(v.spectra, v.num_spectra, np.float32, (v.num_spectra, B)),
+            "complex_spectra": arr(v.complex_spectra, v.num_complex_spectra, np.float64, (v.num_complex_spectra, B, 2)),
+            "scalar_texture_type": arr(v.scalar_texture_type, v.num_scalar_textures, np.int32, (v.num_scalar_textures,)),
+            "scalar_texture_params": arr(v.scalar_texture_params, v.num_scalar_textures, np.float32, (v.num_scalar_textures, 3)),
+            "spectrum_texture_type": arr(v.spectrum_texture_type, v.num_spectrum_textures, np.int32, (v.num_spectrum_textures,)),
+            "spectrum_texture_index": arr(v.spectrum_texture_index, v.num_spectrum_textures, np.int32, (v.num_spectrum_textures, 2)),
+            "spectrum_texture_scale": arr(v.spectrum_texture_scale, v.num_spectrum_textures, np.float32, (v.num_spectrum_textures,)),
+            "material_type": arr(v.material_type, v.num_materials, np.int32, (v.num_materials,)),
+            "material_params": arr(v.material_params, v.num_materials, np.int32, (v.num_materials, 3)),
+            "material_properties": arr(v.material_properties, v.num_material_properties, np.int32, (v.num_material_properties,)),
+            "object_type": arr(v.object_type, v.num_objects, np.int32, (v.num_objects,)),
+            "object_material": arr(v.object_material, v.num_objects, np.int32, (v.num_objects,)),
+            "object_geometry": arr(v.object_geometry, v.num_objects, np.float32, (v.num_objects, 12)),
+            "object_shading": arr(v.object_shading, v.num_objects, np.float32, (v.num_objects, 15)),
+            "instance_ints": arr(v.instance_ints, v.num_instances, np.int32, (v.num_instances, 4)),
+            "instance_matrices": arr(v.instance_matrices, v.num_instances, np.float32, (v.num_instances, 32)),
+            "node_bounds": arr(v.node_bounds, v.num_nodes, np.float32, (v.num_nodes, 6)),
+            "node_ints": arr(v.node_ints, v.num_nodes, np.int32, (v.num_nodes, 2)),
+            "light_instances": arr(v.light_instances, v.num_lights, np.int32, (v.num_lights,)),
+            "light_objects": arr(v.light_objects, v.num_lights, np.int32, (v.num_lights,)),
+            "light_powers": arr(v.light_powers, v.num_lights, np.float32, (v.num_lights,)),
+            "total_light_power": np.float32(v.total_light_power),
+            "background_texture": int(v.background_texture),
+            "camera": np.array(v.camera[:], np.float32),
+        }
+
+    def render_params(self, **overrides) -> RenderParams:
+        p = RenderParams()
+        C.memmove(C.byref(p), C.byref(self.params), C.sizeof(RenderParams))
+        for k, val in overrides.items():
+            setattr(p, k, val)
+        return p
+
+
+def render(scene: Scene, rng_mode: int = LAMBDA_RNG_REFERENCE, seed: int = 42, path_slots: int = 0,
+           progress: np.ndarray | None = None) -> np.ndarray:
+    """Renders on the GPU through lambda_hip_render; returns float32 [H, W, 3]."""
+    p = scene.render_params(rng_mode=rng_mode, seed=seed, path_slots=path_slots)
+    out = np.zeros((p.height, p.width, 3), np.float32)
+    err = C.create_string_buffer(4096)
+    prog = None if progress is None else _abi.ptr(progress, C.c_int32)
+    _check(_abi.hip().lambda_hip_render(C.byref(scene.view), C.byref(p), _abi.ptr(out, C.c_float), prog, err, 4096), err)
+    return out
+
+
+def write_ppm(path: str, image: np.ndarray) -> None:
+    img = np.ascontiguousarray(image, np.float32)
+    err = C.create_string_buffer(4096)
+    _check(_abi.host().lambda_write_ppm(path.encode(), img.shape[1], img.shape[0], _abi.ptr(img, C.c_float), err, 4096), err)
