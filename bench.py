@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain GPU path tracer: paths/s at 1080p on one MI355X.
+
+Workload (default): the instanced-mesh scene -- a level-8 perlin-displaced
+icosphere (1,310,720 triangles) instanced 64 times with seeded rotation and
+scale 0.5-1.5, round-robin Lambertian / mirror / dielectric / thin-film
+materials, 4 quad emitters and a constant environment; 1920x1080, depth 10,
+64 spp, reference PCG32 streams.  One "step" is one full render of the frame
+(132.7M camera paths) with the scene already resident in HBM; the timed
+region of each step ends when the RGB image is back in host memory.
+
+Printed JSON line (rank 0, one GPU):
+  value        paths/s over the K timed renders
+  roofline     the extend (closest-hit traversal) kernel: algorithmic bytes
+               per launch (oracle-counted node/instance/primitive tests per
+               closest-hit ray on a strided pixel sample x rays per launch)
+               over its mean launch time from HIP events, against 8 TB/s
+  cpu_baseline the CPU oracle (bit-identical restatement of the reference
+               CPU path) on a band of rows at reduced spp, all host threads
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "paths/s @1080p; wall time to relMSE 1e-3, guided vs plain, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def make_workload(name: str, scene_dir: str):
+    from lambda_amd import scenes
+    if name == "instanced_mesh":
+        path = scenes.write_instanced_mesh(os.path.join(scene_dir, "instanced_mesh"))
+        cfg = {"workload": "instanced_mesh_1080p", "width": 1920, "height": 1080, "depth": 10, "spp": 64,
+               "mesh": "icosphere level 8, perlin-displaced (amplitude 0.05, frequency 6)",
+               "triangles_per_instance": 1310720, "instances": 64, "emitters": 4}
+    elif name == "closed_box":
+        path = scenes.write_closed_box(os.path.join(scene_dir, "closed_box"))
+        cfg = {"workload": "closed_box_256", "width": 256, "height": 256, "depth": 8, "spp": 16}
+    elif name == "thinfilm_grid":
+        path = scenes.write_thinfilm_grid(os.path.join(scene_dir, "thinfilm_grid"))
+        cfg = {"workload": "thinfilm_grid_512", "width": 512, "height": 512, "depth": 12, "spp": 64}
+    else:
+        raise SystemExit("unknown workload " + name)
+    return path, cfg
+
+
+def cpu_baseline(scene, seconds: float, threads: int) -> dict:
+    """Times the CPU oracle on a band of rows at reduced spp, calibrated to
+    roughly `seconds` of work; paths/s is independent of spp to first order."""
+    import oracle
+    p = scene.render_params()
+    spp, root = 4, 2
+    q = scene.render_params(samples=spp, sqrt_samples=root)
+    mid = p.height // 2
+    rows = 2
+    t = 0.0
+    while True:
+        r0 = max(0, mid - rows // 2)
+        r1 = min(p.height, r0 + rows)
+        t0 = time.perf_counter()
+        oracle.render(scene, q, threads=threads, rows=(r0, r1))
+        t = time.perf_counter() - t0
+        if t >= seconds / 4 or r1 - r0 >= p.height:
+            break
+        rows = min(p.height, max(rows * 2, int(rows * seconds / max(t, 1e-3) / 2)))
+    paths = (r1 - r0) * p.width * spp
+    return {"value": paths / t, "unit": "paths/s", "cores": threads, "kind": "port",
+            "sample": "rows %d-%d of %d (%d pixels) at %d spp, %.1f s; CPU oracle, bit-identical restatement of "
+                      "the reference CPU integrator, OpenMP guided schedule over rows" % (
+                          r0, r1, p.height, (r1 - r0) * p.width, spp, t)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="instanced_mesh", choices=["instanced_mesh", "closed_box", "thinfilm_grid"])
+    ap.add_argument("--rng", default="reference", choices=["reference", "throughput"])
+    ap.add_argument("--path-slots", type=int, default=0)
+    ap.add_argument("--scene-dir", default=os.environ.get("LAMBDA_SCENE_DIR", "/tmp/lambda_bench"))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stats-pixels", type=int, default=20000, help="pixels sampled to count traversal work")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the renderer has no multi-GPU path (frames shard by pixel bands; "
+              "see the design notes). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import lambda_amd
+    import oracle
+
+    name, cus = lambda_amd.device_info()
+    log("device %s, %d CUs" % (name, cus))
+    t0 = time.perf_counter()
+    path, cfg = make_workload(args.workload, args.scene_dir)
+    t_gen = time.perf_counter() - t0
+    log("scene files ready (%.1f s)" % t_gen)
+    t0 = time.perf_counter()
+    cwd = os.getcwd()
+    os.chdir(os.path.dirname(path))
+    try:
+        scene = lambda_amd.Scene.load(os.path.basename(path))
+    finally:
+        os.chdir(cwd)
+    t_parse = time.perf_counter() - t0
+    arrs_nodes = scene.view.num_nodes
+    log("parsed + BVH built in %.1f s (%d nodes, %d objects, %d instances)" % (
+        t_parse, arrs_nodes, scene.view.num_objects, scene.view.num_instances))
+    t0 = time.perf_counter()
+    dev = lambda_amd.DeviceScene(scene)
+    t_upload = time.perf_counter() - t0
+    mode = lambda_amd.LAMBDA_RNG_REFERENCE if args.rng == "reference" else lambda_amd.LAMBDA_RNG_THROUGHPUT
+    p = scene.params
+    paths_per_step = p.width * p.height * p.samples
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        dev.render(rng_mode=mode, path_slots=args.path_slots)
+        log("warmup %d: %.3f s" % (i, time.perf_counter() - t0))
+
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        img = dev.render(rng_mode=mode, path_slots=args.path_slots)
+    elapsed = time.perf_counter() - t0
+    value = paths_per_step * args.steps / elapsed
+    log("timed %d steps: %.3f s -> %.4g paths/s" % (args.steps, elapsed, value))
+    finite = bool(np.all(np.isfinite(img)))
+
+    # Kernel timing from HIP events on the render stream.
+    _, st = dev.render_stats(rng_mode=mode, path_slots=args.path_slots)
+    log("kernel ms: " + ", ".join("%s %.1f" % (k, st[k + "_ms"]) for k in ("extend", "shade", "shadow", "finalize")))
+
+    # Algorithmic bytes per closest-hit ray, counted by the oracle on a
+    # strided pixel sample (all samples of each sampled pixel).
+    npix = p.width * p.height
+    stride = max(1, npix // max(1, args.stats_pixels))
+    t0 = time.perf_counter()
+    ps = oracle.path_stats(scene, stride=stride)
+    t_stats = time.perf_counter() - t0
+    b_closest = oracle.ray_bytes(ps, "closest")
+    b_shadow = oracle.ray_bytes(ps, "shadow")
+    launches = max(1, st["extend_launches"])
+    bytes_per_launch = b_closest * st["closest_rays"] / launches
+    avg_launch_s = st["extend_ms"] / 1e3 / launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    log("traversal work from %d sampled pixels (%.1f s): %.0f B per closest ray, %.0f B per shadow ray" % (
+        ps["paths"] // max(1, p.samples), t_stats, b_closest, b_shadow))
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "extend_traffic_%s.json" % cfg["workload"])
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tj = json.load(fh)
+        traffic = tj.get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        log("CPU baseline on %d threads ..." % threads)
+        cpu = cpu_baseline(scene, args.cpu_seconds, threads)
+        log("CPU baseline %.4g paths/s (%s)" % (cpu["value"], cpu["sample"]))
+
+    cfg.update({"rng": args.rng, "path_slots": st["path_slots"], "setup_s": {
+        "generate": round(t_gen, 2), "parse_and_bvh": round(t_parse, 2), "upload": round(t_upload, 2)}})
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (generated scene, seeded)",
+        "config": cfg,
+        "roofline": {
+            "bound": "hbm", "kernel": "k_extend", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "bytes_per_closest_ray": b_closest, "launches": launches,
+            "avg_launch_ms": avg_launch_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "kernels_ms": {k: st[k + "_ms"] for k in ("extend", "shade", "shadow", "finalize")},
+        "rays": {"closest": st["closest_rays"], "shadow": st["shadow_rays"], "iterations": st["iterations"]},
+        "image_finite": finite,
+        "device": name,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

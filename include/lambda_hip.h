@@ -211,6 +211,23 @@ void lambda_hip_scene_destroy(lambda_hip_scene *scene);
 int lambda_hip_scene_render(lambda_hip_scene *scene, const lambda_render_params *params,
                             float *out_rgb, volatile int32_t *progress, char *err, size_t err_len);
 
+/* Measurements of one render (no reference counterpart: the reference has
+ * no timers).  Kernel times come from HIP events recorded around every launch
+ * on the render's stream. */
+typedef struct lambda_render_stats {
+    int32_t iterations;      /* bounce-loop passes */
+    int32_t path_slots;      /* paths in flight at the start */
+    int64_t closest_rays;    /* rays traced by the extend kernel */
+    int64_t shadow_rays;     /* NEE rays traced by the shadow kernel */
+    double kernel_ms[4];     /* total device time: extend, shade, shadow, finalize */
+    int64_t launches[4];
+    double render_ms;        /* wall time of the whole call */
+} lambda_render_stats;
+
+/* lambda_hip_scene_render plus per-kernel timing into *stats. */
+int lambda_hip_scene_render_stats(lambda_hip_scene *scene, const lambda_render_params *params,
+                                  float *out_rgb, lambda_render_stats *stats, char *err, size_t err_len);
+
 /* Closest-hit record for one ray (math/intersection.h:8-17). */
 typedef struct lambda_hit {
     int32_t hit;        /* 1 if the ray hit an object */
@@ -242,10 +259,11 @@ int lambda_hip_occluded(lambda_hip_scene *scene, const float *rays, const int32_
 
 /*
  * Thin-film reflectance (scene/material.h:263-309) of material `material` for
- * n queries {cos_theta, lambda_nm, front_facing (0/1), thickness scale}: the
- * layer thicknesses come from the material's scalar textures evaluated at the
- * query point point[i][3].  Computed by the same FP32 device code the
- * integrator uses.  out[i] = reflectance.
+ * n queries[i][4] = {cos_theta, lambda_nm, front_facing (0/1), unused} on a
+ * surface with normal (0,0,1) hit by direction (sin_theta, 0, -cos_theta); the
+ * layer thicknesses come from the material's scalar textures evaluated at
+ * points[i][3].  Computed by the same FP32 device code the integrator uses.
+ * out[i] = reflectance.
  */
 int lambda_hip_thinfilm_reflectance(lambda_hip_scene *scene, int32_t material, const float *queries,
                                     const float *points, int32_t n, float *out,

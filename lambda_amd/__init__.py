@@ -21,7 +21,7 @@ import numpy as np
 from . import _abi
 from ._abi import LAMBDA_RNG_REFERENCE, LAMBDA_RNG_THROUGHPUT, RenderParams, SceneView
 
-__all__ = ["LambdaError", "Scene", "render", "write_ppm", "LAMBDA_RNG_REFERENCE", "LAMBDA_RNG_THROUGHPUT"]
+__all__ = ["LambdaError", "Scene", "DeviceScene", "device_info", "render", "write_ppm", "LAMBDA_RNG_REFERENCE", "LAMBDA_RNG_THROUGHPUT"]
 
 
 class LambdaError(RuntimeError):
@@ -131,3 +131,89 @@ def write_ppm(path: str, image: np.ndarray) -> None:
     img = np.ascontiguousarray(image, np.float32)
     err = C.create_string_buffer(4096)
     _check(_abi.host().lambda_write_ppm(path.encode(), img.shape[1], img.shape[0], _abi.ptr(img, C.c_float), err, 4096), err)
+
+
+class DeviceScene:
+    """A scene uploaded to the GPU (lambda_hip_scene_create); the device-side
+    counterpart of the arrays main.cpp copies to device memory
+    (src/main.cpp:84-103)."""
+
+    def __init__(self, scene: Scene):
+        self._lib = _abi.hip()
+        self._h = C.c_void_p()
+        self.scene = scene
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_scene_create(C.byref(scene.view), C.byref(self._h), err, 4096), err)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.lambda_hip_scene_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, rng_mode: int = LAMBDA_RNG_REFERENCE, seed: int = 42, path_slots: int = 0,
+               progress: np.ndarray | None = None, **overrides) -> np.ndarray:
+        p = self.scene.render_params(rng_mode=rng_mode, seed=seed, path_slots=path_slots, **overrides)
+        out = np.zeros((p.height, p.width, 3), np.float32)
+        err = C.create_string_buffer(4096)
+        prog = None if progress is None else _abi.ptr(progress, C.c_int32)
+        _check(self._lib.lambda_hip_scene_render(self._h, C.byref(p), _abi.ptr(out, C.c_float), prog, err, 4096), err)
+        return out
+
+    def render_stats(self, rng_mode: int = LAMBDA_RNG_REFERENCE, seed: int = 42, path_slots: int = 0,
+                     **overrides):
+        """Renders with per-kernel HIP-event timing; returns (image, stats dict)."""
+        p = self.scene.render_params(rng_mode=rng_mode, seed=seed, path_slots=path_slots, **overrides)
+        out = np.zeros((p.height, p.width, 3), np.float32)
+        st = _abi.RenderStats()
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_scene_render_stats(self._h, C.byref(p), _abi.ptr(out, C.c_float), C.byref(st),
+                                                       err, 4096), err)
+        stats = {"iterations": st.iterations, "path_slots": st.path_slots, "closest_rays": st.closest_rays,
+                 "shadow_rays": st.shadow_rays, "render_ms": st.render_ms}
+        for k, name in enumerate(_abi.KERNELS):
+            stats[name + "_ms"] = st.kernel_ms[k]
+            stats[name + "_launches"] = st.launches[k]
+        return out, stats
+
+    def closest_hit(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros(len(rays), _abi.HIT_DTYPE)
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_closest_hit(self._h, _abi.ptr(rays, C.c_float), len(rays),
+                                                out.ctypes.data_as(C.POINTER(_abi.Hit)), err, 4096), err)
+        return out
+
+    def occluded(self, rays: np.ndarray, light_instance: np.ndarray, light_object: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32)
+        li = np.ascontiguousarray(light_instance, np.int32)
+        lo = np.ascontiguousarray(light_object, np.int32)
+        out = np.zeros(len(rays), np.int32)
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_occluded(self._h, _abi.ptr(rays, C.c_float), _abi.ptr(li, C.c_int32),
+                                             _abi.ptr(lo, C.c_int32), len(rays), _abi.ptr(out, C.c_int32),
+                                             err, 4096), err)
+        return out
+
+    def thinfilm(self, material: int, queries: np.ndarray, points: np.ndarray) -> np.ndarray:
+        q = np.ascontiguousarray(queries, np.float32)
+        pts = np.ascontiguousarray(points, np.float32)
+        out = np.zeros(len(q), np.float32)
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_thinfilm_reflectance(self._h, material, _abi.ptr(q, C.c_float),
+                                                         _abi.ptr(pts, C.c_float), len(q),
+                                                         _abi.ptr(out, C.c_float), err, 4096), err)
+        return out
+
+
+def device_info() -> tuple[str, int]:
+    name = C.create_string_buffer(256)
+    cu = C.c_int32(0)
+    err = C.create_string_buffer(4096)
+    _check(_abi.hip().lambda_hip_device_info(name, 256, C.byref(cu), err, 4096), err)
+    return name.value.decode(), cu.value

@@ -58,6 +58,17 @@ class Hit(C.Structure):
     ]
 
 
+class RenderStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32), ("path_slots", C.c_int32), ("closest_rays", C.c_int64),
+        ("shadow_rays", C.c_int64), ("kernel_ms", C.c_double * 4), ("launches", C.c_int64 * 4),
+        ("render_ms", C.c_double),
+    ]
+
+
+KERNELS = ("extend", "shade", "shadow", "finalize")
+
+
 HIT_DTYPE = np.dtype([
     ("hit", np.int32), ("instance", np.int32), ("object", np.int32), ("t", np.float32),
     ("point", np.float32, 3), ("normal", np.float32, 3), ("u", np.float32), ("v", np.float32),
@@ -109,6 +120,7 @@ def hip():
         lib.lambda_hip_scene_destroy.argtypes = [vp]
         lib.lambda_hip_scene_destroy.restype = None
         lib.lambda_hip_scene_render.argtypes = [vp, rp, _f32p, _i32p, C.c_char_p, C.c_size_t]
+        lib.lambda_hip_scene_render_stats.argtypes = [vp, rp, _f32p, C.POINTER(RenderStats), C.c_char_p, C.c_size_t]
         lib.lambda_hip_closest_hit.argtypes = [vp, _f32p, C.c_int32, C.POINTER(Hit), C.c_char_p, C.c_size_t]
         lib.lambda_hip_occluded.argtypes = [vp, _f32p, _i32p, _i32p, C.c_int32, _i32p, C.c_char_p, C.c_size_t]
         lib.lambda_hip_thinfilm_reflectance.argtypes = [vp, C.c_int32, _f32p, _f32p, C.c_int32, _f32p,

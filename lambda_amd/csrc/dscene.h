@@ -1,0 +1,601 @@
+// Device scene: layout in HBM, ray/primitive tests, two-level BVH traversal,
+// textures, BSDFs, thin-film transfer matrices and light sampling.
+//
+// Layout (all arrays 16-byte records, gathered by lane):
+//   node_lo[n] = {min.xyz, bits(index)}, node_hi[n] = {max.xyz, bits(count)}
+//     one node = two 16-byte loads; count > 0 marks a leaf (scene/bvh.h:14-28)
+//   obj_a[n] = {corner | centre, radius}, obj_b[n] = {horizontal, bits(type)},
+//   obj_c[n] = {vertical, 0}, obj_n[n] = {cross(horizontal, vertical), 0}
+//     the intersection test reads a (spheres) or a, b, c (quads, triangles);
+//     obj_n and the triangle shading record obj_s[n][4] are read once per
+//     closest hit
+//   inst[n][8] = {M rows 0-3, M^-1 rows 0-3}, inst_i[n] = {object, count,
+//     BLAS root, material}
+//   spectra are 471-bin float tables, complex spectra 471-bin float2 tables.
+#pragma once
+
+#include "dmath.h"
+
+namespace lam {
+
+struct DeviceScene {
+    const float4 *node_lo, *node_hi;
+    const float4 *obj_a, *obj_b, *obj_c, *obj_n, *obj_s;
+    const int *obj_mat;
+    const float4 *inst;
+    const int4 *inst_i;
+    const int4 *mat;        // {type, p0, p1, p2}
+    const float4 *stex;     // {bits(type), p0, p1, p2}
+    const int4 *xtex;       // {type, a, b, bits(scale)}
+    const float *spectra;   // [S][471]
+    const float2 *cspectra; // [C][471]
+    const int *props;
+    const int2 *light_io;   // {instance, object}
+    const float *light_pow;
+    const float *cie;       // [471][3]
+    const int *perm;        // [256]
+    float4 cam_pos, cam_corner, cam_h, cam_v;
+    float total_power;
+    int num_lights, bg_tex;
+    int num_nodes, num_objects, num_instances;
+};
+
+// ------------------------------------------------------------------ records
+
+struct Hit {
+    int inst, obj;
+    float t;
+    V3 p, n;
+    float u, v;
+    bool front;
+};
+
+__device__ __forceinline__ float dense(const float *s, float lambda) {
+    // 1 nm table, linear interpolation (math/spectrum.h:170-188) in float.
+    float index = lambda - 360;
+    int lo = int(index);
+    if (lo >= kBins - 1) return s[kBins - 1];
+    float a = s[lo], b = s[lo + 1];
+    return a + (index - float(lo)) * (b - a);
+}
+
+__device__ __forceinline__ float2 dense_c(const float2 *s, float lambda) {
+    float index = lambda - 360;
+    int lo = int(index);
+    if (lo >= kBins - 1) return s[kBins - 1];
+    float2 a = s[lo], b = s[lo + 1];
+    float w = index - float(lo);
+    return make_float2(a.x + w * (b.x - a.x), a.y + w * (b.y - a.y));
+}
+
+// ---------------------------------------------------------------- geometry
+// Ray/primitive tests (scene/object.h:301-400).  `best` is the current
+// closest distance; a hit must lie in [1e-4, best).  These return the
+// candidate distance only; hit_record() rebuilds the full record for the
+// winning primitive with the same arithmetic.
+
+__device__ __forceinline__ bool test_sphere(float4 a, V3 o, V3 d, float best, float &t_out) {
+    V3 oc = o - v3(a);
+    float A = len2(d);
+    float B = 2 * dot(d, oc);
+    float C = dot(oc, oc) - a.w * a.w;
+    float D = B * B - 4 * A * C;
+    if (D < 0) return false;
+    D = sqrtf(D);
+    float t1 = (-B - D) / (2 * A);
+    float t2 = (-B + D) / (2 * A);
+    if (t1 < kEps && t2 < kEps) return false;
+    float t = (t1 < kEps) ? t2 : t1;
+    if (t >= best) return false;
+    t_out = t;
+    return true;
+}
+
+__device__ __forceinline__ bool test_quad(V3 c, V3 h, V3 v, V3 nrm, V3 o, V3 d, float best, float &t_out,
+                                          float &alpha, float &beta) {
+    V3 n = normalize(nrm);
+    float den = dot(n, d);
+    if (fabsf(den) < kEps) return false;
+    float t = dot(n, c - o) / den;
+    if (t < kEps || t >= best) return false;
+    V3 p = o + t * d;
+    V3 proj = p - c;
+    alpha = dot(nrm, cross(proj, v)) / len2(nrm);
+    beta = dot(nrm, cross(h, proj)) / len2(nrm);
+    if (alpha < 0 || alpha > 1 || beta < 0 || beta > 1) return false;
+    t_out = t;
+    return true;
+}
+
+__device__ __forceinline__ bool test_tri(V3 c, V3 h, V3 v, V3 o, V3 d, float best, float &t_out, float &alpha,
+                                         float &beta) {
+    V3 rcv = cross(d, v);
+    float det = dot(h, rcv);
+    if (fabsf(det) < kEps2) return false;
+    float inv = 1 / det;
+    V3 oc = o - c;
+    alpha = inv * dot(oc, rcv);
+    if (alpha < 0 || alpha > 1) return false;
+    V3 och = cross(oc, h);
+    beta = inv * dot(d, och);
+    if (beta < 0 || alpha + beta > 1) return false;
+    float t = inv * dot(v, och);
+    if (t < kEps || t >= best) return false;
+    t_out = t;
+    return true;
+}
+
+// Test of object k against the (local) ray; updates best on a closer hit.
+__device__ __forceinline__ bool test_object(const DeviceScene &S, int k, V3 o, V3 d, float &best) {
+    float4 a = S.obj_a[k];
+    float4 b = S.obj_b[k];
+    int type = __float_as_int(b.w);
+    float t, al, be;
+    if (type == LAMBDA_OBJECT_SPHERE) {
+        if (!test_sphere(a, o, d, best, t)) return false;
+    } else {
+        float4 c = S.obj_c[k];
+        if (type == LAMBDA_OBJECT_TRI) {
+            if (!test_tri(v3(a), v3(b), v3(c), o, d, best, t, al, be)) return false;
+        } else {
+            float4 n = S.obj_n[k];
+            if (!test_quad(v3(a), v3(b), v3(c), v3(n), o, d, best, t, al, be)) return false;
+        }
+    }
+    best = t;
+    return true;
+}
+
+// Full intersection record of object k for a local-space ray, as the
+// reference's intersect* fills it (scene/object.h:301-400).
+__device__ __forceinline__ bool object_record(const DeviceScene &S, int k, V3 o, V3 d, float best, Hit &h) {
+    float4 a = S.obj_a[k];
+    float4 b = S.obj_b[k];
+    int type = __float_as_int(b.w);
+    float t, al = 0, be = 0;
+    if (type == LAMBDA_OBJECT_SPHERE) {
+        if (!test_sphere(a, o, d, best, t)) return false;
+        h.t = t;
+        h.p = o + t * d;
+        h.n = normalize(h.p - v3(a));
+        h.u = (atan2f(-h.n.z, h.n.x) + kPi) / (2 * kPi);
+        h.v = acosf(-h.n.y) / kPi;
+        h.front = dot(d, h.n) < 0;
+    } else {
+        float4 c = S.obj_c[k];
+        float4 nn = S.obj_n[k];
+        V3 nrm = v3(nn);
+        if (type == LAMBDA_OBJECT_TRI) {
+            if (!test_tri(v3(a), v3(b), v3(c), o, d, best, t, al, be)) return false;
+            const float4 *sh = S.obj_s + 4 * size_t(k);
+            float4 s0 = sh[0], s1 = sh[1], s2 = sh[2], s3 = sh[3];
+            V3 n0 = v3(s0.x, s0.y, s0.z), n1 = v3(s0.w, s1.x, s1.y), n2 = v3(s1.z, s1.w, s2.x);
+            float w = 1 - al - be;
+            h.t = t;
+            h.p = o + t * d;
+            h.n = normalize(w * n0 + al * n1 + be * n2);
+            h.u = w * s2.y + al * s2.z + be * s2.w;
+            h.v = w * s3.x + al * s3.y + be * s3.z;
+            h.front = dot(d, nrm) < 0;
+        } else {
+            if (!test_quad(v3(a), v3(b), v3(c), nrm, o, d, best, t, al, be)) return false;
+            h.t = t;
+            h.p = o + t * d;
+            h.n = normalize(nrm);
+            h.u = al;
+            h.v = be;
+            h.front = dot(d, h.n) < 0;
+        }
+    }
+    if (!h.front) h.n = h.n * -1;
+    h.obj = k;
+    return true;
+}
+
+__device__ __forceinline__ void local_ray(const DeviceScene &S, int inst, V3 o, V3 d, V3 &lo, V3 &ld) {
+    const float4 *m = S.inst + 8 * size_t(inst);
+    lo = xform(m + 4, o, 1);
+    ld = xform(m + 4, d, 0);
+}
+
+// World-space record of (instance, object) for a world ray: local test, then
+// point by M and normal by the transposed inverse (scene/scene.h:101-104).
+__device__ __forceinline__ bool hit_record(const DeviceScene &S, int inst, int obj, V3 o, V3 d, Hit &h) {
+    V3 lo, ld;
+    local_ray(S, inst, o, d, lo, ld);
+    if (!object_record(S, obj, lo, ld, kMax, h)) return false;
+    const float4 *m = S.inst + 8 * size_t(inst);
+    h.inst = inst;
+    h.p = xform(m, h.p, 1);
+    h.n = normalize(xform_t(m + 4, h.n, 0));
+    return true;
+}
+
+// ------------------------------------------------------------- traversal
+//
+// Depth-first traversal of the instance-level BVH and, per instance leaf, of
+// the instance's object-level BVH (scene/scene.h:51-124).  Children are
+// pushed as (left, right) and popped right first, the reference's order, so
+// equal-distance ties resolve to the same primitive.  A node is skipped when
+// its entry distance exceeds the current closest hit.  Only the first
+// instance of an instance-level leaf is entered (scene/scene.h:79-86).
+//
+// The two stacks live in LDS: entry k of lane `lane` is at stk[k * stride +
+// lane] for the instance level and stk[(kStack + k) * stride + lane] for the
+// object level, so a wave's pushes and pops hit 64 distinct banks.
+
+__device__ __forceinline__ bool slab(float4 lo, float4 hi, V3 o, V3 inv, float &tnear) {
+    float t1 = 0, t2 = kMax;
+    float a = (lo.x - o.x) * inv.x, b = (hi.x - o.x) * inv.x;
+    if (inv.x < 0) { float s = a; a = b; b = s; }
+    t1 = fmaxf(a, t1);
+    t2 = fminf(b, t2);
+    a = (lo.y - o.y) * inv.y;
+    b = (hi.y - o.y) * inv.y;
+    if (inv.y < 0) { float s = a; a = b; b = s; }
+    t1 = fmaxf(a, t1);
+    t2 = fminf(b, t2);
+    a = (lo.z - o.z) * inv.z;
+    b = (hi.z - o.z) * inv.z;
+    if (inv.z < 0) { float s = a; a = b; b = s; }
+    t1 = fmaxf(a, t1);
+    t2 = fminf(b, t2);
+    tnear = t1;
+    return t1 <= t2 && t2 >= 0;
+}
+
+struct TraceStats {
+    uint32_t nodes, instances, prims;
+};
+
+// kAny = false: closest hit; returns true if anything was hit, with
+//   (hit_inst, hit_obj, best).
+// kAny = true: occlusion; best is the light distance, (skip_inst, skip_obj)
+//   the light itself; returns true on the first blocking hit.
+template <bool kAny, bool kStats>
+__device__ __forceinline__ bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_inst,
+                                         int &hit_obj, int skip_inst, int skip_obj, int *stk, int stride,
+                                         TraceStats &st) {
+    const V3 inv = v3(1 / d.x, 1 / d.y, 1 / d.z);
+    int *top = stk;
+    int *sub = stk + kStack * stride;
+    int sp = 0;
+    top[0] = 0;
+    sp = 1;
+    bool found = false;
+    while (sp > 0) {
+        int cur = top[--sp * stride];
+        float4 lo = S.node_lo[cur];
+        float4 hi = S.node_hi[cur];
+        if (kStats) st.nodes++;
+        float tn;
+        if (!slab(lo, hi, o, inv, tn) || tn > best) continue;
+        int count = __float_as_int(hi.w);
+        int index = __float_as_int(lo.w);
+        if (count > 0) {
+            const int ii = index;
+            if (kStats) st.instances++;
+            V3 lo_o, lo_d;
+            local_ray(S, ii, o, d, lo_o, lo_d);
+            const V3 linv = v3(1 / lo_d.x, 1 / lo_d.y, 1 / lo_d.z);
+            int ssp = 1;
+            sub[0] = S.inst_i[ii].z;
+            while (ssp > 0) {
+                int sc = sub[--ssp * stride];
+                float4 slo = S.node_lo[sc];
+                float4 shi = S.node_hi[sc];
+                if (kStats) st.nodes++;
+                float stn;
+                if (!slab(slo, shi, lo_o, linv, stn) || stn > best) continue;
+                int scount = __float_as_int(shi.w);
+                int sindex = __float_as_int(slo.w);
+                if (scount > 0) {
+                    for (int k = sindex; k < sindex + scount; k++) {
+                        if (kAny && ii == skip_inst && k == skip_obj) continue;
+                        if (kStats) st.prims++;
+                        if (test_object(S, k, lo_o, lo_d, best)) {
+                            if (kAny) return true;
+                            hit_inst = ii;
+                            hit_obj = k;
+                            found = true;
+                        }
+                    }
+                } else {
+                    sub[ssp * stride] = sc + 1;
+                    sub[(ssp + 1) * stride] = sindex;
+                    ssp += 2;
+                }
+            }
+        } else {
+            top[sp * stride] = cur + 1;
+            top[(sp + 1) * stride] = index;
+            sp += 2;
+        }
+    }
+    return found;
+}
+
+// --------------------------------------------------------------- textures
+
+__device__ __forceinline__ float scalar_texture(const DeviceScene &S, int t, V3 p) {
+    float4 x = S.stex[t];
+    int type = __float_as_int(x.x);
+    if (type == LAMBDA_SCALAR_CONSTANT) return x.y;
+    if (type == LAMBDA_SCALAR_PERLIN) return x.y + (x.z - x.y) * perlin(S.perm, p * x.w);
+    return x.y + (x.z - x.y) * worley(p * x.w);
+}
+
+// Spectrum texture at one wavelength (scene/texture.h:178-210).  Checker
+// cells are in (u, v) units.
+__device__ __forceinline__ float spectrum_texture(const DeviceScene &S, int t, const Hit &h, float lambda) {
+    int4 x = S.xtex[t];
+    if (x.x == LAMBDA_SPECTRUM_CONSTANT) return dense(S.spectra + size_t(x.y) * kBins, lambda);
+    if (x.x == LAMBDA_SPECTRUM_CHECKER) {
+        float scale = __int_as_float(x.w);
+        int cx = int(floorf(h.u / scale));
+        int cy = int(floorf(h.v / scale));
+        int s = (cx + cy) % 2 == 0 ? x.y : x.z;
+        return dense(S.spectra + size_t(s) * kBins, lambda);
+    }
+    return scalar_texture(S, x.y, h.p);
+}
+
+// Texture that does not depend on the wavelength: evaluated once per vertex.
+__device__ __forceinline__ bool spectrum_texture_is_scalar(const DeviceScene &S, int t) {
+    return S.xtex[t].x == LAMBDA_SPECTRUM_SCALAR;
+}
+
+__device__ __forceinline__ S4 spectrum_texture4(const DeviceScene &S, int t, const Hit &h, const S4 &lam) {
+    S4 r;
+    if (spectrum_texture_is_scalar(S, t)) {
+        float v = scalar_texture(S, S.xtex[t].y, h.p);
+        return s4(v);
+    }
+#pragma unroll
+    for (int j = 0; j < kHero; j++) r.v[j] = spectrum_texture(S, t, h, lam.v[j]);
+    return r;
+}
+
+// Environment lookup by direction: equirectangular (u, v), point = direction
+// (scene/background.h:81-95).
+__device__ __forceinline__ S4 background(const DeviceScene &S, V3 d, const S4 &lam) {
+    Hit h;
+    h.p = d;
+    h.u = (atan2f(d.z, d.x) + kPi) / (2 * kPi);
+    h.v = acosf(d.y) / kPi;
+    return spectrum_texture4(S, S.bg_tex, h, lam);
+}
+
+__device__ __forceinline__ float spectrum_average(const DeviceScene &S, int s) {
+    const float *p = S.spectra + size_t(s) * kBins;
+    float sum = 0;
+    for (int i = 0; i < kBins; i++) sum += p[i];
+    return sum / kBins;
+}
+
+// Mean emitted radiance of a spectrum texture (scene/texture.h:154-176).
+__device__ __forceinline__ float spectrum_texture_average(const DeviceScene &S, int t) {
+    int4 x = S.xtex[t];
+    if (x.x == LAMBDA_SPECTRUM_CONSTANT) return spectrum_average(S, x.y);
+    if (x.x == LAMBDA_SPECTRUM_CHECKER) return float(0.5) * (spectrum_average(S, x.y) + spectrum_average(S, x.z));
+    float4 st = S.stex[x.y];
+    return __float_as_int(st.x) == LAMBDA_SCALAR_CONSTANT ? st.y : (st.y + st.z) / 2;
+}
+
+// ---------------------------------------------------------------- objects
+// Light sampling on objects (scene/object.h:201-299) and instances
+// (scene/instance.h:103-111).
+
+__device__ __forceinline__ float object_area(const DeviceScene &S, int k) {
+    float4 a = S.obj_a[k];
+    int type = __float_as_int(S.obj_b[k].w);
+    if (type == LAMBDA_OBJECT_SPHERE) return 4 * kPi * a.w * a.w;
+    float l = len(v3(S.obj_n[k]));
+    return type == LAMBDA_OBJECT_QUAD ? l : l / 2;
+}
+
+__device__ __forceinline__ float object_pdf(const DeviceScene &S, int k, V3 p, V3 d) {
+    float4 a = S.obj_a[k];
+    float4 b = S.obj_b[k];
+    int type = __float_as_int(b.w);
+    if (type == LAMBDA_OBJECT_SPHERE) {
+        V3 oc = v3(a) - p;
+        float dist2 = len2(oc);
+        float r2 = a.w * a.w;
+        if (dist2 <= r2) return 1 / (4 * kPi);
+        float cmax = sqrtf(1 - r2 / dist2);
+        float ct = dot(normalize(oc), d);
+        if (ct < cmax) return 0;
+        return 1 / (2 * kPi * (1 - cmax));
+    }
+    V3 c = v3(a), h = v3(b), v = v3(S.obj_c[k]), n = v3(S.obj_n[k]);
+    if (type == LAMBDA_OBJECT_QUAD) {
+        float ct = dot(normalize(n), d);
+        if (fabsf(ct) < kEps) return 0;
+        float t = dot(c - p, normalize(n)) / ct;
+        if (t < kEps) return 0;
+        V3 local = (p + d * t) - c;
+        float u = dot(n, cross(local, v)) / len2(n);
+        float w = dot(n, cross(h, local)) / len2(n);
+        if (u < 0 || u > 1 || w < 0 || w > 1) return 0;
+        return t * t / (fabsf(ct) * len(n));
+    }
+    V3 rcv = cross(d, v);
+    float det = dot(h, rcv);
+    if (fabsf(det) < kEps2) return 0;
+    float inv = 1 / det;
+    V3 oc = p - c;
+    float al = inv * dot(oc, rcv);
+    if (al < 0 || al > 1) return 0;
+    V3 och = cross(oc, h);
+    float be = inv * dot(d, och);
+    if (be < 0 || al + be > 1) return 0;
+    float t = inv * dot(v, och);
+    if (t < kEps) return 0;
+    float area = len(n) * 0.5f;
+    float ct = dot(normalize(n), d);
+    return t * t / (fabsf(ct) * area);
+}
+
+__device__ __forceinline__ V3 object_sample(const DeviceScene &S, int k, V3 p, Pcg &rng) {
+    float4 a = S.obj_a[k];
+    float4 b = S.obj_b[k];
+    int type = __float_as_int(b.w);
+    if (type == LAMBDA_OBJECT_SPHERE) {
+        V3 oc = v3(a) - p;
+        float dist2 = len2(oc);
+        float r2 = a.w * a.w;
+        if (dist2 <= r2) return random_unit_vector(rng);
+        float cmax = sqrtf(1 - r2 / dist2);
+        return random_in_cone(oc, cmax, rng);
+    }
+    float u = rng.next();
+    float v = rng.next();
+    if (type == LAMBDA_OBJECT_TRI && u + v > 1) {
+        u = 1 - u;
+        v = 1 - v;
+    }
+    V3 s = v3(a) + v3(b) * u + v3(S.obj_c[k]) * v;
+    return normalize(s - p);
+}
+
+__device__ __forceinline__ int material_of(const DeviceScene &S, int inst, int obj) {
+    int m = S.inst_i[inst].w;
+    return m >= 0 ? m : S.obj_mat[obj];
+}
+
+__device__ __forceinline__ float instance_area(const DeviceScene &S, int inst, int obj) {
+    const float4 *m = S.inst + 8 * size_t(inst);
+    return object_area(S, obj) * len2(v3(m[0].x, m[1].x, m[2].x));
+}
+
+__device__ __forceinline__ float instance_pdf(const DeviceScene &S, int inst, int obj, V3 p, V3 d) {
+    const float4 *m = S.inst + 8 * size_t(inst);
+    return object_pdf(S, obj, xform(m + 4, p, 1), normalize(xform(m + 4, d, 0)));
+}
+
+__device__ __forceinline__ V3 instance_sample(const DeviceScene &S, int inst, int obj, V3 p, Pcg &rng) {
+    const float4 *m = S.inst + 8 * size_t(inst);
+    return normalize(xform(m, object_sample(S, obj, xform(m + 4, p, 1), rng), 0));
+}
+
+// --------------------------------------------------------- thin film (FP32)
+//
+// Transfer-matrix reflectance of a layer stack (scene/material.h:263-309,
+// core/utils.h:173-193) in single-precision complex arithmetic: interface
+// matrices D = (1/t) [[1, r], [r, 1]] for s and p polarisation, propagation
+// P = diag(exp(-i phi), exp(i phi)) with phi = n (2 pi / lambda) d cos theta,
+// M = D_0 P_1 D_1 ... ; R = (|M01/M00|^2_s + |M01/M00|^2_p) / 2.
+
+struct Cf {
+    float re, im;
+};
+__device__ __forceinline__ Cf cf(float r, float i = 0) { return Cf{r, i}; }
+__device__ __forceinline__ Cf operator+(Cf a, Cf b) { return cf(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ Cf operator-(Cf a, Cf b) { return cf(a.re - b.re, a.im - b.im); }
+__device__ __forceinline__ Cf operator*(Cf a, Cf b) { return cf(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+__device__ __forceinline__ Cf operator*(Cf a, float s) { return cf(a.re * s, a.im * s); }
+// Scaled division (Smith): no overflow for large or small |b|.
+__device__ __forceinline__ Cf operator/(Cf a, Cf b) {
+    if (fabsf(b.re) >= fabsf(b.im)) {
+        float r = b.im / b.re;
+        float den = b.re + b.im * r;
+        return cf((a.re + a.im * r) / den, (a.im - a.re * r) / den);
+    }
+    float r = b.re / b.im;
+    float den = b.re * r + b.im;
+    return cf((a.re * r + a.im) / den, (a.im * r - a.re) / den);
+}
+__device__ __forceinline__ float cabs2(Cf a) { return a.re * a.re + a.im * a.im; }
+// Principal square root; the sign of a zero imaginary part selects the
+// branch on the negative real axis, as C99 csqrt does.
+__device__ __forceinline__ Cf csqrt(Cf z) {
+    float m = hypotf(z.re, z.im);
+    if (m == 0) return cf(0, z.im);
+    if (z.re >= 0) {
+        float t = sqrtf(0.5f * (m + z.re));
+        return cf(t, z.im / (2 * t));
+    }
+    float t = sqrtf(0.5f * (m - z.re));
+    return cf(fabsf(z.im) / (2 * t), copysignf(t, z.im));
+}
+__device__ __forceinline__ Cf cexpf(Cf z) {
+    float s, c;
+    sincosf(z.im, &s, &c);
+    float e = expf(z.re);
+    return cf(e * c, e * s);
+}
+
+struct M2 {
+    Cf a, b, c, d;  // [[a, b], [c, d]]
+};
+__device__ __forceinline__ M2 mul(const M2 &x, const M2 &y) {
+    return M2{x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d, x.c * y.a + x.d * y.c, x.c * y.b + x.d * y.d};
+}
+__device__ __forceinline__ M2 interface_s(Cf n1, Cf n2, Cf c1, Cf c2) {
+    Cf a = n1 * c1, b = n2 * c2;
+    Cf r = (a - b) / (a + b);
+    Cf t = (cf(2) * a) / (a + b);
+    Cf one = cf(1) / t;
+    Cf rt = r / t;
+    return M2{one, rt, rt, one};
+}
+__device__ __forceinline__ M2 interface_p(Cf n1, Cf n2, Cf c1, Cf c2) {
+    Cf a = n2 * c1, b = n1 * c2;
+    Cf r = (a - b) / (a + b);
+    Cf t = (cf(2) * (n1 * c1)) / (a + b);
+    Cf one = cf(1) / t;
+    Cf rt = r / t;
+    return M2{one, rt, rt, one};
+}
+
+// Reflectance of thin-film material `mat` at wavelength lambda for a ray of
+// direction d hitting a surface with normal n (facing the ray), at point p
+// (thickness textures are evaluated there).  thick[] carries the layer
+// thicknesses when the caller has already evaluated them.
+__device__ __forceinline__ float thinfilm_reflectance(const DeviceScene &S, int4 mt, bool front, V3 n, V3 d,
+                                                      float lambda, const float *thick) {
+    const int layers = mt.y, n_off = mt.z;
+    const int j_off = front ? 0 : layers + 1;
+    const int j_sign = front ? 1 : -1;
+    auto nidx = [&](int j) { return S.props[n_off + j_off + j_sign * j]; };
+    auto ridx = [&](int j) {
+        float2 v = dense_c(S.cspectra + size_t(nidx(j)) * kBins, lambda);
+        return cf(v.x, v.y);
+    };
+    Cf n0 = ridx(0), n1 = ridx(1);
+    Cf cos_cur = cf(dot(n, -d));
+    Cf sin_next = n0 / n1;
+    sin_next = sin_next * (sin_next * (cf(1) - cos_cur * cos_cur));
+    Cf cos_next = csqrt(cf(1) - sin_next);
+    M2 ms = interface_s(n0, n1, cos_cur, cos_next);
+    M2 mp = interface_p(n0, n1, cos_cur, cos_next);
+    const float k = 2 * kPi / lambda;
+    for (int j = 0; j < layers; j++) {
+        n0 = ridx(j + 1);
+        n1 = ridx(j + 2);
+        cos_cur = cos_next;
+        Cf sin_cur = sin_next;
+        sin_next = n0 / n1;
+        sin_next = sin_next * (sin_next * sin_cur);
+        cos_next = csqrt(cf(1) - sin_next);
+        M2 is = interface_s(n0, n1, cos_cur, cos_next);
+        M2 ip = interface_p(n0, n1, cos_cur, cos_next);
+        float dj = thick[front ? j : layers - 1 - j];
+        Cf phi = ((n0 * k) * dj) * cos_cur;
+        // diag(exp(-i phi), exp(i phi)) times the interface matrix
+        Cf em = cexpf(cf(phi.im, -phi.re));
+        Cf ep = cexpf(cf(-phi.im, phi.re));
+        M2 pi_s{em * is.a, em * is.b, ep * is.c, ep * is.d};
+        M2 pi_p{em * ip.a, em * ip.b, ep * ip.c, ep * ip.d};
+        ms = mul(ms, pi_s);
+        mp = mul(mp, pi_p);
+    }
+    float rs = cabs2(ms.b / ms.a);
+    float rp = cabs2(mp.b / mp.a);
+    return (rs + rp) * 0.5f;
+}
+
+constexpr int kMaxLayers = 8;
+
+}  // namespace lam

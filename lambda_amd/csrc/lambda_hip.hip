@@ -1,0 +1,1230 @@
+// MI355X (gfx950) wavefront spectral path tracer behind the lambda_hip.h ABI.
+//
+// One bounce of every in-flight path is four kernels over a compacted queue
+// of path slots:
+//   extend   closest-hit traversal of the two-level BVH      (scene/scene.h:51-124)
+//   shade    miss / emission / next-event estimation / BSDF sampling /
+//            Russian roulette                                 (render/renderer.h:102-192)
+//   shadow   occlusion traversal of the NEE rays queued by shade
+//   finalize finished samples -> FP64 XYZ framebuffer, new camera samples
+//            for their slots, next queue                      (render/renderer.h:64-92)
+// Queues are compacted with wave-wide ballots: one atomic per wave per queue.
+// Path state lives in structure-of-arrays float4 records so every access of
+// a queue pass is a coalesced 16-byte load or store per lane.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lambda_hip.h"
+#include "dscene.h"
+
+namespace lam {
+#include "spectral_tables.inc"
+}
+
+using namespace lam;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------------ errors
+
+struct Failure {
+    int code;
+    std::string msg;
+};
+
+void set_err(char *err, size_t len, const std::string &msg) {
+    if (!err || len == 0) return;
+    size_t n = std::min(msg.size(), len - 1);
+    std::memcpy(err, msg.data(), n);
+    err[n] = 0;
+}
+
+#define HIP_OK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw Failure{e_ == hipErrorOutOfMemory ? LAMBDA_ENOMEM : LAMBDA_EHIP,            \
+                          std::string(#call) + ": " + hipGetErrorString(e_)};                 \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return;
+        HIP_OK(hipMalloc(&p, count * sizeof(T)));
+    }
+    void upload(const T *src, size_t count) {
+        alloc(count);
+        if (count) HIP_OK(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void upload(const std::vector<T> &v) { upload(v.data(), v.size()); }
+};
+
+// --------------------------------------------------------- wave utilities
+
+// Wave-aggregated slot reservation: one atomicAdd per wave; every lane of
+// the wave must call it (pred false for lanes with nothing to push).
+__device__ __forceinline__ int wave_push(int *counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0) return -1;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    return pred ? base + rank : -1;
+}
+
+// --------------------------------------------------------------- state
+
+enum : int {
+    kFlagSpecular = 1,   // previous vertex was specular (or the camera)
+    kFlagFinished = 2,   // the sample's path has ended
+};
+
+enum : int {
+    kCtrNext = 0,        // next active queue length
+    kCtrShadow = 1,      // shadow queue length
+    kCtrWork = 2,        // next work item (throughput mode)
+    kCtrPixels = 3,      // finished pixels (reference mode) / samples (throughput)
+    kCtrError = 4,       // error bits
+    kCtrCount = 8
+};
+
+struct Paths {
+    float4 *o;        // origin, previous scatter pdf
+    float4 *d;        // direction
+    float4 *lam;      // hero wavelengths
+    float4 *beta;     // throughput
+    float4 *L;        // radiance
+    uint4 *rng;       // PCG32 state and increment (lo, hi, lo, hi)
+    int4 *info;       // pixel, sample index, bounce, flags
+    int2 *hit;        // instance, object (-1 on a miss)
+    int *q_in, *q_out;
+    float4 *s_o, *s_d, *s_c;  // shadow: origin+tmax, direction, contribution
+    int4 *s_i;                // shadow: slot, light instance, light object
+    int *ctr;
+    double *fb;               // XYZ sums, [pixel][3]
+    const int *order;         // pixel visiting order (8x8 tiles)
+};
+
+struct Frame {
+    int width, height, depth, spp, sqrt_spp, mode, npix;
+    float lmin, lmax, lrange;
+    uint64_t seed;
+    long long total_work;     // npix * spp (throughput mode)
+};
+
+// New camera sample `sample` of `pixel` into slot s (render/renderer.h:71-82,
+// render/camera.h:13-16).  Consumes three draws.
+__device__ __forceinline__ void camera_sample(const DeviceScene &S, const Frame &F, const Paths &P, int s, int pixel,
+                                              int sample, Pcg &rng) {
+    const int px = pixel % F.width, py = pixel / F.width;
+    const int i = sample / F.sqrt_spp, j = sample % F.sqrt_spp;
+    float u = (float(px) + (float(i) + rng.next()) / float(F.sqrt_spp)) / float(F.width);
+    float v = (float(py) + (float(j) + rng.next()) / float(F.sqrt_spp)) / float(F.height);
+    float lambda = F.lmin + rng.next() * (F.lmax - F.lmin);
+    float4 lam;
+    lam.x = F.lmin + fmodf(lambda - F.lmin + float(0) * F.lrange / kHero, F.lrange);
+    lam.y = F.lmin + fmodf(lambda - F.lmin + float(1) * F.lrange / kHero, F.lrange);
+    lam.z = F.lmin + fmodf(lambda - F.lmin + float(2) * F.lrange / kHero, F.lrange);
+    lam.w = F.lmin + fmodf(lambda - F.lmin + float(3) * F.lrange / kHero, F.lrange);
+    V3 pos = v3(S.cam_pos), corner = v3(S.cam_corner), ch = v3(S.cam_h), cv = v3(S.cam_v);
+    V3 dir = normalize(corner + ch * u + cv * v - pos);
+    P.o[s] = make_float4(pos.x, pos.y, pos.z, 1.0f);
+    P.d[s] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    P.lam[s] = lam;
+    P.beta[s] = make_float4(1, 1, 1, 1);
+    P.L[s] = make_float4(0, 0, 0, 0);
+    P.info[s] = make_int4(pixel, sample, 0, kFlagSpecular);
+}
+
+__device__ __forceinline__ void store_rng(const Paths &P, int s, const Pcg &r) {
+    P.rng[s] = make_uint4(uint32_t(r.state), uint32_t(r.state >> 32), uint32_t(r.inc), uint32_t(r.inc >> 32));
+}
+__device__ __forceinline__ Pcg load_rng(const Paths &P, int s) {
+    uint4 v = P.rng[s];
+    Pcg r;
+    r.state = uint64_t(v.x) | (uint64_t(v.y) << 32);
+    r.inc = uint64_t(v.z) | (uint64_t(v.w) << 32);
+    return r;
+}
+
+// ---------------------------------------------------------------- kernels
+
+// First sample of every slot.  Reference mode: slot s owns the pixel of
+// rank s for all its samples.  Throughput mode: slot s takes work item s.
+__global__ void __launch_bounds__(kBlock) k_start(DeviceScene S, Frame F, Paths P, int nslots) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    long long w = s;
+    int rank = F.mode == LAMBDA_RNG_REFERENCE ? s : int(w % F.npix);
+    int sample = F.mode == LAMBDA_RNG_REFERENCE ? 0 : int(w / F.npix);
+    int pixel = P.order[rank];
+    Pcg rng = pcg_seed(F.seed, F.mode == LAMBDA_RNG_REFERENCE ? uint64_t(pixel) : uint64_t(w));
+    camera_sample(S, F, P, s, pixel, sample, rng);
+    store_rng(P, s, rng);
+    P.q_in[s] = s;
+}
+
+// Closest hit for every queued path.
+__global__ void __launch_bounds__(kBlock) k_extend(DeviceScene S, Paths P, int count) {
+    __shared__ int stk[2 * kStack * kBlock];
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0) {
+        P.ctr[kCtrNext] = 0;
+        P.ctr[kCtrShadow] = 0;
+    }
+    if (q >= count) return;
+    int s = P.q_in[q];
+    float4 o = P.o[s], d = P.d[s];
+    float best = kMax;
+    int hi = -1, ho = -1;
+    TraceStats st{};
+    traverse<false, false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk + threadIdx.x, kBlock, st);
+    P.hit[s] = make_int2(hi, ho);
+}
+
+__device__ __forceinline__ S4 lambertian_albedo(const DeviceScene &S, int4 mt, const Hit &h, const S4 &lam) {
+    return spectrum_texture4(S, mt.y, h, lam);
+}
+
+// Shading of every queued path: one iteration body of Renderer::trace
+// (render/renderer.h:102-192) after the closest-hit query.
+__global__ void __launch_bounds__(kBlock) k_shade(DeviceScene S, Frame F, Paths P, int count) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = q < count;
+    bool push_shadow = false;
+    int s = 0;
+    float4 sh_o, sh_d, sh_c;
+    int4 sh_i;
+    if (active) {
+        s = P.q_in[q];
+        float4 o4 = P.o[s], d4 = P.d[s];
+        int4 info = P.info[s];
+        int2 hit = P.hit[s];
+        S4 lam = s4(P.lam[s]);
+        S4 beta = s4(P.beta[s]);
+        S4 L = s4(P.L[s]);
+        V3 o = v3(o4), d = v3(d4);
+        float prev_pdf = o4.w;
+        bool specular = (info.w & kFlagSpecular) != 0;
+        const int bounce = info.z;
+        const float total = S.total_power;
+        bool finished = false;
+        Pcg rng = load_rng(P, s);
+
+        if (hit.x < 0) {
+            // Miss: environment with MIS against its NEE (render/renderer.h:105-113).
+            float w = 1;
+            if (!specular && total > 0)
+                w = power_heuristic(prev_pdf, 1 / (4 * kPi) * S.light_pow[S.num_lights - 1] / total);
+            S4 bg = background(S, d, lam);
+            S4 c = beta * bg * w;
+#pragma unroll
+            for (int k = 0; k < kHero; k++) L.v[k] += c.v[k];
+            finished = true;
+        } else {
+            Hit h;
+            if (!hit_record(S, hit.x, hit.y, o, d, h)) {
+                atomicOr(&P.ctr[kCtrError], 1);
+                finished = true;
+            } else {
+                const int mat = material_of(S, hit.x, hit.y);
+                const int4 mt = S.mat[mat];
+                if (mt.x == LAMBDA_MATERIAL_EMISSIVE) {
+                    // Emitter hit (render/renderer.h:119-131).
+                    float w = 1;
+                    if (!specular && total > 0) {
+                        float lp = instance_area(S, hit.x, hit.y) * spectrum_texture_average(S, mt.y);
+                        w = power_heuristic(prev_pdf, instance_pdf(S, hit.x, hit.y, o, d) * lp / total);
+                    }
+                    S4 e = spectrum_texture4(S, mt.y, h, lam);
+                    S4 c = beta * e * w;
+#pragma unroll
+                    for (int k = 0; k < kHero; k++) L.v[k] += c.v[k];
+                    finished = true;
+                } else {
+                    const bool lambertian = mt.x == LAMBDA_MATERIAL_LAMBERTIAN;
+                    S4 albedo = s4(0);
+                    if (lambertian) albedo = lambertian_albedo(S, mt, h, lam);
+                    if (lambertian && total > 0) {
+                        // Next-event estimation (render/renderer.h:133-175).
+                        float target = total * rng.next();
+                        float cum = 0;
+                        int2 lio = S.light_io[0];
+                        float lp = S.light_pow[0];
+                        for (int j = 0; j < S.num_lights; j++) {
+                            cum += S.light_pow[j];
+                            if (cum >= target) {
+                                lio = S.light_io[j];
+                                lp = S.light_pow[j];
+                                break;
+                            }
+                        }
+                        V3 ldir = lio.x != -1 ? instance_sample(S, lio.x, lio.y, h.p, rng) : random_unit_vector(rng);
+                        float lprob = lio.x != -1 ? instance_pdf(S, lio.x, lio.y, h.p, ldir) : 1 / (4 * kPi);
+                        float ct = dot(h.n, ldir);
+                        if (lprob > 0 && ct > 0) {
+                            lprob *= lp / total;
+                            float tmax = kMax;
+                            S4 em;
+                            bool visible_light = true;
+                            if (lio.x != -1) {
+                                // The occlusion query first intersects the
+                                // light itself; a miss counts as occluded
+                                // (scene/scene.h:61-70).
+                                Hit lh;
+                                if (hit_record(S, lio.x, lio.y, h.p, ldir, lh)) {
+                                    tmax = lh.t;
+                                    em = spectrum_texture4(S, S.mat[material_of(S, lio.x, lio.y)].y, lh, lam);
+                                } else {
+                                    visible_light = false;
+                                }
+                            } else {
+                                em = background(S, ldir, lam);
+                            }
+                            if (visible_light) {
+                                S4 att = s4(0);
+                                if (dot(h.n, ldir) > 0) att = albedo / kPi;
+                                float sp = ct > 0 ? ct / kPi : 0;
+                                float w = power_heuristic(lprob, sp);
+                                S4 c = beta * att * ct * em * w / lprob;
+                                push_shadow = true;
+                                sh_o = make_float4(h.p.x, h.p.y, h.p.z, tmax);
+                                sh_d = make_float4(ldir.x, ldir.y, ldir.z, 0);
+                                sh_c = f4(c);
+                                sh_i = make_int4(s, lio.x, lio.y, 0);
+                            }
+                        }
+                    }
+                    // BSDF sampling (scene/material.h:117-261).
+                    S4 att;
+                    V3 nd;
+                    if (lambertian) {
+                        nd = random_in_hemisphere(h.n, rng);
+                        att = dot(h.n, nd) <= 0 ? s4(0) : albedo / kPi;
+                    } else if (mt.x == LAMBDA_MATERIAL_MIRROR) {
+                        nd = reflected(d, h.n);
+                        att = spectrum_texture4(S, mt.y, h, lam);
+                    } else if (mt.x == LAMBDA_MATERIAL_DIELECTRIC) {
+                        float factor = powf(1 + dot(d, h.n), 5.0f);
+                        const float *s0 = S.spectra + size_t(mt.y) * kBins;
+                        const float *s1 = S.spectra + size_t(mt.z) * kBins;
+                        float n0h = dense(s0, lam.v[0]);
+                        float n1h = dense(s1, lam.v[0]);
+                        float ratio = h.front ? n0h / n1h : n1h / n0h;
+                        float r0 = (n0h - n1h) / (n0h + n1h);
+                        r0 *= r0;
+                        float rh = r0 + (1 - r0) * factor;
+                        bool refl = rng.next() < rh;
+                        nd = refl ? reflected(d, h.n) : refracted(d, h.n, ratio);
+                        att.v[0] = 1;
+#pragma unroll
+                        for (int j = 1; j < kHero; j++) {
+                            float n0 = dense(s0, lam.v[j]);
+                            float n1 = dense(s1, lam.v[j]);
+                            float rr = (n0 - n1) / (n0 + n1);
+                            rr *= rr;
+                            float rj = rr + (1 - rr) * factor;
+                            att.v[j] = refl ? rj / rh
+                                            : (fabsf(n0 - n0h) < kEps && fabsf(n1 - n1h) < kEps) ? (1 - rj) / (1 - rh) : 0;
+                        }
+                    } else {
+                        // Thin film: hero wavelength picks reflect/refract.
+                        const int layers = mt.y;
+                        float thick[kMaxLayers];
+                        for (int j = 0; j < layers && j < kMaxLayers; j++)
+                            thick[j] = scalar_texture(S, S.props[mt.w + j], h.p);
+                        const float2 *c_out = S.cspectra + size_t(S.props[mt.z]) * kBins;
+                        const float2 *c_in = S.cspectra + size_t(S.props[mt.z + layers + 1]) * kBins;
+                        float n0h = dense_c(c_out, lam.v[0]).x;
+                        float n1h = dense_c(c_in, lam.v[0]).x;
+                        float ratio = h.front ? n0h / n1h : n1h / n0h;
+                        float rh = thinfilm_reflectance(S, mt, h.front, h.n, d, lam.v[0], thick);
+                        bool refl = rng.next() < rh;
+                        nd = refl ? reflected(d, h.n) : refracted(d, h.n, ratio);
+                        att.v[0] = 1;
+                        for (int j = 1; j < kHero; j++) {
+                            float n0 = dense_c(c_out, lam.v[j]).x;
+                            float n1 = dense_c(c_in, lam.v[j]).x;
+                            float rj = thinfilm_reflectance(S, mt, h.front, h.n, d, lam.v[j], thick);
+                            att.v[j] = refl ? rj / rh
+                                            : (fabsf(n0 - n0h) < kEps && fabsf(n1 - n1h) < kEps) ? (1 - rj) / (1 - rh) : 0;
+                        }
+                    }
+                    o = h.p;
+                    d = nd;
+                    float ct = dot(h.n, d);
+                    prev_pdf = lambertian ? (ct > 0 ? ct / kPi : 0) : 0;
+                    specular = !lambertian;
+                    if (specular) {
+                        beta = beta * att;
+                    } else {
+                        S4 f = att * ct / prev_pdf;
+                        beta = beta * f;
+                    }
+                    if (bounce >= kRRStart) {
+                        float qq = 1 - clampf(s4max(beta), float(0.05), float(0.95));
+                        if (rng.next() < qq) finished = true;
+                        else {
+#pragma unroll
+                            for (int k = 0; k < kHero; k++) beta.v[k] /= 1 - qq;
+                        }
+                    }
+                    if (bounce >= F.depth) finished = true;
+                }
+            }
+        }
+        P.o[s] = make_float4(o.x, o.y, o.z, prev_pdf);
+        P.d[s] = make_float4(d.x, d.y, d.z, 0);
+        P.beta[s] = f4(beta);
+        P.L[s] = f4(L);
+        store_rng(P, s, rng);
+        P.info[s] = make_int4(info.x, info.y, bounce + 1, (specular ? kFlagSpecular : 0) | (finished ? kFlagFinished : 0));
+    }
+    int pos = wave_push(&P.ctr[kCtrShadow], push_shadow);
+    if (push_shadow) {
+        P.s_o[pos] = sh_o;
+        P.s_d[pos] = sh_d;
+        P.s_c[pos] = sh_c;
+        P.s_i[pos] = sh_i;
+    }
+}
+
+// Occlusion test of every queued NEE ray; unblocked contributions are added
+// to their path's radiance in the order the reference adds them.
+__global__ void __launch_bounds__(kBlock) k_shadow(DeviceScene S, Paths P) {
+    __shared__ int stk[2 * kStack * kBlock];
+    const int count = P.ctr[kCtrShadow];
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+        float4 so = P.s_o[q], sd = P.s_d[q];
+        int4 si = P.s_i[q];
+        float best = so.w;
+        int hi = -1, ho = -1;
+        TraceStats st{};
+        bool blocked = traverse<true, false>(S, v3(so), v3(sd), best, hi, ho, si.y, si.z, stk + threadIdx.x, kBlock, st);
+        if (!blocked) {
+            float4 c = P.s_c[q];
+            float4 L = P.L[si.x];
+            L.x += c.x;
+            L.y += c.y;
+            L.z += c.z;
+            L.w += c.w;
+            P.L[si.x] = L;
+        }
+    }
+}
+
+// XYZ of one sample's radiance: the first half of spectrumToRGB
+// (core/utils.h:212-228), in float, normalised by 106.856895.
+__device__ __forceinline__ void sample_xyz(const DeviceScene &S, const Frame &F, float4 Lr, float4 lamr, float &x,
+                                           float &y, float &z) {
+    const float L[4] = {Lr.x, Lr.y, Lr.z, Lr.w};
+    const float lam[4] = {lamr.x, lamr.y, lamr.z, lamr.w};
+    x = 0;
+    y = 0;
+    z = 0;
+    float weight = F.lrange / kHero;
+#pragma unroll
+    for (int i = 0; i < kHero; i++) {
+        float l = lam[i];
+        int lo = int(l);
+        float c[3];
+        if (lo == 830) {
+            c[0] = S.cie[(lo - 360) * 3 + 0];
+            c[1] = S.cie[(lo - 360) * 3 + 1];
+            c[2] = S.cie[(lo - 360) * 3 + 2];
+        } else {
+            int hi = lo + 1;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                c[k] = (float(hi) - l) * S.cie[(lo - 360) * 3 + k] + (l - float(lo)) * S.cie[(hi - 360) * 3 + k];
+        }
+        x += c[0] * L[i] * weight;
+        y += c[1] * L[i] * weight;
+        z += c[2] * L[i] * weight;
+    }
+    x /= float(106.856895);
+    y /= float(106.856895);
+    z /= float(106.856895);
+}
+
+// Retires finished samples into the framebuffer and gives their slots the
+// next sample; compacts the surviving slots into the next queue.
+__global__ void __launch_bounds__(kBlock) k_finalize(DeviceScene S, Frame F, Paths P, int count) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    bool keep = false;
+    int s = 0;
+    bool need_work = false;
+    if (q < count) {
+        s = P.q_in[q];
+        int4 info = P.info[s];
+        if (!(info.w & kFlagFinished)) {
+            keep = true;
+        } else {
+            float x, y, z;
+            sample_xyz(S, F, P.L[s], P.lam[s], x, y, z);
+            double *px = P.fb + size_t(info.x) * 3;
+            unsafeAtomicAdd(px + 0, double(x));
+            unsafeAtomicAdd(px + 1, double(y));
+            unsafeAtomicAdd(px + 2, double(z));
+            if (F.mode == LAMBDA_RNG_REFERENCE) {
+                if (info.y + 1 < F.spp) {
+                    Pcg rng = load_rng(P, s);
+                    camera_sample(S, F, P, s, info.x, info.y + 1, rng);
+                    store_rng(P, s, rng);
+                    keep = true;
+                } else {
+                    atomicAdd(&P.ctr[kCtrPixels], 1);
+                }
+            } else {
+                atomicAdd(&P.ctr[kCtrPixels], 1);
+                need_work = true;
+            }
+        }
+    }
+    if (F.mode != LAMBDA_RNG_REFERENCE) {
+        int w = wave_push(&P.ctr[kCtrWork], need_work);
+        if (need_work && w < F.total_work) {
+            int rank = int(w % F.npix);
+            int sample = int(w / F.npix);
+            int pixel = P.order[rank];
+            Pcg rng = pcg_seed(F.seed, uint64_t(w));
+            camera_sample(S, F, P, s, pixel, sample, rng);
+            store_rng(P, s, rng);
+            keep = true;
+        }
+    }
+    int pos = wave_push(&P.ctr[kCtrNext], keep);
+    if (keep) P.q_out[pos] = s;
+}
+
+// Mean XYZ over samples to linear sRGB (core/utils.h:230), in double.
+__global__ void k_resolve(Frame F, const double *fb, float *rgb) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= F.npix) return;
+    double x = fb[3 * size_t(p) + 0] / F.spp;
+    double y = fb[3 * size_t(p) + 1] / F.spp;
+    double z = fb[3 * size_t(p) + 2] / F.spp;
+    rgb[3 * size_t(p) + 0] = float(x * double(float(3.2406255)) - y * double(float(1.5372080)) - z * double(float(0.4986286)));
+    rgb[3 * size_t(p) + 1] = float(-x * double(float(0.9689307)) + y * double(float(1.8757561)) + z * double(float(0.0415175)));
+    rgb[3 * size_t(p) + 2] = float(x * double(float(0.0557101)) - y * double(float(0.2040211)) + z * double(float(1.0569959)));
+}
+
+__global__ void k_fill_d(double *p, size_t n) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+__global__ void k_fill_i(int *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------------------ query kernels
+
+__global__ void __launch_bounds__(kBlock) k_query_closest(DeviceScene S, const float *rays, int n, lambda_hit *out) {
+    __shared__ int stk[2 * kStack * kBlock];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 o = v3(rays[6 * i + 0], rays[6 * i + 1], rays[6 * i + 2]);
+    V3 d = v3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
+    float best = kMax;
+    int hi = -1, ho = -1;
+    TraceStats st{};
+    bool ok = traverse<false, false>(S, o, d, best, hi, ho, -1, -2, stk + threadIdx.x, kBlock, st);
+    lambda_hit r;
+    r.hit = 0;
+    r.instance = -1;
+    r.object = -1;
+    r.t = kMax;
+    r.point[0] = r.point[1] = r.point[2] = 0;
+    r.normal[0] = r.normal[1] = r.normal[2] = 0;
+    r.u = r.v = 0;
+    r.front_facing = 0;
+    if (ok) {
+        Hit h;
+        if (hit_record(S, hi, ho, o, d, h)) {
+            r.hit = 1;
+            r.instance = hi;
+            r.object = ho;
+            r.t = h.t;
+            r.point[0] = h.p.x; r.point[1] = h.p.y; r.point[2] = h.p.z;
+            r.normal[0] = h.n.x; r.normal[1] = h.n.y; r.normal[2] = h.n.z;
+            r.u = h.u;
+            r.v = h.v;
+            r.front_facing = h.front ? 1 : 0;
+        }
+    }
+    out[i] = r;
+}
+
+__global__ void __launch_bounds__(kBlock) k_query_occluded(DeviceScene S, const float *rays, const int *li,
+                                                           const int *lo, int n, int *out) {
+    __shared__ int stk[2 * kStack * kBlock];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 o = v3(rays[6 * i + 0], rays[6 * i + 1], rays[6 * i + 2]);
+    V3 d = v3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
+    float best = kMax;
+    int inst = li[i], obj = lo[i];
+    if (inst != -1) {
+        Hit lh;
+        if (!hit_record(S, inst, obj, o, d, lh)) {
+            out[i] = 1;
+            return;
+        }
+        best = lh.t;
+    }
+    int hi = -1, ho = -1;
+    TraceStats st{};
+    out[i] = traverse<true, false>(S, o, d, best, hi, ho, inst, obj, stk + threadIdx.x, kBlock, st) ? 1 : 0;
+}
+
+__global__ void k_query_thinfilm(DeviceScene S, int mat, const float *q, const float *pts, int n, float *out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float c = q[4 * i + 0];
+    float lambda = q[4 * i + 1];
+    bool front = q[4 * i + 2] != 0;
+    float s = sqrtf(fmaxf(0.0f, 1 - c * c));
+    V3 p = v3(pts[3 * i + 0], pts[3 * i + 1], pts[3 * i + 2]);
+    int4 mt = S.mat[mat];
+    float thick[kMaxLayers];
+    for (int j = 0; j < mt.y && j < kMaxLayers; j++) thick[j] = scalar_texture(S, S.props[mt.w + j], p);
+    out[i] = thinfilm_reflectance(S, mt, front, v3(0, 0, 1), v3(s, 0, -c), lambda, thick);
+}
+
+// ------------------------------------------------------------ host: scene
+
+void require(bool ok, const std::string &what) {
+    if (!ok) throw Failure{LAMBDA_EINVAL, "invalid scene: " + what};
+}
+
+// Deepest stack a depth-first traversal of the tree at `root` needs, with
+// the push-two/pop-one discipline of the kernels; also checks the indices.
+int tree_stack_depth(const lambda_scene_view &v, int root, int leaf_limit, bool instance_level) {
+    struct Item {
+        int node, sp;
+    };
+    std::vector<Item> work{{root, 1}};
+    int deepest = 1;
+    long long visits = 0;
+    while (!work.empty()) {
+        Item it = work.back();
+        work.pop_back();
+        require(it.node >= 0 && it.node < v.num_nodes, "node index out of range");
+        require(++visits <= 4LL * v.num_nodes + 4, "node graph is not a tree");
+        int idx = v.node_ints[2 * it.node], cnt = v.node_ints[2 * it.node + 1];
+        if (cnt > 0) {
+            require(idx >= 0 && (long long)idx + cnt <= leaf_limit, "leaf range out of range");
+            (void)instance_level;
+            continue;
+        }
+        require(cnt == 0, "negative node count");
+        // popping this node leaves sp-1 entries; pushing two children
+        int sp = it.sp - 1 + 2;
+        deepest = std::max(deepest, sp);
+        work.push_back({idx, sp});      // right child is popped first
+        work.push_back({it.node + 1, sp - 1});
+    }
+    return deepest;
+}
+
+struct HostSceneCopy {
+    std::vector<float4> node_lo, node_hi, obj_a, obj_b, obj_c, obj_n, obj_s, inst;
+    std::vector<int> obj_mat;
+    std::vector<int4> inst_i, mat, xtex;
+    std::vector<float4> stex;
+    std::vector<float2> cspec;
+    std::vector<int2> light_io;
+    std::vector<float> cie;
+};
+
+void validate(const lambda_scene_view &v) {
+    require(v.num_nodes > 0 && v.num_objects > 0 && v.num_instances > 0, "empty scene");
+    require(v.num_lights >= 1, "light list must end with the environment entry");
+    require(v.num_spectra > 0 && v.num_spectrum_textures > 0 && v.num_scalar_textures > 0 && v.num_materials > 0,
+            "missing tables");
+    require(v.spectra && v.spectrum_texture_type && v.spectrum_texture_index && v.spectrum_texture_scale &&
+                v.scalar_texture_type && v.scalar_texture_params && v.material_type && v.material_params &&
+                v.object_type && v.object_material && v.object_geometry && v.object_shading && v.instance_ints &&
+                v.instance_matrices && v.node_bounds && v.node_ints && v.light_instances && v.light_objects &&
+                v.light_powers,
+            "null array");
+    require(v.num_complex_spectra == 0 || v.complex_spectra, "null complex spectra");
+    require(v.num_material_properties == 0 || v.material_properties, "null material properties");
+    for (int i = 0; i < v.num_scalar_textures; i++) {
+        int t = v.scalar_texture_type[i];
+        require(t >= 0 && t <= 2, "scalar texture type");
+    }
+    for (int i = 0; i < v.num_spectrum_textures; i++) {
+        int t = v.spectrum_texture_type[i];
+        int a = v.spectrum_texture_index[2 * i], b = v.spectrum_texture_index[2 * i + 1];
+        require(t >= 0 && t <= 2, "spectrum texture type");
+        if (t == LAMBDA_SPECTRUM_SCALAR) require(a >= 0 && a < v.num_scalar_textures, "spectrum texture index");
+        else require(a >= 0 && a < v.num_spectra, "spectrum texture spectrum");
+        if (t == LAMBDA_SPECTRUM_CHECKER) require(b >= 0 && b < v.num_spectra, "checker spectrum");
+    }
+    for (int i = 0; i < v.num_materials; i++) {
+        const int *p = v.material_params + 3 * i;
+        int t = v.material_type[i];
+        require(t >= 0 && t <= 4, "material type");
+        if (t == LAMBDA_MATERIAL_LAMBERTIAN || t == LAMBDA_MATERIAL_MIRROR || t == LAMBDA_MATERIAL_EMISSIVE)
+            require(p[0] >= 0 && p[0] < v.num_spectrum_textures, "material texture");
+        if (t == LAMBDA_MATERIAL_DIELECTRIC)
+            require(p[0] >= 0 && p[0] < v.num_spectra && p[1] >= 0 && p[1] < v.num_spectra, "dielectric spectra");
+        if (t == LAMBDA_MATERIAL_THINFILM) {
+            int layers = p[0];
+            require(layers >= 0 && layers <= kMaxLayers, "thin film layer count (at most 8)");
+            require(p[1] >= 0 && (long long)p[1] + layers + 2 <= v.num_material_properties, "thin film n range");
+            require(p[2] >= 0 && (long long)p[2] + layers <= v.num_material_properties, "thin film d range");
+            for (int j = 0; j < layers + 2; j++) {
+                int c = v.material_properties[p[1] + j];
+                require(c >= 0 && c < v.num_complex_spectra, "thin film index spectrum");
+            }
+            for (int j = 0; j < layers; j++) {
+                int c = v.material_properties[p[2] + j];
+                require(c >= 0 && c < v.num_scalar_textures, "thin film thickness texture");
+            }
+        }
+    }
+    for (int i = 0; i < v.num_objects; i++) {
+        require(v.object_type[i] >= 0 && v.object_type[i] <= 2, "object type");
+        require(v.object_material[i] >= 0 && v.object_material[i] < v.num_materials, "object material");
+    }
+    for (int i = 0; i < v.num_instances; i++) {
+        const int *q = v.instance_ints + 4 * i;
+        require(q[0] >= 0 && q[1] >= 1 && (long long)q[0] + q[1] <= v.num_objects, "instance object range");
+        require(q[2] >= 0 && q[2] < v.num_nodes, "instance BLAS root");
+        require(q[3] >= -1 && q[3] < v.num_materials, "instance material");
+    }
+    for (int i = 0; i < v.num_lights; i++) {
+        int li = v.light_instances[i], lo = v.light_objects[i];
+        if (li == -1) {
+            require(lo == -1, "environment light object");
+        } else {
+            require(li >= 0 && li < v.num_instances, "light instance");
+            const int *q = v.instance_ints + 4 * li;
+            require(lo >= q[0] && lo < q[0] + q[1], "light object outside its instance");
+        }
+    }
+    require(v.light_instances[v.num_lights - 1] == -1, "last light must be the environment");
+    require(v.background_texture >= 0 && v.background_texture < v.num_spectrum_textures, "background texture");
+    require(tree_stack_depth(v, 0, v.num_instances, true) <= kStack, "instance BVH deeper than 20");
+    for (int i = 0; i < v.num_instances; i++)
+        require(tree_stack_depth(v, v.instance_ints[4 * i + 2], v.num_objects, false) <= kStack,
+                "object BVH deeper than 20");
+}
+
+}  // namespace
+
+struct lambda_hip_scene {
+    DevBuf<float4> node_lo, node_hi, obj_a, obj_b, obj_c, obj_n, obj_s, inst, stex;
+    DevBuf<int> obj_mat, props, perm;
+    DevBuf<int4> inst_i, mat, xtex;
+    DevBuf<float> spectra, light_pow, cie;
+    DevBuf<float2> cspec;
+    DevBuf<int2> light_io;
+    DeviceScene ds{};
+    int num_materials = 0;
+    std::vector<int> material_type;
+};
+
+namespace {
+
+void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g) {
+    validate(v);
+    const size_t N = size_t(v.num_nodes), O = size_t(v.num_objects), I = size_t(v.num_instances);
+    std::vector<float4> lo(N), hi(N);
+    for (size_t i = 0; i < N; i++) {
+        const float *b = v.node_bounds + 6 * i;
+        int idx = v.node_ints[2 * i], cnt = v.node_ints[2 * i + 1];
+        float fi, fc;
+        std::memcpy(&fi, &idx, 4);
+        std::memcpy(&fc, &cnt, 4);
+        lo[i] = make_float4(b[0], b[1], b[2], fi);
+        hi[i] = make_float4(b[3], b[4], b[5], fc);
+    }
+    std::vector<float4> oa(O), ob(O), oc(O), on(O), os(4 * O);
+    std::vector<int> om(O);
+    for (size_t i = 0; i < O; i++) {
+        const float *g0 = v.object_geometry + 12 * i;
+        const float *sh = v.object_shading + 15 * i;
+        int type = v.object_type[i];
+        float ft;
+        std::memcpy(&ft, &type, 4);
+        if (type == LAMBDA_OBJECT_SPHERE) {
+            oa[i] = make_float4(g0[0], g0[1], g0[2], g0[3]);
+            ob[i] = make_float4(0, 0, 0, ft);
+            oc[i] = make_float4(0, 0, 0, 0);
+            on[i] = make_float4(0, 0, 0, 0);
+        } else {
+            oa[i] = make_float4(g0[0], g0[1], g0[2], 0);
+            ob[i] = make_float4(g0[3], g0[4], g0[5], ft);
+            oc[i] = make_float4(g0[6], g0[7], g0[8], 0);
+            on[i] = make_float4(g0[9], g0[10], g0[11], 0);
+        }
+        os[4 * i + 0] = make_float4(sh[0], sh[1], sh[2], sh[3]);
+        os[4 * i + 1] = make_float4(sh[4], sh[5], sh[6], sh[7]);
+        os[4 * i + 2] = make_float4(sh[8], sh[9], sh[10], sh[11]);
+        os[4 * i + 3] = make_float4(sh[12], sh[13], sh[14], 0);
+        om[i] = v.object_material[i];
+    }
+    std::vector<float4> im(8 * I);
+    std::vector<int4> ii(I);
+    for (size_t i = 0; i < I; i++) {
+        const float *m = v.instance_matrices + 32 * i;
+        for (int r = 0; r < 8; r++) im[8 * i + r] = make_float4(m[4 * r], m[4 * r + 1], m[4 * r + 2], m[4 * r + 3]);
+        const int *q = v.instance_ints + 4 * i;
+        ii[i] = make_int4(q[0], q[1], q[2], q[3]);
+    }
+    std::vector<int4> mt(size_t(v.num_materials));
+    g.material_type.resize(size_t(v.num_materials));
+    for (int i = 0; i < v.num_materials; i++) {
+        const int *p = v.material_params + 3 * i;
+        int t = v.material_type[i];
+        g.material_type[i] = t;
+        // thin film: {type, layers, offset of n, offset of d}
+        mt[i] = make_int4(t, p[0], p[1], p[2]);
+    }
+    g.num_materials = v.num_materials;
+    std::vector<float4> st(size_t(v.num_scalar_textures));
+    for (int i = 0; i < v.num_scalar_textures; i++) {
+        int t = v.scalar_texture_type[i];
+        float ft;
+        std::memcpy(&ft, &t, 4);
+        const float *p = v.scalar_texture_params + 3 * i;
+        st[i] = make_float4(ft, p[0], p[1], p[2]);
+    }
+    std::vector<int4> xt(size_t(v.num_spectrum_textures));
+    for (int i = 0; i < v.num_spectrum_textures; i++) {
+        int sc;
+        std::memcpy(&sc, v.spectrum_texture_scale + i, 4);
+        xt[i] = make_int4(v.spectrum_texture_type[i], v.spectrum_texture_index[2 * i], v.spectrum_texture_index[2 * i + 1], sc);
+    }
+    std::vector<float2> cs(size_t(v.num_complex_spectra) * kBins);
+    for (size_t i = 0; i < cs.size(); i++)
+        cs[i] = make_float2(float(v.complex_spectra[2 * i]), float(v.complex_spectra[2 * i + 1]));
+    std::vector<int2> lio(size_t(v.num_lights));
+    for (int i = 0; i < v.num_lights; i++) lio[i] = make_int2(v.light_instances[i], v.light_objects[i]);
+    std::vector<float> cie(kBins * 3);
+    for (int i = 0; i < kBins; i++)
+        for (int k = 0; k < 3; k++) cie[3 * i + k] = float(kCie1931Xyz[i][k]);
+
+    g.node_lo.upload(lo);
+    g.node_hi.upload(hi);
+    g.obj_a.upload(oa);
+    g.obj_b.upload(ob);
+    g.obj_c.upload(oc);
+    g.obj_n.upload(on);
+    g.obj_s.upload(os);
+    g.obj_mat.upload(om);
+    g.inst.upload(im);
+    g.inst_i.upload(ii);
+    g.mat.upload(mt);
+    g.stex.upload(st);
+    g.xtex.upload(xt);
+    g.spectra.upload(v.spectra, size_t(v.num_spectra) * kBins);
+    if (v.num_complex_spectra) g.cspec.upload(cs);
+    if (v.num_material_properties) g.props.upload(v.material_properties, size_t(v.num_material_properties));
+    g.light_io.upload(lio);
+    g.light_pow.upload(v.light_powers, size_t(v.num_lights));
+    g.cie.upload(cie);
+    g.perm.upload(kPerlinPermutation, 256);
+
+    DeviceScene &d = g.ds;
+    d.node_lo = g.node_lo.p;
+    d.node_hi = g.node_hi.p;
+    d.obj_a = g.obj_a.p;
+    d.obj_b = g.obj_b.p;
+    d.obj_c = g.obj_c.p;
+    d.obj_n = g.obj_n.p;
+    d.obj_s = g.obj_s.p;
+    d.obj_mat = g.obj_mat.p;
+    d.inst = g.inst.p;
+    d.inst_i = g.inst_i.p;
+    d.mat = g.mat.p;
+    d.stex = g.stex.p;
+    d.xtex = g.xtex.p;
+    d.spectra = g.spectra.p;
+    d.cspectra = g.cspec.p;
+    d.props = g.props.p;
+    d.light_io = g.light_io.p;
+    d.light_pow = g.light_pow.p;
+    d.cie = g.cie.p;
+    d.perm = g.perm.p;
+    const float *c = v.camera;
+    d.cam_pos = make_float4(c[0], c[1], c[2], 0);
+    d.cam_corner = make_float4(c[3], c[4], c[5], 0);
+    d.cam_h = make_float4(c[6], c[7], c[8], 0);
+    d.cam_v = make_float4(c[9], c[10], c[11], 0);
+    d.total_power = v.total_light_power;
+    d.num_lights = v.num_lights;
+    d.bg_tex = v.background_texture;
+    d.num_nodes = v.num_nodes;
+    d.num_objects = v.num_objects;
+    d.num_instances = v.num_instances;
+}
+
+void check_device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Failure{LAMBDA_ENODEV, "no HIP device visible"};
+}
+
+// Pixels in 8x8 tiles, tiles row-major: neighbouring lanes trace
+// neighbouring pixels.
+std::vector<int> tile_order(int w, int h) {
+    std::vector<int> order;
+    order.reserve(size_t(w) * h);
+    for (int ty = 0; ty < h; ty += 8)
+        for (int tx = 0; tx < w; tx += 8)
+            for (int y = ty; y < std::min(ty + 8, h); y++)
+                for (int x = tx; x < std::min(tx + 8, w); x++) order.push_back(y * w + x);
+    return order;
+}
+
+struct RenderBuffers {
+    DevBuf<float4> o, d, lam, beta, L, s_o, s_d, s_c;
+    DevBuf<uint4> rng;
+    DevBuf<int4> info, s_i;
+    DevBuf<int2> hit;
+    DevBuf<int> qa, qb, ctr, order;
+    DevBuf<double> fb;
+    DevBuf<float> rgb;
+};
+
+void validate_params(const lambda_render_params &p) {
+    if (p.width <= 0 || p.height <= 0 || p.depth < 0 || p.samples < 1 || p.sqrt_samples < 1 ||
+        p.sqrt_samples * p.sqrt_samples != p.samples)
+        throw Failure{LAMBDA_EINVAL, "invalid render parameters (size, depth or samples)"};
+    if (!(p.lambda_min >= 360 && p.lambda_max <= 830 && p.lambda_min < p.lambda_max))
+        throw Failure{LAMBDA_EINVAL, "invalid wavelength range"};
+    if (p.rng_mode != LAMBDA_RNG_REFERENCE && p.rng_mode != LAMBDA_RNG_THROUGHPUT)
+        throw Failure{LAMBDA_EINVAL, "invalid rng_mode"};
+    if ((long long)p.width * p.height > (1LL << 28)) throw Failure{LAMBDA_EINVAL, "image too large"};
+    if (p.rng_mode == LAMBDA_RNG_THROUGHPUT && (long long)p.width * p.height * p.samples >= (1LL << 31))
+        throw Failure{LAMBDA_EINVAL, "throughput mode supports fewer than 2^31 samples per render"};
+    if ((long long)p.width * p.height * p.samples > (1LL << 40)) throw Failure{LAMBDA_EINVAL, "too many samples"};
+}
+
+int grid_for(long long n) { return int(std::max(1LL, (n + kBlock - 1) / kBlock)); }
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+struct StreamGuard {
+    hipStream_t s = nullptr;
+    ~StreamGuard() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_rgb, volatile int32_t *progress,
+                 lambda_render_stats *stats = nullptr) {
+    auto wall0 = std::chrono::steady_clock::now();
+    validate_params(p);
+    Frame F;
+    F.width = p.width;
+    F.height = p.height;
+    F.depth = p.depth;
+    F.spp = p.samples;
+    F.sqrt_spp = p.sqrt_samples;
+    F.mode = p.rng_mode;
+    F.npix = p.width * p.height;
+    F.lmin = p.lambda_min;
+    F.lmax = p.lambda_max;
+    F.lrange = p.lambda_max - p.lambda_min;
+    F.seed = p.seed;
+    F.total_work = (long long)F.npix * F.spp;
+
+    long long slots;
+    if (F.mode == LAMBDA_RNG_REFERENCE) {
+        slots = F.npix;
+    } else {
+        long long want = p.path_slots > 0 ? p.path_slots : (1LL << 21);
+        slots = std::min<long long>(want, F.total_work);
+        slots = std::max<long long>(slots, 1);
+    }
+    if (slots > (1LL << 28)) throw Failure{LAMBDA_EINVAL, "too many path slots"};
+    const int ns = int(slots);
+
+    RenderBuffers B;
+    B.o.alloc(ns); B.d.alloc(ns); B.lam.alloc(ns); B.beta.alloc(ns); B.L.alloc(ns);
+    B.s_o.alloc(ns); B.s_d.alloc(ns); B.s_c.alloc(ns); B.s_i.alloc(ns);
+    B.rng.alloc(ns); B.info.alloc(ns); B.hit.alloc(ns);
+    B.qa.alloc(ns); B.qb.alloc(ns);
+    B.ctr.alloc(kCtrCount);
+    B.fb.alloc(size_t(F.npix) * 3);
+    B.rgb.alloc(size_t(F.npix) * 3);
+    B.order.upload(tile_order(p.width, p.height));
+
+    Paths P;
+    P.o = B.o.p; P.d = B.d.p; P.lam = B.lam.p; P.beta = B.beta.p; P.L = B.L.p;
+    P.rng = B.rng.p; P.info = B.info.p; P.hit = B.hit.p;
+    P.q_in = B.qa.p; P.q_out = B.qb.p;
+    P.s_o = B.s_o.p; P.s_d = B.s_d.p; P.s_c = B.s_c.p; P.s_i = B.s_i.p;
+    P.ctr = B.ctr.p; P.fb = B.fb.p; P.order = B.order.p;
+
+    int *host_ctr = nullptr;
+    HIP_OK(hipHostMalloc((void **)&host_ctr, kCtrCount * sizeof(int), hipHostMallocDefault));
+    struct PinGuard {
+        int *p;
+        ~PinGuard() { if (p) (void)hipHostFree(p); }
+    } pin{host_ctr};
+
+    StreamGuard sg;
+    HIP_OK(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
+    hipStream_t st = sg.s;
+    EventPair ev[4];
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->path_slots = ns;
+        for (auto &e : ev) {
+            HIP_OK(hipEventCreate(&e.a));
+            HIP_OK(hipEventCreate(&e.b));
+        }
+    }
+    k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
+    k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
+    k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns);
+    HIP_OK(hipGetLastError());
+    // Work counter for throughput mode starts after the slots' first items.
+    if (F.mode != LAMBDA_RNG_REFERENCE) {
+        int start = ns;
+        HIP_OK(hipMemcpyAsync(P.ctr + kCtrWork, &start, sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    int count = ns;
+    const long long total_units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
+    // Every pass advances each live path by one segment; a slot runs at most
+    // (depth + 1) segments per sample it owns.
+    const long long per_slot = F.mode == LAMBDA_RNG_REFERENCE ? F.spp : (F.total_work + ns - 1) / ns + 1;
+    const long long max_iters = (long long)(F.depth + 1) * per_slot + 16;
+    long long iters = 0;
+    while (count > 0) {
+        if (++iters > max_iters) throw Failure{LAMBDA_EHIP, "bounce loop did not drain"};
+        const int grid = grid_for(count);
+        if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
+        k_extend<<<grid, kBlock, 0, st>>>(g.ds, P, count);
+        if (stats) {
+            HIP_OK(hipEventRecord(ev[0].b, st));
+            HIP_OK(hipEventRecord(ev[1].a, st));
+        }
+        k_shade<<<grid, kBlock, 0, st>>>(g.ds, F, P, count);
+        if (stats) {
+            HIP_OK(hipEventRecord(ev[1].b, st));
+            HIP_OK(hipEventRecord(ev[2].a, st));
+        }
+        k_shadow<<<grid, kBlock, 0, st>>>(g.ds, P);
+        if (stats) {
+            HIP_OK(hipEventRecord(ev[2].b, st));
+            HIP_OK(hipEventRecord(ev[3].a, st));
+        }
+        k_finalize<<<grid, kBlock, 0, st>>>(g.ds, F, P, count);
+        if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(host_ctr, P.ctr, kCtrCount * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (host_ctr[kCtrError]) throw Failure{LAMBDA_EHIP, "device reported an inconsistent hit record"};
+        if (stats) {
+            stats->iterations++;
+            stats->closest_rays += count;
+            stats->shadow_rays += host_ctr[kCtrShadow];
+            for (int k = 0; k < 4; k++) {
+                float ms = 0;
+                HIP_OK(hipEventElapsedTime(&ms, ev[k].a, ev[k].b));
+                stats->kernel_ms[k] += ms;
+                stats->launches[k]++;
+            }
+        }
+        count = host_ctr[kCtrNext];
+        std::swap(P.q_in, P.q_out);
+        if (progress) {
+            long long done = host_ctr[kCtrPixels];
+            if (F.mode != LAMBDA_RNG_REFERENCE) done = done * F.npix / std::max<long long>(1, total_units);
+            *progress = int32_t(std::min<long long>(done, F.npix));
+        }
+    }
+    k_resolve<<<grid_for(F.npix), kBlock, 0, st>>>(F, P.fb, B.rgb.p);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out_rgb, B.rgb.p, size_t(F.npix) * 3 * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (progress) *progress = F.npix;
+    if (stats)
+        stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+}
+
+template <typename F>
+int guarded(char *err, size_t len, F &&fn) {
+    try {
+        fn();
+        return LAMBDA_OK;
+    } catch (const Failure &f) {
+        set_err(err, len, f.msg);
+        return f.code;
+    } catch (const std::bad_alloc &) {
+        set_err(err, len, "host allocation failed");
+        return LAMBDA_ENOMEM;
+    } catch (const std::exception &e) {
+        set_err(err, len, e.what());
+        return LAMBDA_EINVAL;
+    } catch (...) {
+        set_err(err, len, "unknown failure");
+        return LAMBDA_EINVAL;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lambda_hip_scene_create(const lambda_scene_view *scene, lambda_hip_scene **out, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scene || !out) throw Failure{LAMBDA_EINVAL, "null argument"};
+        *out = nullptr;
+        check_device();
+        auto *g = new lambda_hip_scene;
+        try {
+            upload_scene(*scene, *g);
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+
+void lambda_hip_scene_destroy(lambda_hip_scene *scene) { delete scene; }
+
+int lambda_hip_scene_render(lambda_hip_scene *scene, const lambda_render_params *params, float *out_rgb,
+                            volatile int32_t *progress, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scene || !params || !out_rgb) throw Failure{LAMBDA_EINVAL, "null argument"};
+        render_impl(*scene, *params, out_rgb, progress);
+    });
+}
+
+int lambda_hip_scene_render_stats(lambda_hip_scene *scene, const lambda_render_params *params, float *out_rgb,
+                                  lambda_render_stats *stats, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scene || !params || !out_rgb || !stats) throw Failure{LAMBDA_EINVAL, "null argument"};
+        render_impl(*scene, *params, out_rgb, nullptr, stats);
+    });
+}
+
+int lambda_hip_render(const lambda_scene_view *scene, const lambda_render_params *params, float *out_rgb,
+                      volatile int32_t *progress, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scene || !params || !out_rgb) throw Failure{LAMBDA_EINVAL, "null argument"};
+        validate_params(*params);
+        check_device();
+        lambda_hip_scene g;
+        upload_scene(*scene, g);
+        render_impl(g, *params, out_rgb, progress);
+    });
+}
+
+int lambda_hip_closest_hit(lambda_hip_scene *scene, const float *rays, int32_t n, lambda_hit *hits, char *err,
+                           size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scene || (n > 0 && (!rays || !hits)) || n < 0) throw Failure{LAMBDA_EINVAL, "invalid arguments"};
+        if (n == 0) return;
+        DevBuf<float> r;
+        r.upload(rays, size_t(n) * 6);
+        DevBuf<lambda_hit> h;
+        h.alloc(size_t(n));
+        k_query_closest<<<grid_for(n), kBlock>>>(scene->ds, r.p, n, h.p);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpy(hits, h.p, size_t(n) * sizeof(lambda_hit), hipMemcpyDeviceToHost));
+    });
+}
+
+int lambda_hip_occluded(lambda_hip_scene *scene, const float *rays, const int32_t *light_instance,
+                        const int32_t *light_object, int32_t n, int32_t *occluded, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scene || n < 0 || (n > 0 && (!rays || !light_instance || !light_object || !occluded)))
+            throw Failure{LAMBDA_EINVAL, "invalid arguments"};
+        if (n == 0) return;
+        const lambda_hip_scene &g = *scene;
+        for (int i = 0; i < n; i++) {
+            int li = light_instance[i], lo = light_object[i];
+            if (li == -1) {
+                if (lo != -1) throw Failure{LAMBDA_EINVAL, "environment query needs object -1"};
+            } else if (li < 0 || li >= g.ds.num_instances || lo < 0 || lo >= g.ds.num_objects) {
+                throw Failure{LAMBDA_EINVAL, "light index out of range"};
+            }
+        }
+        DevBuf<float> r;
+        r.upload(rays, size_t(n) * 6);
+        DevBuf<int> li, lo, out;
+        li.upload(light_instance, size_t(n));
+        lo.upload(light_object, size_t(n));
+        out.alloc(size_t(n));
+        k_query_occluded<<<grid_for(n), kBlock>>>(scene->ds, r.p, li.p, lo.p, n, out.p);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpy(occluded, out.p, size_t(n) * sizeof(int), hipMemcpyDeviceToHost));
+    });
+}
+
+int lambda_hip_thinfilm_reflectance(lambda_hip_scene *scene, int32_t material, const float *queries,
+                                    const float *points, int32_t n, float *out, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scene || n < 0 || (n > 0 && (!queries || !points || !out))) throw Failure{LAMBDA_EINVAL, "invalid arguments"};
+        if (material < 0 || material >= scene->num_materials ||
+            scene->material_type[size_t(material)] != LAMBDA_MATERIAL_THINFILM)
+            throw Failure{LAMBDA_EINVAL, "material is not a thin film"};
+        if (n == 0) return;
+        DevBuf<float> q, p, o;
+        q.upload(queries, size_t(n) * 4);
+        p.upload(points, size_t(n) * 3);
+        o.alloc(size_t(n));
+        k_query_thinfilm<<<grid_for(n), kBlock>>>(scene->ds, material, q.p, p.p, n, o.p);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpy(out, o.p, size_t(n) * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int lambda_hip_device_info(char *name, size_t name_len, int32_t *compute_units, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        check_device();
+        int dev = 0;
+        HIP_OK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        HIP_OK(hipGetDeviceProperties(&prop, dev));
+        if (name && name_len) set_err(name, name_len, std::string(prop.name) + " (" + prop.gcnArchName + ")");
+        if (compute_units) *compute_units = prop.multiProcessorCount;
+    });
+}
+
+}  // extern "C"

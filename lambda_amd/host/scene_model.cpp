@@ -158,6 +158,24 @@ Vec3 Instance::bmax(const std::vector<Object> &objects) const {
     return r;
 }
 
+// bmin() and bmax() in one pass over the corners: the running minimum and
+// maximum see the same values in the same order as the separate loops.
+void Instance::bounds(const std::vector<Object> &objects, Vec3 &lo, Vec3 &hi) const {
+    Vec3 a(kMax, kMax, kMax), b(-kMax, -kMax, -kMax);
+    for (int i = 0; i < count; i++) {
+        Vec3 p = objects[object + i].bmin(), q = objects[object + i].bmax();
+        const Vec3 corners[8] = {p, q, Vec3(p.x, p.y, q.z), Vec3(p.x, q.y, p.z), Vec3(p.x, q.y, q.z),
+                                 Vec3(q.x, p.y, p.z), Vec3(q.x, p.y, q.z), Vec3(q.x, q.y, p.z)};
+        for (const Vec3 &c : corners) {
+            Vec3 w = to_world(c);
+            a = vmin(a, w);
+            b = vmax(b, w);
+        }
+    }
+    lo = a;
+    hi = b;
+}
+
 Vec3 Instance::center(const std::vector<Object> &objects) const {
     Vec3 c(0, 0, 0);
     for (int i = 0; i < count; i++) c = c + to_world(objects[object + i].center());
@@ -345,15 +363,17 @@ void build_bvh(Scene &s, const std::vector<int> &offsets) {
     s.nodes.clear();
     if (s.objects.empty()) return;
 
-    // Instance level.
+    // Instance level.  Each instance's bounds loop over all its objects'
+    // transformed box corners; instances are independent, so they are spread
+    // over threads (each instance's own reduction keeps its sequential order).
     {
         PrimBounds pb;
         size_t n = s.instances.size();
         pb.lo.resize(n); pb.hi.resize(n); pb.c.resize(n);
-        for (size_t i = 0; i < n; i++) {
-            pb.lo[i] = s.instances[i].bmin(s.objects);
-            pb.hi[i] = s.instances[i].bmax(s.objects);
-            pb.c[i] = s.instances[i].center(s.objects);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (long i = 0; i < long(n); i++) {
+            s.instances[size_t(i)].bounds(s.objects, pb.lo[size_t(i)], pb.hi[size_t(i)]);
+            pb.c[size_t(i)] = s.instances[size_t(i)].center(s.objects);
         }
         std::vector<int> idx(n);
         std::iota(idx.begin(), idx.end(), 0);

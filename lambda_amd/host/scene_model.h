@@ -73,6 +73,7 @@ struct Instance {
     Vec3 to_world(const Vec3 &p) const { return transform(M, p, 1); }
     Vec3 bmin(const std::vector<Object> &objects) const;
     Vec3 bmax(const std::vector<Object> &objects) const;
+    void bounds(const std::vector<Object> &objects, Vec3 &lo, Vec3 &hi) const;
     Vec3 center(const std::vector<Object> &objects) const;
     Vec3 center(const std::vector<Object> &objects, int i) const { return to_world(objects[object + i].center()); }
     float radius(const std::vector<Object> &objects, int i) const;

@@ -44,7 +44,7 @@ def _load():
         lib.oracle_thinfilm.argtypes = [sv, C.c_int32, f32, C.c_int32, f32, C.POINTER(C.c_double)]
         lib.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_int32, C.POINTER(C.c_uint32)]
         lib.oracle_pcg32.restype = None
-        lib.oracle_path_stats.argtypes = [sv, rp, C.c_int64, C.c_int64, C.c_int32, C.POINTER(C.c_int64)]
+        lib.oracle_path_stats.argtypes = [sv, rp, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.POINTER(C.c_int64)]
         _lib = lib
     return _lib
 
@@ -106,25 +106,35 @@ def pcg32(seed: int, stream: int, n: int) -> np.ndarray:
     return out
 
 
-STAT_NAMES = ("paths", "closest_rays", "shadow_rays", "node_tests", "instance_entries",
-              "sphere_tests", "quad_tests", "triangle_tests")
+STAT_NAMES = ("paths", "closest_rays", "shadow_rays",
+              "closest_nodes", "closest_instances", "closest_spheres", "closest_quads", "closest_triangles",
+              "shadow_nodes", "shadow_instances", "shadow_spheres", "shadow_quads", "shadow_triangles")
 
 
-def path_stats(scene, params=None, pixels=None, threads: int = 0) -> dict:
-    """Traversal work counted on the reference path over a pixel range."""
+def path_stats(scene, params=None, pixels=None, stride: int = 1, threads: int = 0) -> dict:
+    """Traversal work counted on the reference path over every stride-th
+    pixel of a pixel range (all samples of each pixel)."""
     lib = _load()
     p = params if params is not None else scene.params
     a, b = (0, p.width * p.height) if pixels is None else pixels
-    out = np.zeros(8, np.int64)
-    if lib.oracle_path_stats(C.byref(scene.view), C.byref(p), a, b, threads, _p(out, C.c_int64)):
+    out = np.zeros(13, np.int64)
+    if lib.oracle_path_stats(C.byref(scene.view), C.byref(p), a, b, stride, threads, _p(out, C.c_int64)):
         raise RuntimeError("oracle_path_stats failed")
     return dict(zip(STAT_NAMES, (int(x) for x in out)))
 
 
+def ray_bytes(stats: dict, kind: str) -> float:
+    """Algorithmic bytes per ray of one kind ("closest" or "shadow"): 32 per
+    node test, 64 per instance entry, 16/48/36 per sphere/quad/triangle test,
+    plus the 128-byte queue record read and written."""
+    rays = stats[kind + "_rays"]
+    if rays == 0:
+        return 0.0
+    b = (32.0 * stats[kind + "_nodes"] + 64.0 * stats[kind + "_instances"] + 16.0 * stats[kind + "_spheres"]
+         + 48.0 * stats[kind + "_quads"] + 36.0 * stats[kind + "_triangles"])
+    return b / rays + 128.0
+
+
 def algorithmic_bytes(stats: dict) -> float:
-    """Traversal bytes per the roofline definition: 32 per node test, 64 per
-    instance entry, 16/48/36 per sphere/quad/triangle test, plus a 128-byte
-    ray-queue record read and written per ray."""
-    rays = stats["closest_rays"] + stats["shadow_rays"]
-    return (32.0 * stats["node_tests"] + 64.0 * stats["instance_entries"] + 16.0 * stats["sphere_tests"]
-            + 48.0 * stats["quad_tests"] + 36.0 * stats["triangle_tests"] + 128.0 * rays)
+    """Total traversal bytes of the counted paths (both ray kinds)."""
+    return ray_bytes(stats, "closest") * stats["closest_rays"] + ray_bytes(stats, "shadow") * stats["shadow_rays"]

@@ -352,8 +352,11 @@ struct Ray {
     V3 o, d;
     Spec lambdas;
 };
+// 0 paths, 1 closest rays, 2 shadow rays; 3-7 closest-hit work and 8-12
+// shadow work: node tests, instance entries, sphere, quad, triangle tests.
 struct Counters {
-    int64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t c[13] = {0};
+    int base = 3;  // 3 while tracing a closest-hit ray, 8 for a shadow ray
 };
 
 struct SceneData {
@@ -474,7 +477,7 @@ struct SceneData {
     bool intersect(int i, const Ray &r, Hit &h, Counters *cnt) const {
         const Object &o = objects[i];
         if (o.type == LAMBDA_OBJECT_SPHERE) {
-            if (cnt) cnt->c[5]++;
+            if (cnt) cnt->c[cnt->base + 2]++;
             V3 oc = r.o - o.a;
             float a = len2(r.d);
             float b = 2 * dot(r.d, oc);
@@ -498,7 +501,7 @@ struct SceneData {
             return true;
         }
         if (o.type == LAMBDA_OBJECT_QUAD) {
-            if (cnt) cnt->c[6]++;
+            if (cnt) cnt->c[cnt->base + 3]++;
             V3 n = normalize(o.n);
             float den = dot(n, r.d);
             if (fabsf(den) < EPS) return false;
@@ -519,7 +522,7 @@ struct SceneData {
             if (!h.front) h.normal = h.normal * -1;
             return true;
         }
-        if (cnt) cnt->c[7]++;
+        if (cnt) cnt->c[cnt->base + 4]++;
         V3 rcv = cross(r.d, o.v);
         float det = dot(o.h, rcv);
         if (fabsf(det) < EPS2) return false;
@@ -652,7 +655,7 @@ struct SceneData {
         bool found = false;
         if (occ_inst != -1) {
             const Instance &in = instances[occ_inst];
-            if (cnt) cnt->c[4]++;
+            if (cnt) cnt->c[cnt->base + 1]++;
             Ray lr = to_local(in, r);
             if (!intersect(occ_obj, lr, h, cnt)) return true;
             h.instance = occ_inst;
@@ -662,18 +665,18 @@ struct SceneData {
         while (sp > 0) {
             int cur = stack[--sp];
             float tn;
-            if (cnt) cnt->c[3]++;
+            if (cnt) cnt->c[cnt->base + 0]++;
             if (!slab(nodes[cur], r, tn) || tn > h.t) continue;
             if (nodes[cur].count > 0) {
                 int ii = nodes[cur].index;
                 const Instance &in = instances[ii];
-                if (cnt) cnt->c[4]++;
+                if (cnt) cnt->c[cnt->base + 1]++;
                 sub[ssp++] = in.node;
                 Ray lr = to_local(in, r);
                 while (ssp > 0) {
                     int sc = sub[--ssp];
                     float stn;
-                    if (cnt) cnt->c[3]++;
+                    if (cnt) cnt->c[cnt->base + 0]++;
                     if (!slab(nodes[sc], lr, stn) || stn > h.t) continue;
                     if (nodes[sc].count > 0) {
                         int first = nodes[sc].index;
@@ -842,7 +845,10 @@ struct SceneData {
         const int nl = sv->num_lights;
         for (int i = 0; i <= depth; i++) {
             Hit h;
-            if (cnt) cnt->c[1]++;
+            if (cnt) {
+                cnt->c[1]++;
+                cnt->base = 3;
+            }
             if (!hit(r, h, -1, -2, cnt)) {
                 float w = 1;
                 if (!specular && total > 0)
@@ -885,7 +891,10 @@ struct SceneData {
                     lprob *= lp / total;
                     Ray sr{h.point, ldir, r.lambdas};
                     Hit sh;
-                    if (cnt) cnt->c[2]++;
+                    if (cnt) {
+                        cnt->c[2]++;
+                        cnt->base = 8;
+                    }
                     if (!hit(sr, sh, li, lo, cnt)) {
                         Spec em = li != -1 ? emission(material_of(instances[li], lrel), sh, sr.lambdas) : background(sr);
                         Spec att = evaluate_lambertian(mat, h, sr);
@@ -1025,24 +1034,26 @@ void oracle_pcg32(uint64_t seed, uint64_t stream, int32_t n, uint32_t *out) {
 }
 
 int oracle_path_stats(const lambda_scene_view *scene, const lambda_render_params *p, int64_t pixel_begin,
-                      int64_t pixel_end, int32_t threads, int64_t *counts) {
-    if (!scene_ok(scene) || !p || !counts) return 1;
+                      int64_t pixel_end, int64_t pixel_stride, int32_t threads, int64_t *counts) {
+    if (!scene_ok(scene) || !p || !counts || pixel_stride < 1) return 1;
     SceneData S(scene);
     if (threads > 0) omp_set_num_threads(threads);
-    int64_t total[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t total[13] = {0};
+    const int64_t n = (pixel_end - pixel_begin + pixel_stride - 1) / pixel_stride;
 #pragma omp parallel
     {
         Counters local;
 #pragma omp for schedule(guided)
-        for (int64_t k = pixel_begin; k < pixel_end; k++) {
+        for (int64_t i = 0; i < n; i++) {
+            int64_t k = pixel_begin + i * pixel_stride;
             int px = int(k % p->width), py = int(k / p->width);
             Rng rng(p->seed, uint64_t(k));
             S.render_pixel(*p, px, py, rng, &local);
         }
 #pragma omp critical
-        for (int i = 0; i < 8; i++) total[i] += local.c[i];
+        for (int i = 0; i < 13; i++) total[i] += local.c[i];
     }
-    for (int i = 0; i < 8; i++) counts[i] = total[i];
+    for (int i = 0; i < 13; i++) counts[i] = total[i];
     return 0;
 }
 

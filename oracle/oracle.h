@@ -45,13 +45,15 @@ int oracle_thinfilm(const lambda_scene_view *scene, int32_t material, const floa
 /* n PCG32 outputs of stream (seed, stream) (math/random.h:9-31). */
 void oracle_pcg32(uint64_t seed, uint64_t stream, int32_t n, uint32_t *out);
 
-/* Per-path traversal work of the reference path, counted over the pixels
- * [pixel_begin, pixel_end) of the image with the reference RNG streams:
- * counts[0] paths, [1] closest-hit rays, [2] shadow rays, [3] node tests,
- * [4] instance entries (instance-level leaves plus shadow pre-intersections),
- * [5] sphere tests, [6] quad tests, [7] triangle tests. */
+/* Traversal work of the reference path over every pixel_stride-th pixel of
+ * [pixel_begin, pixel_end), reference RNG streams, all samples:
+ * counts[0] paths, [1] closest-hit rays, [2] shadow rays;
+ * counts[3..7] closest-hit work and counts[8..12] shadow-ray work, each as
+ * node tests, instance entries (instance-level leaves, plus the light
+ * pre-intersection of shadow rays), sphere, quad and triangle tests. */
 int oracle_path_stats(const lambda_scene_view *scene, const lambda_render_params *params,
-                      int64_t pixel_begin, int64_t pixel_end, int32_t threads, int64_t *counts);
+                      int64_t pixel_begin, int64_t pixel_end, int64_t pixel_stride, int32_t threads,
+                      int64_t *counts);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,179 @@
+"""Writes the golden vectors under tests/golden/ from the reference itself.
+
+Runs oracle/_ref/ref_harness (built by `make -C oracle ref` from the
+reference sources) on small scenes generated by lambda_amd.scenes, and
+stores only inputs and outputs:
+
+  scenes/*.lrd, scenes/*.obj   the scene files (inputs)
+  <scene>_scene.npz            the reference's parsed arrays after its BVH build
+  <scene>_render.npy           the reference's float RGB image (its own spp)
+  <scene>_hits.npz             closest-hit and occlusion queries and results
+  thinfilm.npz                 thin-film reflectance queries and results
+  rng.npz                      PCG32 draws and randomUnitVector outputs
+  box_4096spp.npz              128x128 closed-box render at 4096 spp
+
+Usage: python tests/golden/make_golden.py [--skip-4096]
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import ref_dump  # noqa: E402
+from lambda_amd import scenes  # noqa: E402
+
+SCENE_DIR = os.path.join(HERE, "scenes")
+
+
+def make_scenes() -> dict:
+    os.makedirs(SCENE_DIR, exist_ok=True)
+    cwd = os.getcwd()
+    os.chdir(SCENE_DIR)  # mesh paths in the .lrd files are relative to here
+    try:
+        out = {
+            "box": scenes.write_closed_box(".", 64, 64, 8, 16, name="box"),
+            "film": scenes.write_thinfilm_grid(".", 48, 48, 12, 16, name="film"),
+            "mesh": scenes.write_instanced_mesh(".", 48, 27, 10, 4, level=3, grid=3, name="mesh"),
+            "room": scenes.write_guiding_room(".", 48, 27, 16, 4, clutter=40, name="room"),
+        }
+    finally:
+        os.chdir(cwd)
+    return {k: os.path.join(SCENE_DIR, os.path.basename(v)) for k, v in out.items()}
+
+
+def ray_set(arrs: dict, rng: np.random.Generator, n: int) -> np.ndarray:
+    """Camera rays, rays between random points in the scene box, and rays
+    aimed at object centres (many of them graze edges and corners)."""
+    cam = arrs["camera"].reshape(4, 3)
+    lo = arrs["node_bounds"][0, :3]
+    hi = arrs["node_bounds"][0, 3:]
+    k = n // 3
+    uv = rng.random((k, 2)).astype(np.float32)
+    tgt = cam[1] + cam[2] * uv[:, :1] + cam[3] * uv[:, 1:2]
+    d0 = tgt - cam[0]
+    r0 = np.concatenate([np.repeat(cam[:1], k, 0), d0 / np.linalg.norm(d0, axis=1, keepdims=True)], 1)
+    p = lo + (hi - lo) * rng.random((k, 3))
+    q = lo + (hi - lo) * rng.random((k, 3))
+    d1 = q - p
+    r1 = np.concatenate([p, d1 / np.linalg.norm(d1, axis=1, keepdims=True)], 1)
+    m = n - 2 * k
+    geo = arrs["object_geometry"]
+    inst = arrs["instance_ints"]
+    mats = arrs["instance_matrices"]
+    pick = rng.integers(0, len(inst), m)
+    centers = []
+    for ii in pick:
+        first, count = inst[ii, 0], inst[ii, 1]
+        obj = first + rng.integers(0, count)
+        g = geo[obj]
+        c = g[:3] + (0.3 * g[3:6] + 0.3 * g[6:9] if arrs["object_type"][obj] != 0 else 0)
+        M = mats[ii, :16].reshape(4, 4)
+        w = M @ np.append(c, 1.0)
+        centers.append(w[:3] / w[3])
+    centers = np.array(centers)
+    src = lo + (hi - lo) * rng.random((m, 3))
+    d2 = centers - src
+    r2 = np.concatenate([src, d2 / np.linalg.norm(d2, axis=1, keepdims=True)], 1)
+    return np.concatenate([r0, r1, r2]).astype(np.float32)
+
+
+def shadow_set(arrs: dict, hits: np.ndarray, rays: np.ndarray, rng: np.random.Generator):
+    """From each hit point, a ray toward a random light entry (the
+    environment included), as next-event estimation issues them."""
+    li_all = arrs["light_instances"]
+    lo_all = arrs["light_objects"]
+    ok = hits["hit"] == 1
+    pts = hits["point"][ok]
+    n = len(pts)
+    pick = rng.integers(0, len(li_all), n)
+    li = li_all[pick].astype(np.int32)
+    lo = lo_all[pick].astype(np.int32)
+    dirs = np.zeros((n, 3), np.float32)
+    geo = arrs["object_geometry"]
+    mats = arrs["instance_matrices"]
+    for i in range(n):
+        if li[i] == -1:
+            v = rng.normal(size=3)
+        else:
+            g = geo[lo[i]]
+            a, b = rng.random(2)
+            c = g[:3] + (a * g[3:6] + b * g[6:9] if arrs["object_type"][lo[i]] != 0 else 0)
+            M = mats[li[i], :16].reshape(4, 4)
+            w = M @ np.append(c, 1.0)
+            v = w[:3] / w[3] - pts[i]
+        dirs[i] = v / np.linalg.norm(v)
+    return np.concatenate([pts, dirs], 1).astype(np.float32), li, lo
+
+
+def main(argv) -> None:
+    if not ref_dump.available():
+        raise SystemExit("reference harness missing: run `make -C oracle ref` first")
+    paths = make_scenes()
+    tmp = tempfile.mkdtemp()
+    try:
+        rng = np.random.Generator(np.random.PCG64(2024))
+        for name, lrd in paths.items():
+            cwd = os.getcwd()
+            os.chdir(SCENE_DIR)
+            try:
+                arrs = ref_dump.dump_scene(lrd, os.path.join(tmp, "scene.bin"))
+                img, info = ref_dump.render(lrd, os.path.join(tmp, "img.f32"))
+                rays = ray_set(arrs, rng, 3000)
+                hits = ref_dump.hits(lrd, rays, None, tmp)
+                srays, li, lo = shadow_set(arrs, hits, rays, rng)
+                occ = ref_dump.hits(lrd, srays, np.stack([li, lo], 1), tmp)["hit"].astype(np.int32)
+            finally:
+                os.chdir(cwd)
+            flat = {k: v for k, v in arrs.items() if isinstance(v, np.ndarray)}
+            flat["render_int"] = np.array([arrs["render"][k] for k in ("width", "height", "depth", "samples", "sqrt_samples")], np.int32)
+            flat["render_float"] = np.array([arrs["render"]["lambda_min"], arrs["render"]["lambda_max"]], np.float32)
+            flat["total_light_power"] = np.float32(arrs["total_light_power"])
+            flat["background_texture"] = np.int32(arrs["background_texture"])
+            np.savez_compressed(os.path.join(HERE, name + "_scene.npz"), **flat)
+            np.save(os.path.join(HERE, name + "_render.npy"),
+                    img.reshape(arrs["render"]["height"], arrs["render"]["width"], 3))
+            np.savez_compressed(os.path.join(HERE, name + "_hits.npz"), rays=rays, hits=hits,
+                                shadow_rays=srays, light_instance=li, light_object=lo, occluded=occ)
+            print(name, "objects", len(arrs["object_type"]), "hits", int(hits["hit"].sum()), "/", len(hits),
+                  "occluded", int(occ.sum()), "/", len(occ), "render %.3fs" % info["render_s"])
+
+        # Thin film: the film scene's materials 1 (perlin) and 2 (worley).
+        os.chdir(SCENE_DIR)
+        q = []
+        for k in range(4000):
+            c = float(rng.random()) if k % 10 else 1.0
+            lam = 360 + 470 * float(rng.random())
+            front = float(k % 2)
+            p = rng.random(3) * 2 - 1
+            q.append([c, lam, front, p[0], p[1], p[2], 0])
+        q = np.array(q, np.float32)
+        tf = {}
+        for m in (1, 2):
+            tf["R%d" % m] = ref_dump.thinfilm(paths["film"], m, q, tmp)
+        np.savez_compressed(os.path.join(HERE, "thinfilm.npz"), queries=q, **tf)
+
+        draws = {"pcg_42_54": ref_dump.pcg(42, 54, 6), "pcg_7_123456789": ref_dump.pcg(7, 123456789, 64),
+                 "unitvec_3_9": ref_dump.unit_vectors(3, 9, 16)}
+        np.savez_compressed(os.path.join(HERE, "rng.npz"), **draws)
+
+        if "--skip-4096" not in argv:
+            big = scenes.write_closed_box(tmp, 128, 128, 8, 4096, name="box_4096spp")
+            img, info = ref_dump.render(big, os.path.join(tmp, "big.f32"))
+            np.savez_compressed(os.path.join(HERE, "box_4096spp.npz"), image=img.reshape(128, 128, 3),
+                                lrd=np.frombuffer(open(big, "rb").read(), np.uint8))
+            print("4096 spp reference render %.1fs on %d threads" % (info["render_s"], info["threads"]))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
