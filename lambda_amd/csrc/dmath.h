@@ -11,6 +11,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Callable from host code too, so the device arithmetic can be exercised on
+// the CPU by test programs.
+#define LAM_FN __host__ __device__ __forceinline__
+
 namespace lam {
 
 constexpr float kEps = float(1e-4);                       // core/constants.h:5
@@ -26,52 +30,52 @@ struct V3 {
     float x, y, z;
 };
 
-__device__ __forceinline__ V3 v3(float a, float b, float c) { return V3{a, b, c}; }
-__device__ __forceinline__ V3 v3(float4 a) { return V3{a.x, a.y, a.z}; }
-__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
-__device__ __forceinline__ V3 operator*(V3 a, float d) { return v3(a.x * d, a.y * d, a.z * d); }
-__device__ __forceinline__ V3 operator*(float d, V3 a) { return a * d; }
-__device__ __forceinline__ V3 operator/(V3 a, float d) { return v3(a.x / d, a.y / d, a.z / d); }
-__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float len2(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
-__device__ __forceinline__ float len(V3 a) { return sqrtf(len2(a)); }
-__device__ __forceinline__ V3 normalize(V3 a) {
+LAM_FN V3 v3(float a, float b, float c) { return V3{a, b, c}; }
+LAM_FN V3 v3(float4 a) { return V3{a.x, a.y, a.z}; }
+LAM_FN V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+LAM_FN V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+LAM_FN V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+LAM_FN V3 operator*(V3 a, float d) { return v3(a.x * d, a.y * d, a.z * d); }
+LAM_FN V3 operator*(float d, V3 a) { return a * d; }
+LAM_FN V3 operator/(V3 a, float d) { return v3(a.x / d, a.y / d, a.z / d); }
+LAM_FN float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+LAM_FN float len2(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+LAM_FN float len(V3 a) { return sqrtf(len2(a)); }
+LAM_FN V3 normalize(V3 a) {
     float l = len(a);
     return v3(a.x / l, a.y / l, a.z / l);
 }
-__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+LAM_FN V3 cross(V3 a, V3 b) {
     return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ __forceinline__ float comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+LAM_FN float comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 
 // Four hero-wavelength samples (math/spectrum.h:8-156).
 struct S4 {
     float v[kHero];
 };
-__device__ __forceinline__ S4 s4(float d) { return S4{{d, d, d, d}}; }
-__device__ __forceinline__ S4 s4(float4 a) { return S4{{a.x, a.y, a.z, a.w}}; }
-__device__ __forceinline__ float4 f4(const S4 &a) { return make_float4(a.v[0], a.v[1], a.v[2], a.v[3]); }
-__device__ __forceinline__ S4 operator*(const S4 &a, const S4 &b) {
+LAM_FN S4 s4(float d) { return S4{{d, d, d, d}}; }
+LAM_FN S4 s4(float4 a) { return S4{{a.x, a.y, a.z, a.w}}; }
+LAM_FN float4 f4(const S4 &a) { return make_float4(a.v[0], a.v[1], a.v[2], a.v[3]); }
+LAM_FN S4 operator*(const S4 &a, const S4 &b) {
     S4 r;
 #pragma unroll
     for (int i = 0; i < kHero; i++) r.v[i] = a.v[i] * b.v[i];
     return r;
 }
-__device__ __forceinline__ S4 operator*(const S4 &a, float d) {
+LAM_FN S4 operator*(const S4 &a, float d) {
     S4 r;
 #pragma unroll
     for (int i = 0; i < kHero; i++) r.v[i] = a.v[i] * d;
     return r;
 }
-__device__ __forceinline__ S4 operator/(const S4 &a, float d) {
+LAM_FN S4 operator/(const S4 &a, float d) {
     S4 r;
 #pragma unroll
     for (int i = 0; i < kHero; i++) r.v[i] = a.v[i] / d;
     return r;
 }
-__device__ __forceinline__ float s4max(const S4 &a) {
+LAM_FN float s4max(const S4 &a) {
     float m = a.v[0];
 #pragma unroll
     for (int i = 1; i < kHero; i++)
@@ -82,7 +86,7 @@ __device__ __forceinline__ float s4max(const S4 &a) {
 // PCG32 XSH-RR (O'Neill); seeding as pcg32_srandom_r (math/random.h:9-31).
 struct Pcg {
     uint64_t state, inc;
-    __device__ __forceinline__ uint32_t step() {
+    LAM_FN uint32_t step() {
         uint64_t old = state;
         state = old * 6364136223846793005ULL + inc;
         uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u);
@@ -90,9 +94,9 @@ struct Pcg {
         return (xs >> rot) | (xs << ((-rot) & 31));
     }
     // u32 / 2^32 rounded to float; may round up to 1.0 like the reference.
-    __device__ __forceinline__ float next() { return float(step()) / float(4294967296.0); }
+    LAM_FN float next() { return float(step()) / float(4294967296.0); }
 };
-__device__ __forceinline__ Pcg pcg_seed(uint64_t seed, uint64_t stream) {
+LAM_FN Pcg pcg_seed(uint64_t seed, uint64_t stream) {
     Pcg r;
     r.state = 0;
     r.inc = (stream << 1u) | 1u;
@@ -102,19 +106,19 @@ __device__ __forceinline__ Pcg pcg_seed(uint64_t seed, uint64_t stream) {
     return r;
 }
 
-__device__ __forceinline__ float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+LAM_FN float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
 // Normalised uniform cube sample; the three draws land in z, y, x in that
 // order, as the reference's compiled argument evaluation does
 // (core/utils.h:20-24).
-__device__ __forceinline__ V3 random_unit_vector(Pcg &r) {
+LAM_FN V3 random_unit_vector(Pcg &r) {
     float c = r.next() * 2 - 1;
     float b = r.next() * 2 - 1;
     float a = r.next() * 2 - 1;
     return normalize(v3(a, b, c));
 }
 
-__device__ __forceinline__ void onb(V3 n, V3 &u, V3 &v, V3 &w) {
+LAM_FN void onb(V3 n, V3 &u, V3 &v, V3 &w) {
     w = normalize(n);
     u = fabsf(w.x) > fabsf(w.y) ? v3(0, 0, 1) : v3(1, 0, 0);
     u = normalize(cross(u, w));
@@ -122,7 +126,7 @@ __device__ __forceinline__ void onb(V3 n, V3 &u, V3 &v, V3 &w) {
 }
 
 // Cosine-weighted direction about n (core/utils.h:26-44).
-__device__ __forceinline__ V3 random_in_hemisphere(V3 n, Pcg &r) {
+LAM_FN V3 random_in_hemisphere(V3 n, Pcg &r) {
     float r1 = r.next();
     float r2 = r.next();
     float phi = 2 * kPi * r1;
@@ -137,7 +141,7 @@ __device__ __forceinline__ V3 random_in_hemisphere(V3 n, Pcg &r) {
 
 // Uniform direction in the cone of half-angle acos(cos_max) about n
 // (core/utils.h:46-65).
-__device__ __forceinline__ V3 random_in_cone(V3 n, float cos_max, Pcg &r) {
+LAM_FN V3 random_in_cone(V3 n, float cos_max, Pcg &r) {
     float r1 = r.next();
     float r2 = r.next();
     float ct = 1 + r1 * (cos_max - 1);
@@ -150,15 +154,15 @@ __device__ __forceinline__ V3 random_in_cone(V3 n, float cos_max, Pcg &r) {
     return x * u + y * v + ct * w;
 }
 
-__device__ __forceinline__ float power_heuristic(float a, float b) {
+LAM_FN float power_heuristic(float a, float b) {
     a *= a;
     b *= b;
     return a / (a + b);
 }
 
-__device__ __forceinline__ V3 reflected(V3 v, V3 n) { return v - 2 * dot(v, n) * n; }
+LAM_FN V3 reflected(V3 v, V3 n) { return v - 2 * dot(v, n) * n; }
 
-__device__ __forceinline__ V3 refracted(V3 v, V3 n, float ratio) {
+LAM_FN V3 refracted(V3 v, V3 n, float ratio) {
     float vn = dot(v, n);
     float root = 1 - ratio * ratio * (1 - vn * vn);
     if (root < 0) return v - 2 * vn * n;
@@ -168,7 +172,7 @@ __device__ __forceinline__ V3 refracted(V3 v, V3 n, float ratio) {
 // Row-major 4x4 homogeneous transform with the divide by the fourth row,
 // replaced by 1 when it is below 1e-4 in magnitude (core/utils.h:89-95).
 // rows[k] holds matrix row k.
-__device__ __forceinline__ V3 xform(const float4 *rows, V3 v, float w) {
+LAM_FN V3 xform(const float4 *rows, V3 v, float w) {
     float4 r0 = rows[0], r1 = rows[1], r2 = rows[2], r3 = rows[3];
     float den = r3.x * v.x + r3.y * v.y + r3.z * v.z + r3.w * w;
     if (fabsf(den) < kEps) den = 1;
@@ -178,7 +182,7 @@ __device__ __forceinline__ V3 xform(const float4 *rows, V3 v, float w) {
 }
 // The same with the transposed matrix: column k plays the role of row k
 // (normals use the transposed inverse, scene/instance.h:48).
-__device__ __forceinline__ V3 xform_t(const float4 *rows, V3 v, float w) {
+LAM_FN V3 xform_t(const float4 *rows, V3 v, float w) {
     float4 r0 = rows[0], r1 = rows[1], r2 = rows[2], r3 = rows[3];
     float den = r0.w * v.x + r1.w * v.y + r2.w * v.z + r3.w * w;
     if (fabsf(den) < kEps) den = 1;
@@ -189,8 +193,8 @@ __device__ __forceinline__ V3 xform_t(const float4 *rows, V3 v, float w) {
 
 // ---------------------------------------------------------------- noise
 // Improved Perlin noise remapped to [0,1] (core/utils.h:97-148).
-__device__ __forceinline__ float fade(float t) { return powf(t, 3.0f) * (t * (6 * t - 15) + 10); }
-__device__ __forceinline__ float grad(int h, float x, float y, float z) {
+LAM_FN float fade(float t) { return powf(t, 3.0f) * (t * (6 * t - 15) + 10); }
+LAM_FN float grad(int h, float x, float y, float z) {
     switch (h % 12) {
         case 0: return x + y;
         case 1: return -x + y;
@@ -206,9 +210,9 @@ __device__ __forceinline__ float grad(int h, float x, float y, float z) {
         default: return -y - z;
     }
 }
-__device__ __forceinline__ float lerp(float a, float b, float x) { return a + x * (b - a); }
+LAM_FN float lerp(float a, float b, float x) { return a + x * (b - a); }
 
-__device__ __forceinline__ float perlin(const int *perm, V3 p) {
+LAM_FN float perlin(const int *perm, V3 p) {
     float fx = floorf(p.x), fy = floorf(p.y), fz = floorf(p.z);
     int i = (int)fx & 255, j = (int)fy & 255, k = (int)fz & 255;
     float tx = p.x - fx, ty = p.y - fy, tz = p.z - fz;
@@ -227,11 +231,11 @@ __device__ __forceinline__ float perlin(const int *perm, V3 p) {
     return (lerp(y1, y2, w) + 1) / 2;
 }
 
-__device__ __forceinline__ float fract(float x) { return x - floorf(x); }
+LAM_FN float fract(float x) { return x - floorf(x); }
 
 // Cellular (F1) distance to one hashed feature point per unit cell over the
 // 27 cells around p (core/utils.h:150-171).
-__device__ __forceinline__ float worley(V3 p) {
+LAM_FN float worley(V3 p) {
     int xi = int(floorf(p.x)), yi = int(floorf(p.y)), zi = int(floorf(p.z));
     float d = 9999;
     const V3 k1 = v3(float(127.1), float(311.7), float(74.7));

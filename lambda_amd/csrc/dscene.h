@@ -15,6 +15,8 @@
 #pragma once
 
 #include "dmath.h"
+#include "ffloat.h"
+#include "lambda_hip.h"
 
 namespace lam {
 
@@ -50,7 +52,7 @@ struct Hit {
     bool front;
 };
 
-__device__ __forceinline__ float dense(const float *s, float lambda) {
+LAM_FN float dense(const float *s, float lambda) {
     // 1 nm table, linear interpolation (math/spectrum.h:170-188) in float.
     float index = lambda - 360;
     int lo = int(index);
@@ -59,7 +61,7 @@ __device__ __forceinline__ float dense(const float *s, float lambda) {
     return a + (index - float(lo)) * (b - a);
 }
 
-__device__ __forceinline__ float2 dense_c(const float2 *s, float lambda) {
+LAM_FN float2 dense_c(const float2 *s, float lambda) {
     float index = lambda - 360;
     int lo = int(index);
     if (lo >= kBins - 1) return s[kBins - 1];
@@ -74,7 +76,7 @@ __device__ __forceinline__ float2 dense_c(const float2 *s, float lambda) {
 // candidate distance only; hit_record() rebuilds the full record for the
 // winning primitive with the same arithmetic.
 
-__device__ __forceinline__ bool test_sphere(float4 a, V3 o, V3 d, float best, float &t_out) {
+LAM_FN bool test_sphere(float4 a, V3 o, V3 d, float best, float &t_out) {
     V3 oc = o - v3(a);
     float A = len2(d);
     float B = 2 * dot(d, oc);
@@ -91,7 +93,7 @@ __device__ __forceinline__ bool test_sphere(float4 a, V3 o, V3 d, float best, fl
     return true;
 }
 
-__device__ __forceinline__ bool test_quad(V3 c, V3 h, V3 v, V3 nrm, V3 o, V3 d, float best, float &t_out,
+LAM_FN bool test_quad(V3 c, V3 h, V3 v, V3 nrm, V3 o, V3 d, float best, float &t_out,
                                           float &alpha, float &beta) {
     V3 n = normalize(nrm);
     float den = dot(n, d);
@@ -107,7 +109,7 @@ __device__ __forceinline__ bool test_quad(V3 c, V3 h, V3 v, V3 nrm, V3 o, V3 d, 
     return true;
 }
 
-__device__ __forceinline__ bool test_tri(V3 c, V3 h, V3 v, V3 o, V3 d, float best, float &t_out, float &alpha,
+LAM_FN bool test_tri(V3 c, V3 h, V3 v, V3 o, V3 d, float best, float &t_out, float &alpha,
                                          float &beta) {
     V3 rcv = cross(d, v);
     float det = dot(h, rcv);
@@ -126,7 +128,7 @@ __device__ __forceinline__ bool test_tri(V3 c, V3 h, V3 v, V3 o, V3 d, float bes
 }
 
 // Test of object k against the (local) ray; updates best on a closer hit.
-__device__ __forceinline__ bool test_object(const DeviceScene &S, int k, V3 o, V3 d, float &best) {
+LAM_FN bool test_object(const DeviceScene &S, int k, V3 o, V3 d, float &best) {
     float4 a = S.obj_a[k];
     float4 b = S.obj_b[k];
     int type = __float_as_int(b.w);
@@ -148,7 +150,7 @@ __device__ __forceinline__ bool test_object(const DeviceScene &S, int k, V3 o, V
 
 // Full intersection record of object k for a local-space ray, as the
 // reference's intersect* fills it (scene/object.h:301-400).
-__device__ __forceinline__ bool object_record(const DeviceScene &S, int k, V3 o, V3 d, float best, Hit &h) {
+LAM_FN bool object_record(const DeviceScene &S, int k, V3 o, V3 d, float best, Hit &h) {
     float4 a = S.obj_a[k];
     float4 b = S.obj_b[k];
     int type = __float_as_int(b.w);
@@ -192,7 +194,7 @@ __device__ __forceinline__ bool object_record(const DeviceScene &S, int k, V3 o,
     return true;
 }
 
-__device__ __forceinline__ void local_ray(const DeviceScene &S, int inst, V3 o, V3 d, V3 &lo, V3 &ld) {
+LAM_FN void local_ray(const DeviceScene &S, int inst, V3 o, V3 d, V3 &lo, V3 &ld) {
     const float4 *m = S.inst + 8 * size_t(inst);
     lo = xform(m + 4, o, 1);
     ld = xform(m + 4, d, 0);
@@ -200,7 +202,7 @@ __device__ __forceinline__ void local_ray(const DeviceScene &S, int inst, V3 o, 
 
 // World-space record of (instance, object) for a world ray: local test, then
 // point by M and normal by the transposed inverse (scene/scene.h:101-104).
-__device__ __forceinline__ bool hit_record(const DeviceScene &S, int inst, int obj, V3 o, V3 d, Hit &h) {
+LAM_FN bool hit_record(const DeviceScene &S, int inst, int obj, V3 o, V3 d, Hit &h) {
     V3 lo, ld;
     local_ray(S, inst, o, d, lo, ld);
     if (!object_record(S, obj, lo, ld, kMax, h)) return false;
@@ -224,7 +226,7 @@ __device__ __forceinline__ bool hit_record(const DeviceScene &S, int inst, int o
 // lane] for the instance level and stk[(kStack + k) * stride + lane] for the
 // object level, so a wave's pushes and pops hit 64 distinct banks.
 
-__device__ __forceinline__ bool slab(float4 lo, float4 hi, V3 o, V3 inv, float &tnear) {
+LAM_FN bool slab(float4 lo, float4 hi, V3 o, V3 inv, float &tnear) {
     float t1 = 0, t2 = kMax;
     float a = (lo.x - o.x) * inv.x, b = (hi.x - o.x) * inv.x;
     if (inv.x < 0) { float s = a; a = b; b = s; }
@@ -253,7 +255,7 @@ struct TraceStats {
 // kAny = true: occlusion; best is the light distance, (skip_inst, skip_obj)
 //   the light itself; returns true on the first blocking hit.
 template <bool kAny, bool kStats>
-__device__ __forceinline__ bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_inst,
+LAM_FN bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_inst,
                                          int &hit_obj, int skip_inst, int skip_obj, int *stk, int stride,
                                          TraceStats &st) {
     const V3 inv = v3(1 / d.x, 1 / d.y, 1 / d.z);
@@ -317,7 +319,7 @@ __device__ __forceinline__ bool traverse(const DeviceScene &S, V3 o, V3 d, float
 
 // --------------------------------------------------------------- textures
 
-__device__ __forceinline__ float scalar_texture(const DeviceScene &S, int t, V3 p) {
+LAM_FN float scalar_texture(const DeviceScene &S, int t, V3 p) {
     float4 x = S.stex[t];
     int type = __float_as_int(x.x);
     if (type == LAMBDA_SCALAR_CONSTANT) return x.y;
@@ -327,7 +329,7 @@ __device__ __forceinline__ float scalar_texture(const DeviceScene &S, int t, V3 
 
 // Spectrum texture at one wavelength (scene/texture.h:178-210).  Checker
 // cells are in (u, v) units.
-__device__ __forceinline__ float spectrum_texture(const DeviceScene &S, int t, const Hit &h, float lambda) {
+LAM_FN float spectrum_texture(const DeviceScene &S, int t, const Hit &h, float lambda) {
     int4 x = S.xtex[t];
     if (x.x == LAMBDA_SPECTRUM_CONSTANT) return dense(S.spectra + size_t(x.y) * kBins, lambda);
     if (x.x == LAMBDA_SPECTRUM_CHECKER) {
@@ -341,11 +343,11 @@ __device__ __forceinline__ float spectrum_texture(const DeviceScene &S, int t, c
 }
 
 // Texture that does not depend on the wavelength: evaluated once per vertex.
-__device__ __forceinline__ bool spectrum_texture_is_scalar(const DeviceScene &S, int t) {
+LAM_FN bool spectrum_texture_is_scalar(const DeviceScene &S, int t) {
     return S.xtex[t].x == LAMBDA_SPECTRUM_SCALAR;
 }
 
-__device__ __forceinline__ S4 spectrum_texture4(const DeviceScene &S, int t, const Hit &h, const S4 &lam) {
+LAM_FN S4 spectrum_texture4(const DeviceScene &S, int t, const Hit &h, const S4 &lam) {
     S4 r;
     if (spectrum_texture_is_scalar(S, t)) {
         float v = scalar_texture(S, S.xtex[t].y, h.p);
@@ -358,7 +360,7 @@ __device__ __forceinline__ S4 spectrum_texture4(const DeviceScene &S, int t, con
 
 // Environment lookup by direction: equirectangular (u, v), point = direction
 // (scene/background.h:81-95).
-__device__ __forceinline__ S4 background(const DeviceScene &S, V3 d, const S4 &lam) {
+LAM_FN S4 background(const DeviceScene &S, V3 d, const S4 &lam) {
     Hit h;
     h.p = d;
     h.u = (atan2f(d.z, d.x) + kPi) / (2 * kPi);
@@ -366,7 +368,7 @@ __device__ __forceinline__ S4 background(const DeviceScene &S, V3 d, const S4 &l
     return spectrum_texture4(S, S.bg_tex, h, lam);
 }
 
-__device__ __forceinline__ float spectrum_average(const DeviceScene &S, int s) {
+LAM_FN float spectrum_average(const DeviceScene &S, int s) {
     const float *p = S.spectra + size_t(s) * kBins;
     float sum = 0;
     for (int i = 0; i < kBins; i++) sum += p[i];
@@ -374,7 +376,7 @@ __device__ __forceinline__ float spectrum_average(const DeviceScene &S, int s) {
 }
 
 // Mean emitted radiance of a spectrum texture (scene/texture.h:154-176).
-__device__ __forceinline__ float spectrum_texture_average(const DeviceScene &S, int t) {
+LAM_FN float spectrum_texture_average(const DeviceScene &S, int t) {
     int4 x = S.xtex[t];
     if (x.x == LAMBDA_SPECTRUM_CONSTANT) return spectrum_average(S, x.y);
     if (x.x == LAMBDA_SPECTRUM_CHECKER) return float(0.5) * (spectrum_average(S, x.y) + spectrum_average(S, x.z));
@@ -386,7 +388,7 @@ __device__ __forceinline__ float spectrum_texture_average(const DeviceScene &S, 
 // Light sampling on objects (scene/object.h:201-299) and instances
 // (scene/instance.h:103-111).
 
-__device__ __forceinline__ float object_area(const DeviceScene &S, int k) {
+LAM_FN float object_area(const DeviceScene &S, int k) {
     float4 a = S.obj_a[k];
     int type = __float_as_int(S.obj_b[k].w);
     if (type == LAMBDA_OBJECT_SPHERE) return 4 * kPi * a.w * a.w;
@@ -394,7 +396,7 @@ __device__ __forceinline__ float object_area(const DeviceScene &S, int k) {
     return type == LAMBDA_OBJECT_QUAD ? l : l / 2;
 }
 
-__device__ __forceinline__ float object_pdf(const DeviceScene &S, int k, V3 p, V3 d) {
+LAM_FN float object_pdf(const DeviceScene &S, int k, V3 p, V3 d) {
     float4 a = S.obj_a[k];
     float4 b = S.obj_b[k];
     int type = __float_as_int(b.w);
@@ -437,7 +439,7 @@ __device__ __forceinline__ float object_pdf(const DeviceScene &S, int k, V3 p, V
     return t * t / (fabsf(ct) * area);
 }
 
-__device__ __forceinline__ V3 object_sample(const DeviceScene &S, int k, V3 p, Pcg &rng) {
+LAM_FN V3 object_sample(const DeviceScene &S, int k, V3 p, Pcg &rng) {
     float4 a = S.obj_a[k];
     float4 b = S.obj_b[k];
     int type = __float_as_int(b.w);
@@ -459,141 +461,99 @@ __device__ __forceinline__ V3 object_sample(const DeviceScene &S, int k, V3 p, P
     return normalize(s - p);
 }
 
-__device__ __forceinline__ int material_of(const DeviceScene &S, int inst, int obj) {
+LAM_FN int material_of(const DeviceScene &S, int inst, int obj) {
     int m = S.inst_i[inst].w;
     return m >= 0 ? m : S.obj_mat[obj];
 }
 
-__device__ __forceinline__ float instance_area(const DeviceScene &S, int inst, int obj) {
+LAM_FN float instance_area(const DeviceScene &S, int inst, int obj) {
     const float4 *m = S.inst + 8 * size_t(inst);
     return object_area(S, obj) * len2(v3(m[0].x, m[1].x, m[2].x));
 }
 
-__device__ __forceinline__ float instance_pdf(const DeviceScene &S, int inst, int obj, V3 p, V3 d) {
+LAM_FN float instance_pdf(const DeviceScene &S, int inst, int obj, V3 p, V3 d) {
     const float4 *m = S.inst + 8 * size_t(inst);
     return object_pdf(S, obj, xform(m + 4, p, 1), normalize(xform(m + 4, d, 0)));
 }
 
-__device__ __forceinline__ V3 instance_sample(const DeviceScene &S, int inst, int obj, V3 p, Pcg &rng) {
+LAM_FN V3 instance_sample(const DeviceScene &S, int inst, int obj, V3 p, Pcg &rng) {
     const float4 *m = S.inst + 8 * size_t(inst);
     return normalize(xform(m, object_sample(S, obj, xform(m + 4, p, 1), rng), 0));
 }
 
-// --------------------------------------------------------- thin film (FP32)
+// --------------------------------------------------------------- thin film
 //
-// Transfer-matrix reflectance of a layer stack (scene/material.h:263-309,
-// core/utils.h:173-193) in single-precision complex arithmetic: interface
-// matrices D = (1/t) [[1, r], [r, 1]] for s and p polarisation, propagation
-// P = diag(exp(-i phi), exp(i phi)) with phi = n (2 pi / lambda) d cos theta,
-// M = D_0 P_1 D_1 ... ; R = (|M01/M00|^2_s + |M01/M00|^2_p) / 2.
+// Reflectance of a layer stack.
+//
+// Lambda forms R = (|M01/M00|^2_s + |M01/M00|^2_p) / 2 from interface
+// matrices D = (1/t) [[1, r], [r, 1]] and propagation matrices
+// P = diag(exp(-i phi), exp(i phi)), phi = n (2 pi / lambda) d cos theta
+// (scene/material.h:263-309, core/utils.h:173-193).  Because every factor is
+// symmetric, M01/M00 is the reflection coefficient of the same stack lit from
+// the far side.  That quantity is computed here with optical admittances,
+// starting from the near medium and climbing layer by layer:
+//   eta_s = n cos theta, eta_p = cos theta / n   (so r = (eta_a - eta_b) /
+//   (eta_a + eta_b) reproduces Lambda's r_s and r_p),
+//   Y <- eta (Y cos delta - i eta sin delta) / (eta cos delta - i Y sin delta),
+//   r = (eta_far - Y) / (eta_far + Y).
+// The two forms are algebraically identical (checked against an FP64
+// restatement of Lambda's matrices on random absorbing multilayers).
+//
+// Precision: near grazing incidence a lossless film is a sharp Fabry-Perot
+// resonance -- R swings from 0 to 1 while sin(delta) moves by about
+// 2 cos(theta) / n -- so the phase (up to ~20 rad) and the recursion need far
+// more than FP32's 24 bits to stay within 1e-5 of the FP64 value.  Both run
+// in float-float arithmetic (ffloat.h); angles follow Lambda's Snell
+// recurrence sin^2_{j+1} = (n_j / n_{j+1})^2 sin^2_j with principal complex
+// square roots, so total internal reflection takes the same branch.
 
-struct Cf {
-    float re, im;
-};
-__device__ __forceinline__ Cf cf(float r, float i = 0) { return Cf{r, i}; }
-__device__ __forceinline__ Cf operator+(Cf a, Cf b) { return cf(a.re + b.re, a.im + b.im); }
-__device__ __forceinline__ Cf operator-(Cf a, Cf b) { return cf(a.re - b.re, a.im - b.im); }
-__device__ __forceinline__ Cf operator*(Cf a, Cf b) { return cf(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
-__device__ __forceinline__ Cf operator*(Cf a, float s) { return cf(a.re * s, a.im * s); }
-// Scaled division (Smith): no overflow for large or small |b|.
-__device__ __forceinline__ Cf operator/(Cf a, Cf b) {
-    if (fabsf(b.re) >= fabsf(b.im)) {
-        float r = b.im / b.re;
-        float den = b.re + b.im * r;
-        return cf((a.re + a.im * r) / den, (a.im - a.re * r) / den);
-    }
-    float r = b.re / b.im;
-    float den = b.re * r + b.im;
-    return cf((a.re * r + a.im) / den, (a.im * r - a.re) / den);
-}
-__device__ __forceinline__ float cabs2(Cf a) { return a.re * a.re + a.im * a.im; }
-// Principal square root; the sign of a zero imaginary part selects the
-// branch on the negative real axis, as C99 csqrt does.
-__device__ __forceinline__ Cf csqrt(Cf z) {
-    float m = hypotf(z.re, z.im);
-    if (m == 0) return cf(0, z.im);
-    if (z.re >= 0) {
-        float t = sqrtf(0.5f * (m + z.re));
-        return cf(t, z.im / (2 * t));
-    }
-    float t = sqrtf(0.5f * (m - z.re));
-    return cf(fabsf(z.im) / (2 * t), copysignf(t, z.im));
-}
-__device__ __forceinline__ Cf cexpf(Cf z) {
-    float s, c;
-    sincosf(z.im, &s, &c);
-    float e = expf(z.re);
-    return cf(e * c, e * s);
-}
-
-struct M2 {
-    Cf a, b, c, d;  // [[a, b], [c, d]]
-};
-__device__ __forceinline__ M2 mul(const M2 &x, const M2 &y) {
-    return M2{x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d, x.c * y.a + x.d * y.c, x.c * y.b + x.d * y.d};
-}
-__device__ __forceinline__ M2 interface_s(Cf n1, Cf n2, Cf c1, Cf c2) {
-    Cf a = n1 * c1, b = n2 * c2;
-    Cf r = (a - b) / (a + b);
-    Cf t = (cf(2) * a) / (a + b);
-    Cf one = cf(1) / t;
-    Cf rt = r / t;
-    return M2{one, rt, rt, one};
-}
-__device__ __forceinline__ M2 interface_p(Cf n1, Cf n2, Cf c1, Cf c2) {
-    Cf a = n2 * c1, b = n1 * c2;
-    Cf r = (a - b) / (a + b);
-    Cf t = (cf(2) * (n1 * c1)) / (a + b);
-    Cf one = cf(1) / t;
-    Cf rt = r / t;
-    return M2{one, rt, rt, one};
-}
-
-// Reflectance of thin-film material `mat` at wavelength lambda for a ray of
-// direction d hitting a surface with normal n (facing the ray), at point p
-// (thickness textures are evaluated there).  thick[] carries the layer
-// thicknesses when the caller has already evaluated them.
-__device__ __forceinline__ float thinfilm_reflectance(const DeviceScene &S, int4 mt, bool front, V3 n, V3 d,
-                                                      float lambda, const float *thick) {
+// Reflectance of thin-film material mt = {type, layers, offset of n, offset
+// of d} at wavelength lambda, for a ray of direction d hitting a surface with
+// normal n (facing the ray).  thick[] holds the layer thicknesses (nm) in
+// material order; front selects the stack orientation (material.h:264-268).
+LAM_FN float thinfilm_reflectance(const DeviceScene &S, int4 mt, bool front, V3 n, V3 d, float lambda,
+                                  const float *thick) {
     const int layers = mt.y, n_off = mt.z;
     const int j_off = front ? 0 : layers + 1;
     const int j_sign = front ? 1 : -1;
-    auto nidx = [&](int j) { return S.props[n_off + j_off + j_sign * j]; };
-    auto ridx = [&](int j) {
-        float2 v = dense_c(S.cspectra + size_t(nidx(j)) * kBins, lambda);
-        return cf(v.x, v.y);
+    auto index_of = [&](int j) {
+        float2 v = dense_c(S.cspectra + size_t(S.props[n_off + j_off + j_sign * j]) * kBins, lambda);
+        return cff(v.x, v.y);
     };
-    Cf n0 = ridx(0), n1 = ridx(1);
-    Cf cos_cur = cf(dot(n, -d));
-    Cf sin_next = n0 / n1;
-    sin_next = sin_next * (sin_next * (cf(1) - cos_cur * cos_cur));
-    Cf cos_next = csqrt(cf(1) - sin_next);
-    M2 ms = interface_s(n0, n1, cos_cur, cos_next);
-    M2 mp = interface_p(n0, n1, cos_cur, cos_next);
-    const float k = 2 * kPi / lambda;
-    for (int j = 0; j < layers; j++) {
-        n0 = ridx(j + 1);
-        n1 = ridx(j + 2);
-        cos_cur = cos_next;
-        Cf sin_cur = sin_next;
-        sin_next = n0 / n1;
-        sin_next = sin_next * (sin_next * sin_cur);
-        cos_next = csqrt(cf(1) - sin_next);
-        M2 is = interface_s(n0, n1, cos_cur, cos_next);
-        M2 ip = interface_p(n0, n1, cos_cur, cos_next);
-        float dj = thick[front ? j : layers - 1 - j];
-        Cf phi = ((n0 * k) * dj) * cos_cur;
-        // diag(exp(-i phi), exp(i phi)) times the interface matrix
-        Cf em = cexpf(cf(phi.im, -phi.re));
-        Cf ep = cexpf(cf(-phi.im, phi.re));
-        M2 pi_s{em * is.a, em * is.b, ep * is.c, ep * is.d};
-        M2 pi_p{em * ip.a, em * ip.b, ep * ip.c, ep * ip.d};
-        ms = mul(ms, pi_s);
-        mp = mul(mp, pi_p);
+    const CFF one = cff(1.0f);
+    // Snell recurrence, outer medium first.
+    CFF nj = index_of(0);
+    CFF cj = cff(dot(n, -d));
+    CFF ys = nj * cj, yp = cj / nj;
+    CFF n_next = index_of(1);
+    CFF ratio = nj / n_next;
+    CFF sin2 = ratio * (ratio * (one - cj * cj));
+    CFF c_next = cffsqrt(one - sin2);
+    // 2 pi / lambda rounded to float, as Lambda forms it (material.h:293)
+    const FF k = ff(2 * kPi / lambda);
+    for (int j = 1; j <= layers; j++) {
+        nj = n_next;
+        cj = c_next;
+        n_next = index_of(j + 1);
+        ratio = nj / n_next;
+        sin2 = ratio * (ratio * sin2);
+        c_next = cffsqrt(one - sin2);
+        const FF dj = ff(thick[front ? j - 1 : layers - j]);
+        const CFF delta = ((nj * k) * dj) * cj;
+        // cos and sin of the complex phase, both divided by cosh(Im delta)
+        FF sa, ca;
+        ffsincos(delta.re, sa, ca);
+        const FF th = fftanh(delta.im);
+        const CFF cd = CFF{ca, -(sa * th)};
+        const CFF isd = CFF{-(ca * th), sa};  // i * sin(delta) / cosh
+        const CFF es = nj * cj, ep = cj / nj;
+        ys = es * (ys * cd - es * isd) / (es * cd - ys * isd);
+        yp = ep * (yp * cd - ep * isd) / (ep * cd - yp * isd);
     }
-    float rs = cabs2(ms.b / ms.a);
-    float rp = cabs2(mp.b / mp.a);
-    return (rs + rp) * 0.5f;
+    const CFF es_far = n_next * c_next, ep_far = c_next / n_next;
+    const FF rs = cffabs2(es_far - ys) / cffabs2(es_far + ys);
+    const FF rp = cffabs2(ep_far - yp) / cffabs2(ep_far + yp);
+    return ffloat((rs + rp) * 0.5f);
 }
 
 constexpr int kMaxLayers = 8;

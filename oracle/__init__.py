@@ -41,7 +41,7 @@ def _load():
         lib.oracle_render.argtypes = [sv, rp, f32, C.c_int32, C.c_int32, C.c_int32]
         lib.oracle_closest_hit.argtypes = [sv, f32, C.c_int32, C.POINTER(abi.Hit)]
         lib.oracle_occluded.argtypes = [sv, f32, i32, i32, C.c_int32, i32]
-        lib.oracle_thinfilm.argtypes = [sv, C.c_int32, f32, C.c_int32, f32, C.POINTER(C.c_double)]
+        lib.oracle_thinfilm.argtypes = [sv, C.c_int32, f32, C.c_int32, f32, C.POINTER(C.c_double), i32]
         lib.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_int32, C.POINTER(C.c_uint32)]
         lib.oracle_pcg32.restype = None
         lib.oracle_path_stats.argtypes = [sv, rp, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.POINTER(C.c_int64)]
@@ -87,17 +87,20 @@ def occluded(scene, rays: np.ndarray, light_instance: np.ndarray, light_object: 
     return out
 
 
-def thinfilm(scene, material: int, queries: np.ndarray):
+def thinfilm(scene, material: int, queries: np.ndarray, with_flags: bool = False):
     """Returns (reference float value, full-FP64 value) per query row
-    {cos_theta, lambda, front, px, py, pz, unused}."""
+    {cos_theta, lambda, front, px, py, pz, unused}; with_flags adds the
+    per-query flag of a square root whose branch is decided by a rounding
+    residue."""
     lib = _load()
     q = np.ascontiguousarray(queries, np.float32)
     ref = np.zeros(len(q), np.float32)
     f64 = np.zeros(len(q), np.float64)
+    amb = np.zeros(len(q), np.int32)
     if lib.oracle_thinfilm(C.byref(scene.view), material, _p(q, C.c_float), len(q), _p(ref, C.c_float),
-                           _p(f64, C.c_double)):
+                           _p(f64, C.c_double), _p(amb, C.c_int32)):
         raise RuntimeError("oracle_thinfilm: material %d is not a thin film" % material)
-    return ref, f64
+    return (ref, f64, amb.astype(bool)) if with_flags else (ref, f64)
 
 
 def pcg32(seed: int, stream: int, n: int) -> np.ndarray:

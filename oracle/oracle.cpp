@@ -704,7 +704,19 @@ struct SceneData {
     }
 
     // ------------------------------------------------ scene/material.h
-    float thinfilm_reflectance(int mat, const Hit &h, const Ray &r, float lambda, double *fp64) const {
+    // A square root whose argument lies on the negative real axis up to a
+    // rounding residue in its imaginary part: the branch (sign of the
+    // result's imaginary part) is then decided by that residue.
+    // With complex indices the imaginary part is a rounding residue of
+    // either sign (or an exact zero of either sign); with real indices it is
+    // an exact +0 in every implementation and the branch is determined.
+    static bool noise_branch(Complex z, bool complex_indices) {
+        return z.real() < 0 && std::fabs(z.imag()) <= 1e-12 * std::fabs(z.real()) &&
+               (z.imag() != 0 || complex_indices);
+    }
+
+    float thinfilm_reflectance(int mat, const Hit &h, const Ray &r, float lambda, double *fp64,
+                               bool *ambiguous = nullptr) const {
         const int32_t *mp = sv->material_params + 3 * mat;
         const int32_t *props = sv->material_properties;
         int layers = mp[0], n_off = mp[1], d_off = mp[2];
@@ -717,6 +729,10 @@ struct SceneData {
         Complex sin_next = n0 / n1;
         sin_next *= sin_next * (Complex(1) - cos_cur * cos_cur);
         Complex cos_next = std::sqrt(Complex(1) - sin_next);
+        bool any_complex = false;
+        for (int j = 0; j < layers + 2; j++)
+            if (dense_c(cspectrum(props[n_off + j]), lambda).imag() != 0) any_complex = true;
+        bool amb = noise_branch(Complex(1) - sin_next, any_complex);
         M2 ms = interface_s(n0, n1, cos_cur, cos_next);
         M2 mpm = interface_p(n0, n1, cos_cur, cos_next);
         for (int j = 0; j < layers; j++) {
@@ -728,6 +744,7 @@ struct SceneData {
             sin_next = n0 / n1;
             sin_next *= sin_next * sin_cur;
             cos_next = std::sqrt(Complex(1) - sin_next);
+            amb = amb || noise_branch(Complex(1) - sin_next, any_complex);
             M2 is = interface_s(n0, n1, cos_cur, cos_next);
             M2 ip = interface_p(n0, n1, cos_cur, cos_next);
             Complex phi = n0 * Complex(2 * PI / lambda) * double(scalar_texture(props[d_off + d_index], h)) * cos_cur;
@@ -735,6 +752,7 @@ struct SceneData {
             ms = mul(ms, mul(pr, is));
             mpm = mul(mpm, mul(pr, ip));
         }
+        if (ambiguous) *ambiguous = amb;
         if (fp64) {
             double rs = std::abs(ms.d[1] / ms.d[0]);
             double rp = std::abs(mpm.d[1] / mpm.d[0]);
@@ -1006,7 +1024,7 @@ int oracle_occluded(const lambda_scene_view *scene, const float *rays, const int
 }
 
 int oracle_thinfilm(const lambda_scene_view *scene, int32_t material, const float *q, int32_t n, float *out_ref,
-                    double *out_fp64) {
+                    double *out_fp64, int32_t *out_ambiguous) {
     if (!scene || material < 0 || material >= scene->num_materials ||
         scene->material_type[material] != LAMBDA_MATERIAL_THINFILM)
         return 1;
@@ -1021,9 +1039,11 @@ int oracle_thinfilm(const lambda_scene_view *scene, int32_t material, const floa
         h.front = x[2] != 0;
         Ray r{mk(0, 0, 0), mk(s, 0, -c), spec(550)};
         double d = 0;
-        float f = S.thinfilm_reflectance(material, h, r, x[1], &d);
+        bool amb = false;
+        float f = S.thinfilm_reflectance(material, h, r, x[1], &d, &amb);
         if (out_ref) out_ref[i] = f;
         if (out_fp64) out_fp64[i] = d;
+        if (out_ambiguous) out_ambiguous[i] = amb ? 1 : 0;
     }
     return 0;
 }

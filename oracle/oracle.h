@@ -38,9 +38,12 @@ int oracle_occluded(const lambda_scene_view *scene, const float *rays, const int
  * {cos_theta, lambda, front_facing, px, py, pz, unused}, normal (0,0,1) and
  * direction (sin, 0, -cos).  out_ref[i] is the reference's value (complex
  * double arithmetic, magnitudes rounded to float at the end,
- * scene/material.h:302-308); out_fp64[i] keeps every step in double. */
+ * scene/material.h:302-308); out_fp64[i] keeps every step in double.
+ * out_ambiguous[i] (may be NULL) is 1 when a complex square root met its
+ * branch cut up to a rounding residue (absorbing layers under total internal
+ * reflection), so the reference's branch is decided by rounding noise. */
 int oracle_thinfilm(const lambda_scene_view *scene, int32_t material, const float *queries, int32_t n,
-                    float *out_ref, double *out_fp64);
+                    float *out_ref, double *out_fp64, int32_t *out_ambiguous);
 
 /* n PCG32 outputs of stream (seed, stream) (math/random.h:9-31). */
 void oracle_pcg32(uint64_t seed, uint64_t stream, int32_t n, uint32_t *out);

@@ -16,6 +16,7 @@ from conftest import GOLDEN_SCENES, golden, load_golden_scene
 
 import lambda_amd
 import oracle
+from thinfilm_cases import STACKS, queries, stack_scene, within_tolerance
 
 pytestmark = pytest.mark.gpu
 
@@ -88,7 +89,9 @@ def test_closest_hits_large_random_set(tmp_path):
     assert np.array_equal(got["t"][m].view(np.uint32), want["t"][m].view(np.uint32))
 
 
-def test_thinfilm_fp32_within_1e5_of_fp64_oracle(devices, scenes):
+def test_thinfilm_within_1e5_of_fp64_oracle(devices, scenes):
+    """Check (b) on the golden queries: perlin and worley thickness on a soap
+    film, textures evaluated on the GPU at the query points."""
     g = golden("thinfilm.npz")
     q7 = g["queries"]
     q4 = np.zeros((len(q7), 4), np.float32)
@@ -100,6 +103,24 @@ def test_thinfilm_fp32_within_1e5_of_fp64_oracle(devices, scenes):
         err = np.abs(got.astype(np.float64) - fp64)
         print("material %d: max |R_gpu - R_fp64| = %.3g" % (m, err.max()), flush=True)
         assert err.max() < 1e-5
+
+
+@pytest.mark.parametrize("k", range(len(STACKS)))
+def test_thinfilm_stacks_within_1e5_of_fp64_oracle(k):
+    """Check (b) over absorbing, metallic, multilayer and back-lit stacks,
+    grazing angles included (tolerance relative above R = 1)."""
+    ns, ds = STACKS[k]
+    scene = stack_scene(ns, ds)
+    q = queries(4000, seed=100 + k)
+    _, fp64, amb = oracle.thinfilm(scene, 1, q, with_flags=True)
+    dev = lambda_amd.DeviceScene(scene)
+    q4 = np.zeros((len(q), 4), np.float32)
+    q4[:, :3] = q[:, :3]
+    got = dev.thinfilm(1, q4, np.zeros((len(q), 3), np.float32))
+    dev.close()
+    err, n_amb = within_tolerance(got, fp64, amb)
+    print("stack %d: max error %.3g (%d noise-branch queries skipped)" % (k, err, n_amb), flush=True)
+    assert err < 1e-5
 
 
 @pytest.mark.parametrize("name", GOLDEN_SCENES)
@@ -124,9 +145,20 @@ def test_check_c_4096spp_against_reference(tmp_path):
     dev = lambda_amd.DeviceScene(s)
     img = dev.render()
     dev.close()
-    err = rel_rmse(img, g["image"])
-    print("4096 spp relRMSE vs reference: %.3g" % err, flush=True)
+    ref = g["image"]
+    # The reference estimator itself yields NaN in a handful of pixels: a
+    # cosine-weighted direction drawn with randomFloat() == 1.0 lies in the
+    # tangent plane, its pdf is 0 and throughput becomes 0/0
+    # (render/renderer.h:179-183).  The GPU follows the same arithmetic, so
+    # both images are compared on the pixels finite in both.
+    bad_ref = ~np.isfinite(ref).all(-1)
+    bad_gpu = ~np.isfinite(img).all(-1)
+    m = ~(bad_ref | bad_gpu)
+    err = rel_rmse(img[m], ref[m])
+    print("4096 spp relRMSE vs reference: %.3g (non-finite pixels: reference %d, gpu %d, both %d)"
+          % (err, bad_ref.sum(), bad_gpu.sum(), (bad_ref & bad_gpu).sum()), flush=True)
     assert err < 1e-3
+    assert bad_gpu.sum() <= 2 * bad_ref.sum() + 8
 
 
 def test_throughput_mode_is_unbiased_against_reference_mode(devices):
