@@ -11,10 +11,11 @@ region of each step ends when the RGB image is back in host memory.
 
 Printed JSON line (rank 0, one GPU):
   value        paths/s over the K timed renders
-  roofline     the extend (closest-hit traversal) kernel: algorithmic bytes
-               per launch (oracle-counted node/instance/primitive tests per
-               closest-hit ray on a strided pixel sample x rays per launch)
-               over its mean launch time from HIP events, against 8 TB/s
+  roofline     the trace kernel (closest-hit and occlusion traversal):
+               algorithmic bytes per launch (oracle-counted node, instance and
+               primitive tests per ray of each kind on a strided pixel sample,
+               times the rays of each kind per launch) over its mean launch
+               time from HIP events, against 8 TB/s
   cpu_baseline the CPU oracle (bit-identical restatement of the reference
                CPU path) on a band of rows at reduced spp, all host threads
 
@@ -58,24 +59,29 @@ def make_workload(name: str, scene_dir: str):
 
 
 def cpu_baseline(scene, seconds: float, threads: int) -> dict:
-    """Times the CPU oracle on a band of rows at reduced spp, calibrated to
-    roughly `seconds` of work; paths/s is independent of spp to first order."""
+    """Times the CPU oracle on a band of rows, calibrated to roughly `seconds`
+    of work: the band grows to the full frame at 4 spp, then the spp grows
+    (16, 64); paths/s is independent of spp to first order."""
     import oracle
     p = scene.render_params()
-    spp, root = 4, 2
-    q = scene.render_params(samples=spp, sqrt_samples=root)
     mid = p.height // 2
-    rows = 2
-    t = 0.0
+    rows, root = 2, 2
     while True:
+        spp = root * root
+        q = scene.render_params(samples=spp, sqrt_samples=root)
         r0 = max(0, mid - rows // 2)
         r1 = min(p.height, r0 + rows)
         t0 = time.perf_counter()
         oracle.render(scene, q, threads=threads, rows=(r0, r1))
         t = time.perf_counter() - t0
-        if t >= seconds / 4 or r1 - r0 >= p.height:
+        if t >= seconds / 4:
             break
-        rows = min(p.height, max(rows * 2, int(rows * seconds / max(t, 1e-3) / 2)))
+        if r1 - r0 < p.height:
+            rows = min(p.height, max(rows * 2, int(rows * seconds / max(t, 1e-3) / 2)))
+        elif root < 8 and root * root < p.samples:
+            root *= 2
+        else:
+            break
     paths = (r1 - r0) * p.width * spp
     return {"value": paths / t, "unit": "paths/s", "cores": threads, "kind": "port",
             "sample": "rows %d-%d of %d (%d pixels) at %d spp, %.1f s; CPU oracle, bit-identical restatement of "
@@ -145,7 +151,7 @@ def main() -> int:
 
     # Kernel timing from HIP events on the render stream.
     _, st = dev.render_stats(rng_mode=mode, path_slots=args.path_slots)
-    log("kernel ms: " + ", ".join("%s %.1f" % (k, st[k + "_ms"]) for k in ("extend", "shade", "shadow", "finalize")))
+    log("kernel ms: " + ", ".join("%s %.1f" % (k, st[k + "_ms"]) for k in ("trace", "shade", "finalize")))
 
     # Algorithmic bytes per closest-hit ray, counted by the oracle on a
     # strided pixel sample (all samples of each sampled pixel).
@@ -156,15 +162,17 @@ def main() -> int:
     t_stats = time.perf_counter() - t0
     b_closest = oracle.ray_bytes(ps, "closest")
     b_shadow = oracle.ray_bytes(ps, "shadow")
-    launches = max(1, st["extend_launches"])
-    bytes_per_launch = b_closest * st["closest_rays"] / launches
-    avg_launch_s = st["extend_ms"] / 1e3 / launches
+    # k_trace handles both ray kinds in one launch
+    launches = max(1, st["trace_launches"])
+    bytes_total = b_closest * st["closest_rays"] + b_shadow * st["shadow_rays"]
+    bytes_per_launch = bytes_total / launches
+    avg_launch_s = st["trace_ms"] / 1e3 / launches
     achieved = bytes_per_launch / avg_launch_s / 1e9
     log("traversal work from %d sampled pixels (%.1f s): %.0f B per closest ray, %.0f B per shadow ray" % (
         ps["paths"] // max(1, p.samples), t_stats, b_closest, b_shadow))
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "extend_traffic_%s.json" % cfg["workload"])
+    tpath = os.path.join(ROOT, "profiles", "trace_traffic_%s.json" % cfg["workload"])
     if os.path.exists(tpath):
         with open(tpath) as fh:
             tj = json.load(fh)
@@ -194,14 +202,14 @@ def main() -> int:
         "data": "synthetic (generated scene, seeded)",
         "config": cfg,
         "roofline": {
-            "bound": "hbm", "kernel": "k_extend", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "bound": "hbm", "kernel": "k_trace", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "bytes_per_closest_ray": b_closest, "launches": launches,
+            "bytes_per_closest_ray": b_closest, "bytes_per_shadow_ray": b_shadow, "launches": launches,
             "avg_launch_ms": avg_launch_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
-        "kernels_ms": {k: st[k + "_ms"] for k in ("extend", "shade", "shadow", "finalize")},
+        "kernels_ms": {k: st[k + "_ms"] for k in ("trace", "shade", "finalize")},
         "rays": {"closest": st["closest_rays"], "shadow": st["shadow_rays"], "iterations": st["iterations"]},
         "image_finite": finite,
         "device": name,

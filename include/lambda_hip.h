@@ -217,9 +217,10 @@ int lambda_hip_scene_render(lambda_hip_scene *scene, const lambda_render_params 
 typedef struct lambda_render_stats {
     int32_t iterations;      /* bounce-loop passes */
     int32_t path_slots;      /* paths in flight at the start */
-    int64_t closest_rays;    /* rays traced by the extend kernel */
-    int64_t shadow_rays;     /* NEE rays traced by the shadow kernel */
-    double kernel_ms[4];     /* total device time: extend, shade, shadow, finalize */
+    int64_t closest_rays;    /* closest-hit rays traced */
+    int64_t shadow_rays;     /* next-event occlusion rays traced */
+    double kernel_ms[4];     /* total device time: trace (closest-hit and occlusion
+                                rays), shade, finalize, unused */
     int64_t launches[4];
     double render_ms;        /* wall time of the whole call */
 } lambda_render_stats;

@@ -66,7 +66,7 @@ class RenderStats(C.Structure):
     ]
 
 
-KERNELS = ("extend", "shade", "shadow", "finalize")
+KERNELS = ("trace", "shade", "finalize", "unused")
 
 
 HIT_DTYPE = np.dtype([

@@ -233,21 +233,43 @@ LAM_FN float perlin(const int *perm, V3 p) {
 
 LAM_FN float fract(float x) { return x - floorf(x); }
 
-// Cellular (F1) distance to one hashed feature point per unit cell over the
-// 27 cells around p (core/utils.h:150-171).
-LAM_FN float worley(V3 p) {
-    int xi = int(floorf(p.x)), yi = int(floorf(p.y)), zi = int(floorf(p.z));
-    float d = 9999;
+// Feature point of a unit cell: the cell corner plus a hash of its integer
+// coordinates, fract(sin(dot(cell, k)) * 43758.5453) per axis
+// (core/utils.h:161-166).
+LAM_FN V3 worley_feature(int cx, int cy, int cz) {
+    V3 c = v3(float(cx), float(cy), float(cz));
     const V3 k1 = v3(float(127.1), float(311.7), float(74.7));
     const V3 k2 = v3(float(269.5), float(183.3), float(246.1));
     const V3 k3 = v3(float(113.5), float(271.9), float(124.6));
     const float amp = float(43758.5453);
+    V3 f = v3(fract(sinf(dot(c, k1)) * amp), fract(sinf(dot(c, k2)) * amp), fract(sinf(dot(c, k3)) * amp));
+    return f + c;
+}
+
+// Precomputed feature points of a box of cells.  The hash multiplies the
+// last bit of sin() by 43758, so the points are computed once on the host,
+// with the same float operations and the host's libm, and looked up here;
+// cells outside the box are hashed on the device.
+struct WorleyTable {
+    const float4 *points;  // [nz][ny][nx] feature points
+    int x0, y0, z0, nx, ny, nz;
+};
+
+// Cellular (F1) distance to one feature point per unit cell over the 27
+// cells around p (core/utils.h:150-171).
+LAM_FN float worley(V3 p, const WorleyTable &tab) {
+    int xi = int(floorf(p.x)), yi = int(floorf(p.y)), zi = int(floorf(p.z));
+    float d = 9999;
     for (int xo = -1; xo <= 1; xo++)
         for (int yo = -1; yo <= 1; yo++)
             for (int zo = -1; zo <= 1; zo++) {
-                V3 c = v3(float(xi + xo), float(yi + yo), float(zi + zo));
-                V3 f = v3(fract(sinf(dot(c, k1)) * amp), fract(sinf(dot(c, k2)) * amp), fract(sinf(dot(c, k3)) * amp));
-                f = f + c;
+                int cx = xi + xo, cy = yi + yo, cz = zi + zo;
+                unsigned lx = unsigned(cx - tab.x0), ly = unsigned(cy - tab.y0), lz = unsigned(cz - tab.z0);
+                V3 f;
+                if (tab.points && lx < unsigned(tab.nx) && ly < unsigned(tab.ny) && lz < unsigned(tab.nz))
+                    f = v3(tab.points[(size_t(lz) * tab.ny + ly) * tab.nx + lx]);
+                else
+                    f = worley_feature(cx, cy, cz);
                 d = fminf(d, len(p - f));
             }
     return d;

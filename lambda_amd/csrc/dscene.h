@@ -36,6 +36,11 @@ struct DeviceScene {
     const float *light_pow;
     const float *cie;       // [471][3]
     const int *perm;        // [256]
+    const float4 *worley_points;  // feature points of all tables
+    const int4 *worley_box;       // [scalar textures] {x0, y0, z0, offset}
+    const int4 *worley_dim;       // [scalar textures] {nx, ny, nz, 0}; nx = 0: no table
+    const int *tlas_rank;   // [instances] order of the instance's leaf in the reference's DFS
+    const int *obj_rank;    // [objects] order within its object-level BVH in that DFS
     float4 cam_pos, cam_corner, cam_h, cam_v;
     float total_power;
     int num_lights, bg_tex;
@@ -127,27 +132,6 @@ LAM_FN bool test_tri(V3 c, V3 h, V3 v, V3 o, V3 d, float best, float &t_out, flo
     return true;
 }
 
-// Test of object k against the (local) ray; updates best on a closer hit.
-LAM_FN bool test_object(const DeviceScene &S, int k, V3 o, V3 d, float &best) {
-    float4 a = S.obj_a[k];
-    float4 b = S.obj_b[k];
-    int type = __float_as_int(b.w);
-    float t, al, be;
-    if (type == LAMBDA_OBJECT_SPHERE) {
-        if (!test_sphere(a, o, d, best, t)) return false;
-    } else {
-        float4 c = S.obj_c[k];
-        if (type == LAMBDA_OBJECT_TRI) {
-            if (!test_tri(v3(a), v3(b), v3(c), o, d, best, t, al, be)) return false;
-        } else {
-            float4 n = S.obj_n[k];
-            if (!test_quad(v3(a), v3(b), v3(c), v3(n), o, d, best, t, al, be)) return false;
-        }
-    }
-    best = t;
-    return true;
-}
-
 // Full intersection record of object k for a local-space ray, as the
 // reference's intersect* fills it (scene/object.h:301-400).
 LAM_FN bool object_record(const DeviceScene &S, int k, V3 o, V3 d, float best, Hit &h) {
@@ -215,16 +199,24 @@ LAM_FN bool hit_record(const DeviceScene &S, int inst, int obj, V3 o, V3 d, Hit 
 
 // ------------------------------------------------------------- traversal
 //
-// Depth-first traversal of the instance-level BVH and, per instance leaf, of
-// the instance's object-level BVH (scene/scene.h:51-124).  Children are
-// pushed as (left, right) and popped right first, the reference's order, so
-// equal-distance ties resolve to the same primitive.  A node is skipped when
-// its entry distance exceeds the current closest hit.  Only the first
-// instance of an instance-level leaf is entered (scene/scene.h:79-86).
+// Two-level BVH traversal (scene/scene.h:51-124) over the reference's own
+// nodes, nearest child first.  At an inner node both children's boxes are
+// tested; the nearer is entered and the farther pushed.  A node is skipped
+// when its entry distance exceeds the current closest hit -- the reference's
+// test -- and a popped node is re-tested against the hit found meanwhile.
+// Only the first instance of an instance-level leaf is entered, as in the
+// reference (scene/scene.h:79-86).
 //
-// The two stacks live in LDS: entry k of lane `lane` is at stk[k * stride +
-// lane] for the instance level and stk[(kStack + k) * stride + lane] for the
-// object level, so a wave's pushes and pops hit 64 distinct banks.
+// Visiting order changes which of several primitives at exactly the same
+// distance is met first.  The reference keeps the first one in its own
+// depth-first order (right child before left, scene/scene.h:112-119), so on
+// an exact tie the candidate with the smaller reference rank wins:
+// rank = (instance-level leaf order << 32) | object order within its BVH,
+// both precomputed on the host by replaying that order.
+//
+// Stacks live in LDS: entry k of a lane is at stk[k * stride + lane] for the
+// instance level and at stk[(kStack + k) * stride + lane] for the object
+// level, so a wave's pushes and pops touch 64 distinct banks.
 
 LAM_FN bool slab(float4 lo, float4 hi, V3 o, V3 inv, float &tnear) {
     float t1 = 0, t2 = kMax;
@@ -250,71 +242,144 @@ struct TraceStats {
     uint32_t nodes, instances, prims;
 };
 
+LAM_FN uint64_t ref_rank(const DeviceScene &S, int inst, int obj) {
+    return (uint64_t(uint32_t(S.tlas_rank[inst])) << 32) | uint64_t(uint32_t(S.obj_rank[obj]));
+}
+
+// Test of object k against the (local) ray.  A closer hit always wins; a hit
+// at exactly the current distance wins when it comes first in the
+// reference's order (closest-hit mode only).
+template <bool kAny>
+LAM_FN bool test_object_ranked(const DeviceScene &S, int k, V3 o, V3 d, float &best, int inst, int &hit_inst,
+                               int &hit_obj) {
+    float4 a = S.obj_a[k];
+    float4 b = S.obj_b[k];
+    int type = __float_as_int(b.w);
+    // accept t <= best here, decide ties below
+    float limit = kAny ? best : nextafterf(best, INFINITY);
+    float t, al, be;
+    if (type == LAMBDA_OBJECT_SPHERE) {
+        if (!test_sphere(a, o, d, limit, t)) return false;
+    } else {
+        float4 c = S.obj_c[k];
+        if (type == LAMBDA_OBJECT_TRI) {
+            if (!test_tri(v3(a), v3(b), v3(c), o, d, limit, t, al, be)) return false;
+        } else {
+            float4 n = S.obj_n[k];
+            if (!test_quad(v3(a), v3(b), v3(c), v3(n), o, d, limit, t, al, be)) return false;
+        }
+    }
+    if (!kAny && t == best && hit_obj >= 0 && ref_rank(S, inst, k) > ref_rank(S, hit_inst, hit_obj))
+        return false;
+    best = t;
+    hit_inst = inst;
+    hit_obj = k;
+    return true;
+}
+
 // kAny = false: closest hit; returns true if anything was hit, with
 //   (hit_inst, hit_obj, best).
 // kAny = true: occlusion; best is the light distance, (skip_inst, skip_obj)
 //   the light itself; returns true on the first blocking hit.
 template <bool kAny, bool kStats>
-LAM_FN bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_inst,
-                                         int &hit_obj, int skip_inst, int skip_obj, int *stk, int stride,
-                                         TraceStats &st) {
+LAM_FN bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_inst, int &hit_obj, int skip_inst,
+                     int skip_obj, int *stk, int stride, TraceStats &st) {
     const V3 inv = v3(1 / d.x, 1 / d.y, 1 / d.z);
     int *top = stk;
     int *sub = stk + kStack * stride;
-    int sp = 0;
-    top[0] = 0;
-    sp = 1;
+    int sp = 0, ssp = 0;
     bool found = false;
-    while (sp > 0) {
-        int cur = top[--sp * stride];
-        float4 lo = S.node_lo[cur];
-        float4 hi = S.node_hi[cur];
-        if (kStats) st.nodes++;
-        float tn;
-        if (!slab(lo, hi, o, inv, tn) || tn > best) continue;
-        int count = __float_as_int(hi.w);
-        int index = __float_as_int(lo.w);
-        if (count > 0) {
-            const int ii = index;
-            if (kStats) st.instances++;
-            V3 lo_o, lo_d;
-            local_ray(S, ii, o, d, lo_o, lo_d);
-            const V3 linv = v3(1 / lo_d.x, 1 / lo_d.y, 1 / lo_d.z);
-            int ssp = 1;
-            sub[0] = S.inst_i[ii].z;
-            while (ssp > 0) {
-                int sc = sub[--ssp * stride];
-                float4 slo = S.node_lo[sc];
-                float4 shi = S.node_hi[sc];
+    bool in_blas = false;
+    int inst = -1;
+    V3 lo_o = o, lo_d = d, linv = inv;
+    // current node (already tested): index and count words
+    float tn;
+    float4 nlo = S.node_lo[0], nhi = S.node_hi[0];
+    if (kStats) st.nodes++;
+    if (!slab(nlo, nhi, o, inv, tn) || tn > best) return false;
+    int cur = 0;
+    int idx = __float_as_int(nlo.w), cnt = __float_as_int(nhi.w);
+    while (true) {
+        bool have_next = false;
+        if (cnt > 0) {
+            if (!in_blas) {
+                // instance-level leaf: enter the first instance's BVH
+                inst = idx;
+                if (kStats) st.instances++;
+                local_ray(S, inst, o, d, lo_o, lo_d);
+                linv = v3(1 / lo_d.x, 1 / lo_d.y, 1 / lo_d.z);
+                const int root = S.inst_i[inst].z;
+                float4 rlo = S.node_lo[root], rhi = S.node_hi[root];
                 if (kStats) st.nodes++;
-                float stn;
-                if (!slab(slo, shi, lo_o, linv, stn) || stn > best) continue;
-                int scount = __float_as_int(shi.w);
-                int sindex = __float_as_int(slo.w);
-                if (scount > 0) {
-                    for (int k = sindex; k < sindex + scount; k++) {
-                        if (kAny && ii == skip_inst && k == skip_obj) continue;
-                        if (kStats) st.prims++;
-                        if (test_object(S, k, lo_o, lo_d, best)) {
-                            if (kAny) return true;
-                            hit_inst = ii;
-                            hit_obj = k;
-                            found = true;
-                        }
+                if (slab(rlo, rhi, lo_o, linv, tn) && !(tn > best)) {
+                    in_blas = true;
+                    cur = root;
+                    idx = __float_as_int(rlo.w);
+                    cnt = __float_as_int(rhi.w);
+                    continue;
+                }
+            } else {
+                for (int k = idx; k < idx + cnt; k++) {
+                    if (kAny && inst == skip_inst && k == skip_obj) continue;
+                    if (kStats) st.prims++;
+                    if (test_object_ranked<kAny>(S, k, lo_o, lo_d, best, inst, hit_inst, hit_obj)) {
+                        if (kAny) return true;
+                        found = true;
                     }
-                } else {
-                    sub[ssp * stride] = sc + 1;
-                    sub[(ssp + 1) * stride] = sindex;
-                    ssp += 2;
                 }
             }
         } else {
-            top[sp * stride] = cur + 1;
-            top[(sp + 1) * stride] = index;
-            sp += 2;
+            const V3 &ro = in_blas ? lo_o : o;
+            const V3 &ri = in_blas ? linv : inv;
+            const int l = cur + 1, r = idx;
+            float4 llo = S.node_lo[l], lhi = S.node_hi[l];
+            float4 rlo = S.node_lo[r], rhi = S.node_hi[r];
+            if (kStats) st.nodes += 2;
+            float tl, tr;
+            bool hl = slab(llo, lhi, ro, ri, tl) && !(tl > best);
+            bool hr = slab(rlo, rhi, ro, ri, tr) && !(tr > best);
+            if (hl && hr) {
+                bool left_first = tl <= tr;
+                int far = left_first ? r : l;
+                int *stack = in_blas ? sub : top;
+                int &ptr = in_blas ? ssp : sp;
+                stack[ptr * stride] = far;
+                ptr++;
+                cur = left_first ? l : r;
+                idx = __float_as_int(left_first ? llo.w : rlo.w);
+                cnt = __float_as_int(left_first ? lhi.w : rhi.w);
+                continue;
+            }
+            if (hl || hr) {
+                cur = hl ? l : r;
+                idx = __float_as_int(hl ? llo.w : rlo.w);
+                cnt = __float_as_int(hl ? lhi.w : rhi.w);
+                continue;
+            }
+        }
+        // pop the next node that is still worth visiting
+        while (!have_next) {
+            int cand;
+            if (in_blas) {
+                if (ssp == 0) {
+                    in_blas = false;
+                    continue;
+                }
+                cand = sub[--ssp * stride];
+            } else {
+                if (sp == 0) return found;
+                cand = top[--sp * stride];
+            }
+            float4 clo = S.node_lo[cand], chi = S.node_hi[cand];
+            if (kStats) st.nodes++;
+            if (slab(clo, chi, in_blas ? lo_o : o, in_blas ? linv : inv, tn) && !(tn > best)) {
+                cur = cand;
+                idx = __float_as_int(clo.w);
+                cnt = __float_as_int(chi.w);
+                have_next = true;
+            }
         }
     }
-    return found;
 }
 
 // --------------------------------------------------------------- textures
@@ -324,7 +389,9 @@ LAM_FN float scalar_texture(const DeviceScene &S, int t, V3 p) {
     int type = __float_as_int(x.x);
     if (type == LAMBDA_SCALAR_CONSTANT) return x.y;
     if (type == LAMBDA_SCALAR_PERLIN) return x.y + (x.z - x.y) * perlin(S.perm, p * x.w);
-    return x.y + (x.z - x.y) * worley(p * x.w);
+    const int4 box = S.worley_box[t], dim = S.worley_dim[t];
+    WorleyTable tab{dim.x > 0 ? S.worley_points + box.w : nullptr, box.x, box.y, box.z, dim.x, dim.y, dim.z};
+    return x.y + (x.z - x.y) * worley(p * x.w, tab);
 }
 
 // Spectrum texture at one wavelength (scene/texture.h:178-210).  Checker

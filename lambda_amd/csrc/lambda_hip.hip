@@ -1,16 +1,20 @@
 // MI355X (gfx950) wavefront spectral path tracer behind the lambda_hip.h ABI.
 //
-// One bounce of every in-flight path is four kernels over a compacted queue
-// of path slots:
-//   extend   closest-hit traversal of the two-level BVH      (scene/scene.h:51-124)
+// Each pass over the in-flight paths is three kernels:
+//   trace    closest-hit traversal for the paths about to shade, and, in the
+//            same launch, the occlusion rays the previous shade pass queued
+//            (their contributions land in the path radiance)   (scene/scene.h:51-124)
+//   finalize samples that ended in the previous shade pass: radiance -> FP64
+//            XYZ framebuffer, and a new camera sample for the slot
+//                                                             (render/renderer.h:64-92)
 //   shade    miss / emission / next-event estimation / BSDF sampling /
 //            Russian roulette                                 (render/renderer.h:102-192)
-//   shadow   occlusion traversal of the NEE rays queued by shade
-//   finalize finished samples -> FP64 XYZ framebuffer, new camera samples
-//            for their slots, next queue                      (render/renderer.h:64-92)
-// Queues are compacted with wave-wide ballots: one atomic per wave per queue.
-// Path state lives in structure-of-arrays float4 records so every access of
-// a queue pass is a coalesced 16-byte load or store per lane.
+// A path's next-event contribution is added before its next emission term
+// and before its sample is retired, and every slot draws its random numbers
+// in the reference's order, so reference-stream renders follow Lambda's
+// estimator step for step.  Queues are compacted with wave-wide ballots: one
+// atomic per wave per queue.  Path state lives in structure-of-arrays float4
+// records so every queue pass reads and writes 16 bytes per lane.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -109,11 +113,12 @@ enum : int {
 };
 
 enum : int {
-    kCtrNext = 0,        // next active queue length
-    kCtrShadow = 1,      // shadow queue length
+    kCtrNext = 0,        // next closest-hit queue length
+    kCtrShadow = 1,      // occlusion queue length
     kCtrWork = 2,        // next work item (throughput mode)
     kCtrPixels = 3,      // finished pixels (reference mode) / samples (throughput)
     kCtrError = 4,       // error bits
+    kCtrDone = 5,        // samples ended by the last shade pass
     kCtrCount = 8
 };
 
@@ -127,6 +132,7 @@ struct Paths {
     int4 *info;       // pixel, sample index, bounce, flags
     int2 *hit;        // instance, object (-1 on a miss)
     int *q_in, *q_out;
+    int *done;                // slots whose sample ended in the last shade pass
     float4 *s_o, *s_d, *s_c;  // shadow: origin+tmax, direction, contribution
     int4 *s_i;                // shadow: slot, light instance, light object
     int *ctr;
@@ -193,22 +199,40 @@ __global__ void __launch_bounds__(kBlock) k_start(DeviceScene S, Frame F, Paths 
     P.q_in[s] = s;
 }
 
-// Closest hit for every queued path.
-__global__ void __launch_bounds__(kBlock) k_extend(DeviceScene S, Paths P, int count) {
+// Threads [0, n_closest) find the closest hit of queued path q_in[t];
+// threads [n_closest, n_closest + n_shadow) test queued occlusion ray
+// t - n_closest and, if the light is visible, add its contribution to the
+// path's radiance.
+__global__ void __launch_bounds__(kBlock) k_trace(DeviceScene S, Paths P, int n_closest, int n_shadow) {
     __shared__ int stk[2 * kStack * kBlock];
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q == 0) {
-        P.ctr[kCtrNext] = 0;
-        P.ctr[kCtrShadow] = 0;
-    }
-    if (q >= count) return;
-    int s = P.q_in[q];
-    float4 o = P.o[s], d = P.d[s];
-    float best = kMax;
-    int hi = -1, ho = -1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) P.ctr[kCtrNext] = 0;
     TraceStats st{};
-    traverse<false, false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk + threadIdx.x, kBlock, st);
-    P.hit[s] = make_int2(hi, ho);
+    if (t < n_closest) {
+        const int s = P.q_in[t];
+        float4 o = P.o[s], d = P.d[s];
+        float best = kMax;
+        int hi = -1, ho = -1;
+        traverse<false, false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk + threadIdx.x, kBlock, st);
+        P.hit[s] = make_int2(hi, ho);
+    } else if (t < n_closest + n_shadow) {
+        const int q = t - n_closest;
+        float4 so = P.s_o[q], sd = P.s_d[q];
+        int4 si = P.s_i[q];
+        float best = so.w;
+        int hi = -1, ho = -1;
+        bool blocked = traverse<true, false>(S, v3(so), v3(sd), best, hi, ho, si.y, si.z, stk + threadIdx.x,
+                                             kBlock, st);
+        if (!blocked) {
+            float4 c = P.s_c[q];
+            float4 L = P.L[si.x];
+            L.x += c.x;
+            L.y += c.y;
+            L.z += c.z;
+            L.w += c.w;
+            P.L[si.x] = L;
+        }
+    }
 }
 
 __device__ __forceinline__ S4 lambertian_albedo(const DeviceScene &S, int4 mt, const Hit &h, const S4 &lam) {
@@ -221,6 +245,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DeviceScene S, Frame F, Paths 
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     bool active = q < count;
     bool push_shadow = false;
+    bool path_done = false;
     int s = 0;
     float4 sh_o, sh_d, sh_c;
     int4 sh_i;
@@ -408,6 +433,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DeviceScene S, Frame F, Paths 
         P.L[s] = f4(L);
         store_rng(P, s, rng);
         P.info[s] = make_int4(info.x, info.y, bounce + 1, (specular ? kFlagSpecular : 0) | (finished ? kFlagFinished : 0));
+        path_done = finished;
     }
     int pos = wave_push(&P.ctr[kCtrShadow], push_shadow);
     if (push_shadow) {
@@ -416,30 +442,14 @@ __global__ void __launch_bounds__(kBlock) k_shade(DeviceScene S, Frame F, Paths 
         P.s_c[pos] = sh_c;
         P.s_i[pos] = sh_i;
     }
-}
-
-// Occlusion test of every queued NEE ray; unblocked contributions are added
-// to their path's radiance in the order the reference adds them.
-__global__ void __launch_bounds__(kBlock) k_shadow(DeviceScene S, Paths P) {
-    __shared__ int stk[2 * kStack * kBlock];
-    const int count = P.ctr[kCtrShadow];
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
-        float4 so = P.s_o[q], sd = P.s_d[q];
-        int4 si = P.s_i[q];
-        float best = so.w;
-        int hi = -1, ho = -1;
-        TraceStats st{};
-        bool blocked = traverse<true, false>(S, v3(so), v3(sd), best, hi, ho, si.y, si.z, stk + threadIdx.x, kBlock, st);
-        if (!blocked) {
-            float4 c = P.s_c[q];
-            float4 L = P.L[si.x];
-            L.x += c.x;
-            L.y += c.y;
-            L.z += c.z;
-            L.w += c.w;
-            P.L[si.x] = L;
-        }
-    }
+    // Continuing paths trace again next pass; ended ones wait for their
+    // occlusion ray and are then retired by finalize.
+    const bool cont = active && !path_done;
+    const int pn = wave_push(&P.ctr[kCtrNext], cont);
+    if (cont) P.q_out[pn] = s;
+    const bool ended = active && path_done;
+    const int pd = wave_push(&P.ctr[kCtrDone], ended);
+    if (ended) P.done[pd] = s;
 }
 
 // XYZ of one sample's radiance: the first half of spectrumToRGB
@@ -476,38 +486,39 @@ __device__ __forceinline__ void sample_xyz(const DeviceScene &S, const Frame &F,
     z /= float(106.856895);
 }
 
-// Retires finished samples into the framebuffer and gives their slots the
-// next sample; compacts the surviving slots into the next queue.
-__global__ void __launch_bounds__(kBlock) k_finalize(DeviceScene S, Frame F, Paths P, int count) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
+// Retires the samples that ended in the previous shade pass (their
+// occlusion rays have been traced since): radiance to the framebuffer, then
+// the slot's next sample, which joins the closest-hit queue.
+__global__ void __launch_bounds__(kBlock) k_finalize(DeviceScene S, Frame F, Paths P, int n_done) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0) {
+        P.ctr[kCtrShadow] = 0;
+        P.ctr[kCtrDone] = 0;
+    }
     bool keep = false;
-    int s = 0;
     bool need_work = false;
-    if (q < count) {
-        s = P.q_in[q];
+    int s = 0;
+    if (q < n_done) {
+        s = P.done[q];
         int4 info = P.info[s];
-        if (!(info.w & kFlagFinished)) {
-            keep = true;
-        } else {
-            float x, y, z;
-            sample_xyz(S, F, P.L[s], P.lam[s], x, y, z);
-            double *px = P.fb + size_t(info.x) * 3;
-            unsafeAtomicAdd(px + 0, double(x));
-            unsafeAtomicAdd(px + 1, double(y));
-            unsafeAtomicAdd(px + 2, double(z));
-            if (F.mode == LAMBDA_RNG_REFERENCE) {
-                if (info.y + 1 < F.spp) {
-                    Pcg rng = load_rng(P, s);
-                    camera_sample(S, F, P, s, info.x, info.y + 1, rng);
-                    store_rng(P, s, rng);
-                    keep = true;
-                } else {
-                    atomicAdd(&P.ctr[kCtrPixels], 1);
-                }
+        float x, y, z;
+        sample_xyz(S, F, P.L[s], P.lam[s], x, y, z);
+        double *px = P.fb + size_t(info.x) * 3;
+        unsafeAtomicAdd(px + 0, double(x));
+        unsafeAtomicAdd(px + 1, double(y));
+        unsafeAtomicAdd(px + 2, double(z));
+        if (F.mode == LAMBDA_RNG_REFERENCE) {
+            if (info.y + 1 < F.spp) {
+                Pcg rng = load_rng(P, s);
+                camera_sample(S, F, P, s, info.x, info.y + 1, rng);
+                store_rng(P, s, rng);
+                keep = true;
             } else {
                 atomicAdd(&P.ctr[kCtrPixels], 1);
-                need_work = true;
             }
+        } else {
+            atomicAdd(&P.ctr[kCtrPixels], 1);
+            need_work = true;
         }
     }
     if (F.mode != LAMBDA_RNG_REFERENCE) {
@@ -747,7 +758,9 @@ void validate(const lambda_scene_view &v) {
 
 struct lambda_hip_scene {
     DevBuf<float4> node_lo, node_hi, obj_a, obj_b, obj_c, obj_n, obj_s, inst, stex;
-    DevBuf<int> obj_mat, props, perm;
+    DevBuf<int> obj_mat, props, perm, tlas_rank, obj_rank;
+    DevBuf<float4> worley_points;
+    DevBuf<int4> worley_box, worley_dim;
     DevBuf<int4> inst_i, mat, xtex;
     DevBuf<float> spectra, light_pow, cie;
     DevBuf<float2> cspec;
@@ -858,6 +871,79 @@ void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g) {
     g.light_pow.upload(v.light_powers, size_t(v.num_lights));
     g.cie.upload(cie);
     g.perm.upload(kPerlinPermutation, 256);
+    {
+        // Replay the reference's depth-first order (left pushed before right,
+        // right popped first; scene/scene.h:112-119) without culling and number
+        // the leaves in the order it meets them.  Traversal breaks exact
+        // distance ties by these ranks.
+        std::vector<int> tr(I, 0x7fffffff), orank(O, 0x7fffffff);
+        auto replay = [&](int root, auto &&on_leaf) {
+            std::vector<int> st{root};
+            while (!st.empty()) {
+                int n = st.back();
+                st.pop_back();
+                int idx = v.node_ints[2 * n], cnt = v.node_ints[2 * n + 1];
+                if (cnt > 0) on_leaf(idx, cnt);
+                else {
+                    st.push_back(n + 1);
+                    st.push_back(idx);
+                }
+            }
+        };
+        int r = 0;
+        replay(0, [&](int idx, int) { tr[size_t(idx)] = r++; });
+        std::vector<char> seen(size_t(v.num_nodes), 0);
+        for (size_t i = 0; i < I; i++) {
+            int root = v.instance_ints[4 * i + 2];
+            if (seen[size_t(root)]) continue;
+            seen[size_t(root)] = 1;
+            int k = 0;
+            replay(root, [&](int idx, int cnt) {
+                for (int j = 0; j < cnt; j++) orank[size_t(idx + j)] = k++;
+            });
+        }
+        g.tlas_rank.upload(tr);
+        g.obj_rank.upload(orank);
+    }
+    {
+        // Worley feature-point tables: every cell a lookup can reach, for
+        // points on the geometry (inside the scene box) and for environment
+        // directions (inside the unit cube), scaled by the frequency.
+        const size_t kMaxCells = size_t(1) << 23;
+        std::vector<float4> pts;
+        std::vector<int4> box(size_t(v.num_scalar_textures), make_int4(0, 0, 0, 0));
+        std::vector<int4> dim(size_t(v.num_scalar_textures), make_int4(0, 0, 0, 0));
+        const float *root = v.node_bounds;
+        for (int t = 0; t < v.num_scalar_textures; t++) {
+            if (v.scalar_texture_type[t] != LAMBDA_SCALAR_WORLEY) continue;
+            const float f = v.scalar_texture_params[3 * t + 2];
+            long long lo[3], hi[3];
+            bool ok = std::isfinite(f);
+            for (int a = 0; a < 3 && ok; a++) {
+                double c[4] = {double(root[a]) * f, double(root[3 + a]) * f, -1.0 * f, 1.0 * f};
+                double mn = std::min(std::min(c[0], c[1]), std::min(c[2], c[3]));
+                double mx = std::max(std::max(c[0], c[1]), std::max(c[2], c[3]));
+                ok = std::isfinite(mn) && std::isfinite(mx) && std::fabs(mn) < 1e7 && std::fabs(mx) < 1e7;
+                lo[a] = (long long)std::floor(mn) - 2;
+                hi[a] = (long long)std::floor(mx) + 2;
+            }
+            if (!ok) continue;
+            const size_t nx = size_t(hi[0] - lo[0] + 1), ny = size_t(hi[1] - lo[1] + 1), nz = size_t(hi[2] - lo[2] + 1);
+            if (nx * ny * nz > kMaxCells || pts.size() + nx * ny * nz > 4 * kMaxCells) continue;
+            box[size_t(t)] = make_int4(int(lo[0]), int(lo[1]), int(lo[2]), int(pts.size()));
+            dim[size_t(t)] = make_int4(int(nx), int(ny), int(nz), 0);
+            for (size_t z = 0; z < nz; z++)
+                for (size_t y = 0; y < ny; y++)
+                    for (size_t x = 0; x < nx; x++) {
+                        V3 q = worley_feature(int(lo[0] + (long long)x), int(lo[1] + (long long)y), int(lo[2] + (long long)z));
+                        pts.push_back(make_float4(q.x, q.y, q.z, 0));
+                    }
+        }
+        if (pts.empty()) pts.push_back(make_float4(0, 0, 0, 0));
+        g.worley_points.upload(pts);
+        g.worley_box.upload(box);
+        g.worley_dim.upload(dim);
+    }
 
     DeviceScene &d = g.ds;
     d.node_lo = g.node_lo.p;
@@ -880,6 +966,11 @@ void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g) {
     d.light_pow = g.light_pow.p;
     d.cie = g.cie.p;
     d.perm = g.perm.p;
+    d.tlas_rank = g.tlas_rank.p;
+    d.worley_points = g.worley_points.p;
+    d.worley_box = g.worley_box.p;
+    d.worley_dim = g.worley_dim.p;
+    d.obj_rank = g.obj_rank.p;
     const float *c = v.camera;
     d.cam_pos = make_float4(c[0], c[1], c[2], 0);
     d.cam_corner = make_float4(c[3], c[4], c[5], 0);
@@ -915,7 +1006,7 @@ struct RenderBuffers {
     DevBuf<uint4> rng;
     DevBuf<int4> info, s_i;
     DevBuf<int2> hit;
-    DevBuf<int> qa, qb, ctr, order;
+    DevBuf<int> qa, qb, done, ctr, order;
     DevBuf<double> fb;
     DevBuf<float> rgb;
 };
@@ -984,7 +1075,7 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
     B.o.alloc(ns); B.d.alloc(ns); B.lam.alloc(ns); B.beta.alloc(ns); B.L.alloc(ns);
     B.s_o.alloc(ns); B.s_d.alloc(ns); B.s_c.alloc(ns); B.s_i.alloc(ns);
     B.rng.alloc(ns); B.info.alloc(ns); B.hit.alloc(ns);
-    B.qa.alloc(ns); B.qb.alloc(ns);
+    B.qa.alloc(ns); B.qb.alloc(ns); B.done.alloc(ns);
     B.ctr.alloc(kCtrCount);
     B.fb.alloc(size_t(F.npix) * 3);
     B.rgb.alloc(size_t(F.npix) * 3);
@@ -993,7 +1084,7 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
     Paths P;
     P.o = B.o.p; P.d = B.d.p; P.lam = B.lam.p; P.beta = B.beta.p; P.L = B.L.p;
     P.rng = B.rng.p; P.info = B.info.p; P.hit = B.hit.p;
-    P.q_in = B.qa.p; P.q_out = B.qb.p;
+    P.q_in = B.qa.p; P.q_out = B.qb.p; P.done = B.done.p;
     P.s_o = B.s_o.p; P.s_d = B.s_d.p; P.s_c = B.s_c.p; P.s_i = B.s_i.p;
     P.ctr = B.ctr.p; P.fb = B.fb.p; P.order = B.order.p;
 
@@ -1025,50 +1116,46 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
         int start = ns;
         HIP_OK(hipMemcpyAsync(P.ctr + kCtrWork, &start, sizeof(int), hipMemcpyHostToDevice, st));
     }
-    int count = ns;
+    int n_closest = ns, n_shadow = 0, n_done = 0;
     const long long total_units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
     // Every pass advances each live path by one segment; a slot runs at most
-    // (depth + 1) segments per sample it owns.
+    // depth + 1 segments per sample it owns, plus one pass to retire it.
     const long long per_slot = F.mode == LAMBDA_RNG_REFERENCE ? F.spp : (F.total_work + ns - 1) / ns + 1;
-    const long long max_iters = (long long)(F.depth + 1) * per_slot + 16;
+    const long long max_iters = (long long)(F.depth + 2) * per_slot + 16;
     long long iters = 0;
-    while (count > 0) {
+    while (n_closest > 0 || n_shadow > 0 || n_done > 0) {
         if (++iters > max_iters) throw Failure{LAMBDA_EHIP, "bounce loop did not drain"};
-        const int grid = grid_for(count);
         if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
-        k_extend<<<grid, kBlock, 0, st>>>(g.ds, P, count);
+        k_trace<<<grid_for((long long)n_closest + n_shadow), kBlock, 0, st>>>(g.ds, P, n_closest, n_shadow);
         if (stats) {
             HIP_OK(hipEventRecord(ev[0].b, st));
-            HIP_OK(hipEventRecord(ev[1].a, st));
-        }
-        k_shade<<<grid, kBlock, 0, st>>>(g.ds, F, P, count);
-        if (stats) {
-            HIP_OK(hipEventRecord(ev[1].b, st));
             HIP_OK(hipEventRecord(ev[2].a, st));
         }
-        k_shadow<<<grid, kBlock, 0, st>>>(g.ds, P);
+        k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done);
         if (stats) {
             HIP_OK(hipEventRecord(ev[2].b, st));
-            HIP_OK(hipEventRecord(ev[3].a, st));
+            HIP_OK(hipEventRecord(ev[1].a, st));
         }
-        k_finalize<<<grid, kBlock, 0, st>>>(g.ds, F, P, count);
-        if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
+        k_shade<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, n_closest);
+        if (stats) HIP_OK(hipEventRecord(ev[1].b, st));
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(host_ctr, P.ctr, kCtrCount * sizeof(int), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
         if (host_ctr[kCtrError]) throw Failure{LAMBDA_EHIP, "device reported an inconsistent hit record"};
         if (stats) {
             stats->iterations++;
-            stats->closest_rays += count;
-            stats->shadow_rays += host_ctr[kCtrShadow];
-            for (int k = 0; k < 4; k++) {
+            stats->closest_rays += n_closest;
+            stats->shadow_rays += n_shadow;
+            for (int k = 0; k < 3; k++) {
                 float ms = 0;
                 HIP_OK(hipEventElapsedTime(&ms, ev[k].a, ev[k].b));
                 stats->kernel_ms[k] += ms;
                 stats->launches[k]++;
             }
         }
-        count = host_ctr[kCtrNext];
+        n_closest = host_ctr[kCtrNext];
+        n_shadow = host_ctr[kCtrShadow];
+        n_done = host_ctr[kCtrDone];
         std::swap(P.q_in, P.q_out);
         if (progress) {
             long long done = host_ctr[kCtrPixels];
