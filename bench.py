@@ -89,6 +89,27 @@ def cpu_baseline(scene, seconds: float, threads: int) -> dict:
                           r0, r1, p.height, (r1 - r0) * p.width, spp, t)}
 
 
+def copy_probe(nbytes: int) -> None:
+    """One device-to-device hipMemcpy of nbytes through the HIP runtime the
+    integrator itself uses (the runtime's copy-buffer kernel reads and writes
+    nbytes each): the known-size point that scales hardware byte counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL)
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)):
+            raise RuntimeError("hipMalloc of %d bytes failed" % nbytes)
+    try:
+        if hip.hipMemset(src, 1, C.c_size_t(nbytes)) or hip.hipDeviceSynchronize():
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) or hip.hipDeviceSynchronize():  # 3: device to device
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    log("copy probe: %d bytes read, %d bytes written" % (nbytes, nbytes))
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +122,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stats-pixels", type=int, default=20000, help="pixels sampled to count traversal work")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="first run one device-to-device copy of BYTES (a plain copy kernel of known traffic, "
+                         "the calibration point for hardware byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -114,6 +138,8 @@ def main() -> int:
 
     name, cus = lambda_amd.device_info()
     log("device %s, %d CUs" % (name, cus))
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     t0 = time.perf_counter()
     path, cfg = make_workload(args.workload, args.scene_dir)
     t_gen = time.perf_counter() - t0

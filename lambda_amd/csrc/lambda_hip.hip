@@ -1309,7 +1309,8 @@ int lambda_hip_device_info(char *name, size_t name_len, int32_t *compute_units, 
         HIP_OK(hipGetDevice(&dev));
         hipDeviceProp_t prop;
         HIP_OK(hipGetDeviceProperties(&prop, dev));
-        if (name && name_len) set_err(name, name_len, std::string(prop.name) + " (" + prop.gcnArchName + ")");
+        std::string label = prop.name[0] ? std::string(prop.name) + " (" + prop.gcnArchName + ")" : prop.gcnArchName;
+        if (name && name_len) set_err(name, name_len, label);
         if (compute_units) *compute_units = prop.multiProcessorCount;
     });
 }
