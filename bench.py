@@ -118,6 +118,8 @@ def main() -> int:
     ap.add_argument("--workload", default="instanced_mesh", choices=["instanced_mesh", "closed_box", "thinfilm_grid"])
     ap.add_argument("--rng", default="reference", choices=["reference", "throughput"])
     ap.add_argument("--path-slots", type=int, default=0)
+    ap.add_argument("--tail-paths", type=int, default=0,
+                    help="live paths below which the render finishes in one persistent launch (0 automatic, -1 never)")
     ap.add_argument("--scene-dir", default=os.environ.get("LAMBDA_SCENE_DIR", "/tmp/lambda_bench"))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -164,20 +166,20 @@ def main() -> int:
 
     for i in range(args.warmup):
         t0 = time.perf_counter()
-        dev.render(rng_mode=mode, path_slots=args.path_slots)
+        dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths)
         log("warmup %d: %.3f s" % (i, time.perf_counter() - t0))
 
     t0 = time.perf_counter()
     for i in range(args.steps):
-        img = dev.render(rng_mode=mode, path_slots=args.path_slots)
+        img = dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths)
     elapsed = time.perf_counter() - t0
     value = paths_per_step * args.steps / elapsed
     log("timed %d steps: %.3f s -> %.4g paths/s" % (args.steps, elapsed, value))
     finite = bool(np.all(np.isfinite(img)))
 
     # Kernel timing from HIP events on the render stream.
-    _, st = dev.render_stats(rng_mode=mode, path_slots=args.path_slots)
-    log("kernel ms: " + ", ".join("%s %.1f" % (k, st[k + "_ms"]) for k in ("trace", "shade", "finalize")))
+    _, st = dev.render_stats(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths)
+    log("kernel ms: " + ", ".join("%s %.1f" % (k, st[k + "_ms"]) for k in ("trace", "shade", "finalize", "persist")))
 
     # Algorithmic bytes per closest-hit ray, counted by the oracle on a
     # strided pixel sample (all samples of each sampled pixel).
@@ -235,7 +237,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
-        "kernels_ms": {k: st[k + "_ms"] for k in ("trace", "shade", "finalize")},
+        "kernels_ms": {k: st[k + "_ms"] for k in ("trace", "shade", "finalize", "persist")},
         "rays": {"closest": st["closest_rays"], "shadow": st["shadow_rays"], "iterations": st["iterations"]},
         "image_finite": finite,
         "device": name,

@@ -181,6 +181,9 @@ typedef struct lambda_render_params {
     uint64_t seed;         /* the reference always passes 42 */
     int32_t rng_mode;      /* enum lambda_rng_mode */
     int32_t path_slots;    /* paths in flight (throughput mode); 0 = automatic */
+    int32_t tail_paths;    /* once fewer paths than this are live, one persistent
+                              launch runs each remaining path to its end instead
+                              of one launch per bounce; 0 = automatic, -1 = never */
 } lambda_render_params;
 
 /*
@@ -220,7 +223,7 @@ typedef struct lambda_render_stats {
     int64_t closest_rays;    /* closest-hit rays traced */
     int64_t shadow_rays;     /* next-event occlusion rays traced */
     double kernel_ms[4];     /* total device time: trace (closest-hit and occlusion
-                                rays), shade, finalize, unused */
+                                rays), shade, finalize, persistent tail */
     int64_t launches[4];
     double render_ms;        /* wall time of the whole call */
 } lambda_render_stats;

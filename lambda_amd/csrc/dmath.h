@@ -30,6 +30,10 @@ struct V3 {
     float x, y, z;
 };
 
+// Bit casts usable on host and device.
+LAM_FN int as_int(float x) { return __builtin_bit_cast(int, x); }
+LAM_FN float as_float(int x) { return __builtin_bit_cast(float, x); }
+
 LAM_FN V3 v3(float a, float b, float c) { return V3{a, b, c}; }
 LAM_FN V3 v3(float4 a) { return V3{a.x, a.y, a.z}; }
 LAM_FN V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }

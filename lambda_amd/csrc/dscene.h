@@ -1,14 +1,18 @@
 // Device scene: layout in HBM, ray/primitive tests, two-level BVH traversal,
 // textures, BSDFs, thin-film transfer matrices and light sampling.
 //
-// Layout (all arrays 16-byte records, gathered by lane):
-//   node_lo[n] = {min.xyz, bits(index)}, node_hi[n] = {max.xyz, bits(count)}
-//     one node = two 16-byte loads; count > 0 marks a leaf (scene/bvh.h:14-28)
-//   obj_a[n] = {corner | centre, radius}, obj_b[n] = {horizontal, bits(type)},
-//   obj_c[n] = {vertical, 0}, obj_n[n] = {cross(horizontal, vertical), 0}
-//     the intersection test reads a (spheres) or a, b, c (quads, triangles);
-//     obj_n and the triangle shading record obj_s[n][4] are read once per
-//     closest hit
+// Layout:
+//   wide[8 n .. 8 n + 7]: a node of the 4-wide BVH, 128 bytes, one cache line:
+//     {lo.x[4]}, {hi.x[4]}, {lo.y[4]}, {hi.y[4]}, {lo.z[4]}, {hi.z[4]},
+//     {bits(child)[4]}, {bits(count)[4]}; per slot count == 0: inner child at
+//     wide node `child`, count > 0: leaf over [child, child + count) (objects,
+//     or the first instance at the instance level), count < 0: empty slot.
+//     Every box is a node box of the reference's own BVH (scene/bvh.h:14-28).
+//   inst_root[i]: wide node of instance i's object-level BVH.
+//   obj[4 n .. 4 n + 3] = {corner | centre, radius}, {horizontal, bits(type)},
+//     {vertical, 0}, {cross(horizontal, vertical), 0}: one 64-byte record, so
+//     a triangle test reads one cache line; obj_s[n][4] is the triangle
+//     shading record, read once per closest hit.
 //   inst[n][8] = {M rows 0-3, M^-1 rows 0-3}, inst_i[n] = {object, count,
 //     BLAS root, material}
 //   spectra are 471-bin float tables, complex spectra 471-bin float2 tables.
@@ -21,8 +25,10 @@
 namespace lam {
 
 struct DeviceScene {
-    const float4 *node_lo, *node_hi;
-    const float4 *obj_a, *obj_b, *obj_c, *obj_n, *obj_s;
+    const float4 *wide;     // 4-wide BVH nodes, 8 float4 each
+    const int *inst_root;   // [instances] wide root of the object-level BVH
+    const float4 *obj;      // [objects][4]
+    const float4 *obj_s;    // [objects][4] triangle shading records
     const int *obj_mat;
     const float4 *inst;
     const int4 *inst_i;
@@ -135,9 +141,9 @@ LAM_FN bool test_tri(V3 c, V3 h, V3 v, V3 o, V3 d, float best, float &t_out, flo
 // Full intersection record of object k for a local-space ray, as the
 // reference's intersect* fills it (scene/object.h:301-400).
 LAM_FN bool object_record(const DeviceScene &S, int k, V3 o, V3 d, float best, Hit &h) {
-    float4 a = S.obj_a[k];
-    float4 b = S.obj_b[k];
-    int type = __float_as_int(b.w);
+    float4 a = S.obj[4 * size_t(k)];
+    float4 b = S.obj[4 * size_t(k) + 1];
+    int type = as_int(b.w);
     float t, al = 0, be = 0;
     if (type == LAMBDA_OBJECT_SPHERE) {
         if (!test_sphere(a, o, d, best, t)) return false;
@@ -148,8 +154,8 @@ LAM_FN bool object_record(const DeviceScene &S, int k, V3 o, V3 d, float best, H
         h.v = acosf(-h.n.y) / kPi;
         h.front = dot(d, h.n) < 0;
     } else {
-        float4 c = S.obj_c[k];
-        float4 nn = S.obj_n[k];
+        float4 c = S.obj[4 * size_t(k) + 2];
+        float4 nn = S.obj[4 * size_t(k) + 3];
         V3 nrm = v3(nn);
         if (type == LAMBDA_OBJECT_TRI) {
             if (!test_tri(v3(a), v3(b), v3(c), o, d, best, t, al, be)) return false;
@@ -199,13 +205,21 @@ LAM_FN bool hit_record(const DeviceScene &S, int inst, int obj, V3 o, V3 d, Hit 
 
 // ------------------------------------------------------------- traversal
 //
-// Two-level BVH traversal (scene/scene.h:51-124) over the reference's own
-// nodes, nearest child first.  At an inner node both children's boxes are
-// tested; the nearer is entered and the farther pushed.  A node is skipped
-// when its entry distance exceeds the current closest hit -- the reference's
-// test -- and a popped node is re-tested against the hit found meanwhile.
-// Only the first instance of an instance-level leaf is entered, as in the
-// reference (scene/scene.h:79-86).
+// Two-level BVH traversal (scene/scene.h:51-124) over a 4-wide BVH whose
+// boxes and leaves are the reference's own: each wide node holds up to four
+// descendants of one reference node, chosen at upload by expanding the
+// child of largest surface area.  A reference node's box contains its
+// descendants' boxes exactly, and the slab test is monotone in the box
+// bounds, so a descendant whose box is hit (and not beyond the closest hit)
+// has every intermediate box hit as well: testing the four descendants
+// directly finds the same candidates as the reference's walk.
+//
+// At a wide node all four boxes are tested; object leaves are intersected
+// at once, inner children (and instance leaves at the instance level) are
+// sorted by entry distance, the nearest is entered and the others pushed
+// with their entry distance, so that a popped entry beyond the closest hit
+// is dropped without a memory access.  Only the first instance of an
+// instance-level leaf is entered, as in the reference (scene/scene.h:79-86).
 //
 // Visiting order changes which of several primitives at exactly the same
 // distance is met first.  The reference keeps the first one in its own
@@ -213,24 +227,22 @@ LAM_FN bool hit_record(const DeviceScene &S, int inst, int obj, V3 o, V3 d, Hit 
 // an exact tie the candidate with the smaller reference rank wins:
 // rank = (instance-level leaf order << 32) | object order within its BVH,
 // both precomputed on the host by replaying that order.
-//
-// Stacks live in LDS: entry k of a lane is at stk[k * stride + lane] for the
-// instance level and at stk[(kStack + k) * stride + lane] for the object
-// level, so a wave's pushes and pops touch 64 distinct banks.
 
-LAM_FN bool slab(float4 lo, float4 hi, V3 o, V3 inv, float &tnear) {
+// Slab test of one box (scene/bvh.h:30-52): entry distance clamped at 0,
+// exit clamped at 1e30.
+LAM_FN bool slab(float lox, float loy, float loz, float hix, float hiy, float hiz, V3 o, V3 inv, float &tnear) {
     float t1 = 0, t2 = kMax;
-    float a = (lo.x - o.x) * inv.x, b = (hi.x - o.x) * inv.x;
+    float a = (lox - o.x) * inv.x, b = (hix - o.x) * inv.x;
     if (inv.x < 0) { float s = a; a = b; b = s; }
     t1 = fmaxf(a, t1);
     t2 = fminf(b, t2);
-    a = (lo.y - o.y) * inv.y;
-    b = (hi.y - o.y) * inv.y;
+    a = (loy - o.y) * inv.y;
+    b = (hiy - o.y) * inv.y;
     if (inv.y < 0) { float s = a; a = b; b = s; }
     t1 = fmaxf(a, t1);
     t2 = fminf(b, t2);
-    a = (lo.z - o.z) * inv.z;
-    b = (hi.z - o.z) * inv.z;
+    a = (loz - o.z) * inv.z;
+    b = (hiz - o.z) * inv.z;
     if (inv.z < 0) { float s = a; a = b; b = s; }
     t1 = fmaxf(a, t1);
     t2 = fminf(b, t2);
@@ -238,36 +250,27 @@ LAM_FN bool slab(float4 lo, float4 hi, V3 o, V3 inv, float &tnear) {
     return t1 <= t2 && t2 >= 0;
 }
 
-struct TraceStats {
-    uint32_t nodes, instances, prims;
-};
-
 LAM_FN uint64_t ref_rank(const DeviceScene &S, int inst, int obj) {
     return (uint64_t(uint32_t(S.tlas_rank[inst])) << 32) | uint64_t(uint32_t(S.obj_rank[obj]));
 }
 
-// Test of object k against the (local) ray.  A closer hit always wins; a hit
-// at exactly the current distance wins when it comes first in the
-// reference's order (closest-hit mode only).
+// Test of object k, whose record starts {a, b, c}, against the (local) ray.
+// A closer hit always wins; a hit at exactly the current distance wins when
+// it comes first in the reference's order (closest-hit mode only).
 template <bool kAny>
-LAM_FN bool test_object_ranked(const DeviceScene &S, int k, V3 o, V3 d, float &best, int inst, int &hit_inst,
-                               int &hit_obj) {
-    float4 a = S.obj_a[k];
-    float4 b = S.obj_b[k];
-    int type = __float_as_int(b.w);
+LAM_FN bool test_object_ranked(const DeviceScene &S, int k, float4 a, float4 b, float4 c, V3 o, V3 d, float &best,
+                               int inst, int &hit_inst, int &hit_obj) {
+    int type = as_int(b.w);
     // accept t <= best here, decide ties below
     float limit = kAny ? best : nextafterf(best, INFINITY);
     float t, al, be;
     if (type == LAMBDA_OBJECT_SPHERE) {
         if (!test_sphere(a, o, d, limit, t)) return false;
+    } else if (type == LAMBDA_OBJECT_TRI) {
+        if (!test_tri(v3(a), v3(b), v3(c), o, d, limit, t, al, be)) return false;
     } else {
-        float4 c = S.obj_c[k];
-        if (type == LAMBDA_OBJECT_TRI) {
-            if (!test_tri(v3(a), v3(b), v3(c), o, d, limit, t, al, be)) return false;
-        } else {
-            float4 n = S.obj_n[k];
-            if (!test_quad(v3(a), v3(b), v3(c), v3(n), o, d, limit, t, al, be)) return false;
-        }
+        float4 n = S.obj[4 * size_t(k) + 3];
+        if (!test_quad(v3(a), v3(b), v3(c), v3(n), o, d, limit, t, al, be)) return false;
     }
     if (!kAny && t == best && hit_obj >= 0 && ref_rank(S, inst, k) > ref_rank(S, hit_inst, hit_obj))
         return false;
@@ -277,106 +280,162 @@ LAM_FN bool test_object_ranked(const DeviceScene &S, int k, V3 o, V3 d, float &b
     return true;
 }
 
+// Entries that wait for a visit: a wide node (code >= 0) or an instance
+// (code = -1 - instance), with the entry distance of its box.  The top
+// kRing entries of each lane live in LDS (entry j at slot j mod kRing,
+// stride = block size, so a wave touches 64 consecutive words); deeper
+// entries are moved to a global area laid out [entry][thread] when the ring
+// is full and brought back when the stack shrinks to them.  The host sizes
+// the global area from the exact worst-case depth of the tree.
+template <int kRing>
+struct TraversalStack {
+    int *lcode;          // LDS, kRing * stride
+    float *ldist;        // LDS, kRing * stride
+    int stride;
+    int2 *spill;         // global, entry j of this thread at spill[j * spill_stride]
+    size_t spill_stride;
+    int sp = 0;          // entries on the stack
+    int lo = 0;          // entries below lo are in the global area
+
+    LAM_FN void push(int code, float dist) {
+        if (sp - lo == kRing) {
+            const int s = (lo & (kRing - 1)) * stride;
+            spill[size_t(lo) * spill_stride] = make_int2(lcode[s], as_int(ldist[s]));
+            lo++;
+        }
+        const int s = (sp & (kRing - 1)) * stride;
+        lcode[s] = code;
+        ldist[s] = dist;
+        sp++;
+    }
+    LAM_FN void pop(int &code, float &dist) {
+        sp--;
+        if (sp < lo) {
+            int2 e = spill[size_t(sp) * spill_stride];
+            code = e.x;
+            dist = as_float(e.y);
+            lo = sp;
+        } else {
+            const int s = (sp & (kRing - 1)) * stride;
+            code = lcode[s];
+            dist = ldist[s];
+        }
+    }
+};
+
+// Sorts up to four (distance, code) pairs by distance (a five-exchange
+// network); misses carry +inf.
+LAM_FN void sort4(float (&t)[4], int (&c)[4]) {
+#define LAM_CSWAP(i, j)                                            \
+    if (t[j] < t[i]) {                                             \
+        float tt = t[i]; t[i] = t[j]; t[j] = tt;                   \
+        int cc = c[i]; c[i] = c[j]; c[j] = cc;                     \
+    }
+    LAM_CSWAP(0, 1)
+    LAM_CSWAP(2, 3)
+    LAM_CSWAP(0, 2)
+    LAM_CSWAP(1, 3)
+    LAM_CSWAP(1, 2)
+#undef LAM_CSWAP
+}
+
 // kAny = false: closest hit; returns true if anything was hit, with
 //   (hit_inst, hit_obj, best).
 // kAny = true: occlusion; best is the light distance, (skip_inst, skip_obj)
 //   the light itself; returns true on the first blocking hit.
-template <bool kAny, bool kStats>
+template <bool kAny, int kRing>
 LAM_FN bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_inst, int &hit_obj, int skip_inst,
-                     int skip_obj, int *stk, int stride, TraceStats &st) {
+                     int skip_obj, TraversalStack<kRing> &stk) {
     const V3 inv = v3(1 / d.x, 1 / d.y, 1 / d.z);
-    int *top = stk;
-    int *sub = stk + kStack * stride;
-    int sp = 0, ssp = 0;
     bool found = false;
     bool in_blas = false;
+    int blas_base = 0;
     int inst = -1;
-    V3 lo_o = o, lo_d = d, linv = inv;
-    // current node (already tested): index and count words
-    float tn;
-    float4 nlo = S.node_lo[0], nhi = S.node_hi[0];
-    if (kStats) st.nodes++;
-    if (!slab(nlo, nhi, o, inv, tn) || tn > best) return false;
-    int cur = 0;
-    int idx = __float_as_int(nlo.w), cnt = __float_as_int(nhi.w);
+    V3 ro = o, rd = d, ri = inv;   // ray of the current level
+    int cur = 0;                   // wide node (>= 0) or instance (< 0) to visit
     while (true) {
-        bool have_next = false;
-        if (cnt > 0) {
-            if (!in_blas) {
-                // instance-level leaf: enter the first instance's BVH
-                inst = idx;
-                if (kStats) st.instances++;
-                local_ray(S, inst, o, d, lo_o, lo_d);
-                linv = v3(1 / lo_d.x, 1 / lo_d.y, 1 / lo_d.z);
-                const int root = S.inst_i[inst].z;
-                float4 rlo = S.node_lo[root], rhi = S.node_hi[root];
-                if (kStats) st.nodes++;
-                if (slab(rlo, rhi, lo_o, linv, tn) && !(tn > best)) {
-                    in_blas = true;
-                    cur = root;
-                    idx = __float_as_int(rlo.w);
-                    cnt = __float_as_int(rhi.w);
-                    continue;
-                }
+        if (cur < 0) {
+            // enter an instance: its object-level BVH in local space
+            inst = -1 - cur;
+            local_ray(S, inst, o, d, ro, rd);
+            ri = v3(1 / rd.x, 1 / rd.y, 1 / rd.z);
+            in_blas = true;
+            blas_base = stk.sp;
+            cur = S.inst_root[inst];
+        }
+        const float4 *w = S.wide + 8 * size_t(cur);
+        const float4 lx = w[0], hx = w[1], ly = w[2], hy = w[3], lz = w[4], hz = w[5];
+        const float4 cw = w[6], nw = w[7];
+        const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, HX[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
+        const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
+        const int CH[4] = {as_int(cw.x), as_int(cw.y), as_int(cw.z), as_int(cw.w)};
+        const int CN[4] = {as_int(nw.x), as_int(nw.y), as_int(nw.z), as_int(nw.w)};
+        float key[4];
+        int code[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            float tn;
+            bool hit = CN[j] >= 0 && slab(LX[j], LY[j], LZ[j], HX[j], HY[j], HZ[j], ro, ri, tn) && !(tn > best);
+            key[j] = INFINITY;
+            code[j] = 0;
+            if (!hit) continue;
+            if (CN[j] == 0) {
+                key[j] = tn;
+                code[j] = CH[j];
+            } else if (!in_blas) {
+                key[j] = tn;
+                code[j] = -1 - CH[j];
             } else {
-                for (int k = idx; k < idx + cnt; k++) {
-                    if (kAny && inst == skip_inst && k == skip_obj) continue;
-                    if (kStats) st.prims++;
-                    if (test_object_ranked<kAny>(S, k, lo_o, lo_d, best, inst, hit_inst, hit_obj)) {
+                // objects two at a time, both records requested before
+                // either is tested (a lone last object is read twice)
+                const int end = CH[j] + CN[j];
+                for (int k = CH[j]; k < end; k += 2) {
+                    const bool two = k + 1 < end;
+                    const float4 *r0 = S.obj + 4 * size_t(k);
+                    const float4 *r1 = two ? r0 + 4 : r0;
+                    const float4 a0 = r0[0], b0 = r0[1], c0 = r0[2];
+                    const float4 a1 = r1[0], b1 = r1[1], c1 = r1[2];
+                    if (!(kAny && inst == skip_inst && k == skip_obj) &&
+                        test_object_ranked<kAny>(S, k, a0, b0, c0, ro, rd, best, inst, hit_inst, hit_obj)) {
+                        if (kAny) return true;
+                        found = true;
+                    }
+                    if (two && !(kAny && inst == skip_inst && k + 1 == skip_obj) &&
+                        test_object_ranked<kAny>(S, k + 1, a1, b1, c1, ro, rd, best, inst, hit_inst, hit_obj)) {
                         if (kAny) return true;
                         found = true;
                     }
                 }
             }
-        } else {
-            const V3 &ro = in_blas ? lo_o : o;
-            const V3 &ri = in_blas ? linv : inv;
-            const int l = cur + 1, r = idx;
-            float4 llo = S.node_lo[l], lhi = S.node_hi[l];
-            float4 rlo = S.node_lo[r], rhi = S.node_hi[r];
-            if (kStats) st.nodes += 2;
-            float tl, tr;
-            bool hl = slab(llo, lhi, ro, ri, tl) && !(tl > best);
-            bool hr = slab(rlo, rhi, ro, ri, tr) && !(tr > best);
-            if (hl && hr) {
-                bool left_first = tl <= tr;
-                int far = left_first ? r : l;
-                int *stack = in_blas ? sub : top;
-                int &ptr = in_blas ? ssp : sp;
-                stack[ptr * stride] = far;
-                ptr++;
-                cur = left_first ? l : r;
-                idx = __float_as_int(left_first ? llo.w : rlo.w);
-                cnt = __float_as_int(left_first ? lhi.w : rhi.w);
-                continue;
-            }
-            if (hl || hr) {
-                cur = hl ? l : r;
-                idx = __float_as_int(hl ? llo.w : rlo.w);
-                cnt = __float_as_int(hl ? lhi.w : rhi.w);
-                continue;
-            }
         }
-        // pop the next node that is still worth visiting
-        while (!have_next) {
-            int cand;
-            if (in_blas) {
-                if (ssp == 0) {
-                    in_blas = false;
-                    continue;
-                }
-                cand = sub[--ssp * stride];
-            } else {
-                if (sp == 0) return found;
-                cand = top[--sp * stride];
+        sort4(key, code);
+        // entries beyond a hit found at this node are dropped
+        int m = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) m += !(key[j] > best) ? 1 : 0;
+        if (m > 0) {
+            for (int j = m - 1; j >= 1; j--) stk.push(code[j], key[j]);
+            cur = code[0];
+            continue;
+        }
+        // pop the next entry still worth visiting
+        bool next = false;
+        while (!next) {
+            if (in_blas && stk.sp == blas_base) {
+                in_blas = false;
+                ro = o;
+                rd = d;
+                ri = inv;
+                continue;
             }
-            float4 clo = S.node_lo[cand], chi = S.node_hi[cand];
-            if (kStats) st.nodes++;
-            if (slab(clo, chi, in_blas ? lo_o : o, in_blas ? linv : inv, tn) && !(tn > best)) {
-                cur = cand;
-                idx = __float_as_int(clo.w);
-                cnt = __float_as_int(chi.w);
-                have_next = true;
+            if (stk.sp == 0) return found;
+            int c;
+            float t;
+            stk.pop(c, t);
+            if (!(t > best)) {
+                cur = c;
+                next = true;
             }
         }
     }
@@ -386,7 +445,7 @@ LAM_FN bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_ins
 
 LAM_FN float scalar_texture(const DeviceScene &S, int t, V3 p) {
     float4 x = S.stex[t];
-    int type = __float_as_int(x.x);
+    int type = as_int(x.x);
     if (type == LAMBDA_SCALAR_CONSTANT) return x.y;
     if (type == LAMBDA_SCALAR_PERLIN) return x.y + (x.z - x.y) * perlin(S.perm, p * x.w);
     const int4 box = S.worley_box[t], dim = S.worley_dim[t];
@@ -400,7 +459,7 @@ LAM_FN float spectrum_texture(const DeviceScene &S, int t, const Hit &h, float l
     int4 x = S.xtex[t];
     if (x.x == LAMBDA_SPECTRUM_CONSTANT) return dense(S.spectra + size_t(x.y) * kBins, lambda);
     if (x.x == LAMBDA_SPECTRUM_CHECKER) {
-        float scale = __int_as_float(x.w);
+        float scale = as_float(x.w);
         int cx = int(floorf(h.u / scale));
         int cy = int(floorf(h.v / scale));
         int s = (cx + cy) % 2 == 0 ? x.y : x.z;
@@ -448,7 +507,7 @@ LAM_FN float spectrum_texture_average(const DeviceScene &S, int t) {
     if (x.x == LAMBDA_SPECTRUM_CONSTANT) return spectrum_average(S, x.y);
     if (x.x == LAMBDA_SPECTRUM_CHECKER) return float(0.5) * (spectrum_average(S, x.y) + spectrum_average(S, x.z));
     float4 st = S.stex[x.y];
-    return __float_as_int(st.x) == LAMBDA_SCALAR_CONSTANT ? st.y : (st.y + st.z) / 2;
+    return as_int(st.x) == LAMBDA_SCALAR_CONSTANT ? st.y : (st.y + st.z) / 2;
 }
 
 // ---------------------------------------------------------------- objects
@@ -456,17 +515,17 @@ LAM_FN float spectrum_texture_average(const DeviceScene &S, int t) {
 // (scene/instance.h:103-111).
 
 LAM_FN float object_area(const DeviceScene &S, int k) {
-    float4 a = S.obj_a[k];
-    int type = __float_as_int(S.obj_b[k].w);
+    float4 a = S.obj[4 * size_t(k)];
+    int type = as_int(S.obj[4 * size_t(k) + 1].w);
     if (type == LAMBDA_OBJECT_SPHERE) return 4 * kPi * a.w * a.w;
-    float l = len(v3(S.obj_n[k]));
+    float l = len(v3(S.obj[4 * size_t(k) + 3]));
     return type == LAMBDA_OBJECT_QUAD ? l : l / 2;
 }
 
 LAM_FN float object_pdf(const DeviceScene &S, int k, V3 p, V3 d) {
-    float4 a = S.obj_a[k];
-    float4 b = S.obj_b[k];
-    int type = __float_as_int(b.w);
+    float4 a = S.obj[4 * size_t(k)];
+    float4 b = S.obj[4 * size_t(k) + 1];
+    int type = as_int(b.w);
     if (type == LAMBDA_OBJECT_SPHERE) {
         V3 oc = v3(a) - p;
         float dist2 = len2(oc);
@@ -477,7 +536,7 @@ LAM_FN float object_pdf(const DeviceScene &S, int k, V3 p, V3 d) {
         if (ct < cmax) return 0;
         return 1 / (2 * kPi * (1 - cmax));
     }
-    V3 c = v3(a), h = v3(b), v = v3(S.obj_c[k]), n = v3(S.obj_n[k]);
+    V3 c = v3(a), h = v3(b), v = v3(S.obj[4 * size_t(k) + 2]), n = v3(S.obj[4 * size_t(k) + 3]);
     if (type == LAMBDA_OBJECT_QUAD) {
         float ct = dot(normalize(n), d);
         if (fabsf(ct) < kEps) return 0;
@@ -507,9 +566,9 @@ LAM_FN float object_pdf(const DeviceScene &S, int k, V3 p, V3 d) {
 }
 
 LAM_FN V3 object_sample(const DeviceScene &S, int k, V3 p, Pcg &rng) {
-    float4 a = S.obj_a[k];
-    float4 b = S.obj_b[k];
-    int type = __float_as_int(b.w);
+    float4 a = S.obj[4 * size_t(k)];
+    float4 b = S.obj[4 * size_t(k) + 1];
+    int type = as_int(b.w);
     if (type == LAMBDA_OBJECT_SPHERE) {
         V3 oc = v3(a) - p;
         float dist2 = len2(oc);
@@ -524,7 +583,7 @@ LAM_FN V3 object_sample(const DeviceScene &S, int k, V3 p, Pcg &rng) {
         u = 1 - u;
         v = 1 - v;
     }
-    V3 s = v3(a) + v3(b) * u + v3(S.obj_c[k]) * v;
+    V3 s = v3(a) + v3(b) * u + v3(S.obj[4 * size_t(k) + 2]) * v;
     return normalize(s - p);
 }
 

@@ -30,6 +30,7 @@
 
 #include "lambda_hip.h"
 #include "dscene.h"
+#include "scene_prep.h"
 
 namespace lam {
 #include "spectral_tables.inc"
@@ -184,6 +185,31 @@ __device__ __forceinline__ Pcg load_rng(const Paths &P, int s) {
 
 // ---------------------------------------------------------------- kernels
 
+// Traversal stacks: kRing entries per lane in LDS, deeper entries in a
+// global area of the launch, [entry][thread] (allocated by the host when the
+// scene's worst-case depth exceeds kRing).
+constexpr int kRing = 16;
+// Live paths below which a render switches to the persistent schedule.
+constexpr long long kAutoTailPaths = 65536;
+using Stack = TraversalStack<kRing>;
+
+__device__ __forceinline__ Stack make_stack(int *s_code, float *s_dist, int2 *spill) {
+    Stack s;
+    s.lcode = s_code + threadIdx.x;
+    s.ldist = s_dist + threadIdx.x;
+    s.stride = kBlock;
+    const size_t threads = size_t(gridDim.x) * blockDim.x;
+    s.spill = spill ? spill + (size_t(blockIdx.x) * blockDim.x + threadIdx.x) : nullptr;
+    s.spill_stride = threads;
+    return s;
+}
+
+template <bool kAny>
+__device__ __forceinline__ bool trace_ray(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_inst, int &hit_obj,
+                                         int skip_inst, int skip_obj, Stack &stk) {
+    return lam::traverse<kAny, kRing>(S, o, d, best, hit_inst, hit_obj, skip_inst, skip_obj, stk);
+}
+
 // First sample of every slot.  Reference mode: slot s owns the pixel of
 // rank s for all its samples.  Throughput mode: slot s takes work item s.
 __global__ void __launch_bounds__(kBlock) k_start(DeviceScene S, Frame F, Paths P, int nslots) {
@@ -203,17 +229,19 @@ __global__ void __launch_bounds__(kBlock) k_start(DeviceScene S, Frame F, Paths 
 // threads [n_closest, n_closest + n_shadow) test queued occlusion ray
 // t - n_closest and, if the light is visible, add its contribution to the
 // path's radiance.
-__global__ void __launch_bounds__(kBlock) k_trace(DeviceScene S, Paths P, int n_closest, int n_shadow) {
-    __shared__ int stk[2 * kStack * kBlock];
+__global__ void __launch_bounds__(kBlock) k_trace(DeviceScene S, Paths P, int n_closest, int n_shadow,
+                                                  int2 *spill) {
+    __shared__ int s_code[kRing * kBlock];
+    __shared__ float s_dist[kRing * kBlock];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) P.ctr[kCtrNext] = 0;
-    TraceStats st{};
+    Stack stk = make_stack(s_code, s_dist, spill);
     if (t < n_closest) {
         const int s = P.q_in[t];
         float4 o = P.o[s], d = P.d[s];
         float best = kMax;
         int hi = -1, ho = -1;
-        traverse<false, false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk + threadIdx.x, kBlock, st);
+        trace_ray<false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk);
         P.hit[s] = make_int2(hi, ho);
     } else if (t < n_closest + n_shadow) {
         const int q = t - n_closest;
@@ -221,8 +249,7 @@ __global__ void __launch_bounds__(kBlock) k_trace(DeviceScene S, Paths P, int n_
         int4 si = P.s_i[q];
         float best = so.w;
         int hi = -1, ho = -1;
-        bool blocked = traverse<true, false>(S, v3(so), v3(sd), best, hi, ho, si.y, si.z, stk + threadIdx.x,
-                                             kBlock, st);
+        bool blocked = trace_ray<true>(S, v3(so), v3(sd), best, hi, ho, si.y, si.z, stk);
         if (!blocked) {
             float4 c = P.s_c[q];
             float4 L = P.L[si.x];
@@ -239,18 +266,21 @@ __device__ __forceinline__ S4 lambertian_albedo(const DeviceScene &S, int4 mt, c
     return spectrum_texture4(S, mt.y, h, lam);
 }
 
-// Shading of every queued path: one iteration body of Renderer::trace
-// (render/renderer.h:102-192) after the closest-hit query.
-__global__ void __launch_bounds__(kBlock) k_shade(DeviceScene S, Frame F, Paths P, int count) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = q < count;
-    bool push_shadow = false;
-    bool path_done = false;
-    int s = 0;
-    float4 sh_o, sh_d, sh_c;
-    int4 sh_i;
-    if (active) {
-        s = P.q_in[q];
+// Occlusion ray queued by a shading step: origin and light distance,
+// direction, contribution, and {slot, light instance, light object}.
+struct ShadowRay {
+    float4 o, d, c;
+    int4 i;
+};
+
+// One iteration body of Renderer::trace (render/renderer.h:102-192) for the
+// path in slot s after its closest-hit query (P.hit[s]).  Updates the slot's
+// state; returns true when the sample's path has ended.  *push_shadow tells
+// whether next-event estimation produced an occlusion ray.
+__device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F, const Paths &P, int s,
+                                           bool &push_shadow, ShadowRay &sh) {
+    push_shadow = false;
+    {
         float4 o4 = P.o[s], d4 = P.d[s];
         int4 info = P.info[s];
         int2 hit = P.hit[s];
@@ -342,10 +372,10 @@ __global__ void __launch_bounds__(kBlock) k_shade(DeviceScene S, Frame F, Paths 
                                 float w = power_heuristic(lprob, sp);
                                 S4 c = beta * att * ct * em * w / lprob;
                                 push_shadow = true;
-                                sh_o = make_float4(h.p.x, h.p.y, h.p.z, tmax);
-                                sh_d = make_float4(ldir.x, ldir.y, ldir.z, 0);
-                                sh_c = f4(c);
-                                sh_i = make_int4(s, lio.x, lio.y, 0);
+                                sh.o = make_float4(h.p.x, h.p.y, h.p.z, tmax);
+                                sh.d = make_float4(ldir.x, ldir.y, ldir.z, 0);
+                                sh.c = f4(c);
+                                sh.i = make_int4(s, lio.x, lio.y, 0);
                             }
                         }
                     }
@@ -433,14 +463,28 @@ __global__ void __launch_bounds__(kBlock) k_shade(DeviceScene S, Frame F, Paths 
         P.L[s] = f4(L);
         store_rng(P, s, rng);
         P.info[s] = make_int4(info.x, info.y, bounce + 1, (specular ? kFlagSpecular : 0) | (finished ? kFlagFinished : 0));
-        path_done = finished;
+        return finished;
+    }
+}
+
+// Shading of every queued path.
+__global__ void __launch_bounds__(kBlock) k_shade(DeviceScene S, Frame F, Paths P, int count) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = q < count;
+    bool push_shadow = false;
+    bool path_done = false;
+    int s = 0;
+    ShadowRay sh;
+    if (active) {
+        s = P.q_in[q];
+        path_done = shade_slot(S, F, P, s, push_shadow, sh);
     }
     int pos = wave_push(&P.ctr[kCtrShadow], push_shadow);
     if (push_shadow) {
-        P.s_o[pos] = sh_o;
-        P.s_d[pos] = sh_d;
-        P.s_c[pos] = sh_c;
-        P.s_i[pos] = sh_i;
+        P.s_o[pos] = sh.o;
+        P.s_d[pos] = sh.d;
+        P.s_c[pos] = sh.c;
+        P.s_i[pos] = sh.i;
     }
     // Continuing paths trace again next pass; ended ones wait for their
     // occlusion ray and are then retired by finalize.
@@ -489,6 +533,41 @@ __device__ __forceinline__ void sample_xyz(const DeviceScene &S, const Frame &F,
 // Retires the samples that ended in the previous shade pass (their
 // occlusion rays have been traced since): radiance to the framebuffer, then
 // the slot's next sample, which joins the closest-hit queue.
+// Adds the ended sample of slot s to its pixel's XYZ sum.  In reference
+// mode the pixel's next sample then starts in the same slot, continuing the
+// pixel's PCG32 stream (render/renderer.h:69-91); returns true when it did.
+__device__ __forceinline__ bool retire_sample(const DeviceScene &S, const Frame &F, const Paths &P, int s) {
+    int4 info = P.info[s];
+    float x, y, z;
+    sample_xyz(S, F, P.L[s], P.lam[s], x, y, z);
+    double *px = P.fb + size_t(info.x) * 3;
+    unsafeAtomicAdd(px + 0, double(x));
+    unsafeAtomicAdd(px + 1, double(y));
+    unsafeAtomicAdd(px + 2, double(z));
+    if (F.mode == LAMBDA_RNG_REFERENCE && info.y + 1 < F.spp) {
+        Pcg rng = load_rng(P, s);
+        camera_sample(S, F, P, s, info.x, info.y + 1, rng);
+        store_rng(P, s, rng);
+        return true;
+    }
+    atomicAdd(&P.ctr[kCtrPixels], 1);
+    return false;
+}
+
+// Throughput mode: work item w = sample * npix + rank starts in slot s with
+// its own PCG32 stream (seed, w).
+__device__ __forceinline__ void start_work(const DeviceScene &S, const Frame &F, const Paths &P, int s, long long w) {
+    int rank = int(w % F.npix);
+    int sample = int(w / F.npix);
+    int pixel = P.order[rank];
+    Pcg rng = pcg_seed(F.seed, uint64_t(w));
+    camera_sample(S, F, P, s, pixel, sample, rng);
+    store_rng(P, s, rng);
+}
+
+// Retires the samples that ended in the previous shade pass (their
+// occlusion rays have been traced since): radiance to the framebuffer, then
+// the slot's next sample, which joins the closest-hit queue.
 __global__ void __launch_bounds__(kBlock) k_finalize(DeviceScene S, Frame F, Paths P, int n_done) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q == 0) {
@@ -500,41 +579,68 @@ __global__ void __launch_bounds__(kBlock) k_finalize(DeviceScene S, Frame F, Pat
     int s = 0;
     if (q < n_done) {
         s = P.done[q];
-        int4 info = P.info[s];
-        float x, y, z;
-        sample_xyz(S, F, P.L[s], P.lam[s], x, y, z);
-        double *px = P.fb + size_t(info.x) * 3;
-        unsafeAtomicAdd(px + 0, double(x));
-        unsafeAtomicAdd(px + 1, double(y));
-        unsafeAtomicAdd(px + 2, double(z));
-        if (F.mode == LAMBDA_RNG_REFERENCE) {
-            if (info.y + 1 < F.spp) {
-                Pcg rng = load_rng(P, s);
-                camera_sample(S, F, P, s, info.x, info.y + 1, rng);
-                store_rng(P, s, rng);
-                keep = true;
-            } else {
-                atomicAdd(&P.ctr[kCtrPixels], 1);
-            }
-        } else {
-            atomicAdd(&P.ctr[kCtrPixels], 1);
-            need_work = true;
-        }
+        keep = retire_sample(S, F, P, s);
+        need_work = F.mode != LAMBDA_RNG_REFERENCE;
     }
     if (F.mode != LAMBDA_RNG_REFERENCE) {
         int w = wave_push(&P.ctr[kCtrWork], need_work);
         if (need_work && w < F.total_work) {
-            int rank = int(w % F.npix);
-            int sample = int(w / F.npix);
-            int pixel = P.order[rank];
-            Pcg rng = pcg_seed(F.seed, uint64_t(w));
-            camera_sample(S, F, P, s, pixel, sample, rng);
-            store_rng(P, s, rng);
+            start_work(S, F, P, s, w);
             keep = true;
         }
     }
     int pos = wave_push(&P.ctr[kCtrNext], keep);
     if (keep) P.q_out[pos] = s;
+}
+
+// Persistent per-thread schedule for the last paths of a render: each
+// thread takes one live slot and runs it to the end -- closest hit, shading,
+// the occlusion ray at once, retirement, the slot's next sample -- without
+// waiting for other paths.  Threads [0, n_in) take q_in (paths that need
+// their closest hit); the next P.ctr[kCtrNext] threads take q_out (samples
+// just started by k_finalize).  No occlusion ray or ended sample may be
+// pending when it starts.  In throughput mode threads keep taking work items
+// until none are left.
+__global__ void __launch_bounds__(kBlock) k_persist(DeviceScene S, Frame F, Paths P, int n_in, int2 *spill) {
+    __shared__ int s_code[kRing * kBlock];
+    __shared__ float s_dist[kRing * kBlock];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n_new = P.ctr[kCtrNext];
+    if (t >= n_in + n_new) return;
+    const int s = t < n_in ? P.q_in[t] : P.q_out[t - n_in];
+    Stack stk = make_stack(s_code, s_dist, spill);
+    while (true) {
+        {
+            float4 o = P.o[s], d = P.d[s];
+            float best = kMax;
+            int hi = -1, ho = -1;
+            stk.sp = stk.lo = 0;
+            trace_ray<false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk);
+            P.hit[s] = make_int2(hi, ho);
+        }
+        bool push = false;
+        ShadowRay sh;
+        const bool ended = shade_slot(S, F, P, s, push, sh);
+        if (push) {
+            float best = sh.o.w;
+            int hi = -1, ho = -1;
+            stk.sp = stk.lo = 0;
+            if (!trace_ray<true>(S, v3(sh.o), v3(sh.d), best, hi, ho, sh.i.y, sh.i.z, stk)) {
+                float4 L = P.L[s];
+                L.x += sh.c.x;
+                L.y += sh.c.y;
+                L.z += sh.c.z;
+                L.w += sh.c.w;
+                P.L[s] = L;
+            }
+        }
+        if (!ended) continue;
+        if (retire_sample(S, F, P, s)) continue;
+        if (F.mode == LAMBDA_RNG_REFERENCE) break;
+        const int w = atomicAdd(&P.ctr[kCtrWork], 1);
+        if (w >= F.total_work) break;
+        start_work(S, F, P, s, w);
+    }
 }
 
 // Mean XYZ over samples to linear sRGB (core/utils.h:230), in double.
@@ -560,16 +666,18 @@ __global__ void k_fill_i(int *p, int n) {
 
 // ------------------------------------------------------------ query kernels
 
-__global__ void __launch_bounds__(kBlock) k_query_closest(DeviceScene S, const float *rays, int n, lambda_hit *out) {
-    __shared__ int stk[2 * kStack * kBlock];
+__global__ void __launch_bounds__(kBlock) k_query_closest(DeviceScene S, const float *rays, int n, lambda_hit *out,
+                                                          int2 *spill) {
+    __shared__ int s_code[kRing * kBlock];
+    __shared__ float s_dist[kRing * kBlock];
+    Stack stk = make_stack(s_code, s_dist, spill);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     V3 o = v3(rays[6 * i + 0], rays[6 * i + 1], rays[6 * i + 2]);
     V3 d = v3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
     float best = kMax;
     int hi = -1, ho = -1;
-    TraceStats st{};
-    bool ok = traverse<false, false>(S, o, d, best, hi, ho, -1, -2, stk + threadIdx.x, kBlock, st);
+    bool ok = trace_ray<false>(S, o, d, best, hi, ho, -1, -2, stk);
     lambda_hit r;
     r.hit = 0;
     r.instance = -1;
@@ -597,8 +705,10 @@ __global__ void __launch_bounds__(kBlock) k_query_closest(DeviceScene S, const f
 }
 
 __global__ void __launch_bounds__(kBlock) k_query_occluded(DeviceScene S, const float *rays, const int *li,
-                                                           const int *lo, int n, int *out) {
-    __shared__ int stk[2 * kStack * kBlock];
+                                                           const int *lo, int n, int *out, int2 *spill) {
+    __shared__ int s_code[kRing * kBlock];
+    __shared__ float s_dist[kRing * kBlock];
+    Stack stk = make_stack(s_code, s_dist, spill);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     V3 o = v3(rays[6 * i + 0], rays[6 * i + 1], rays[6 * i + 2]);
@@ -614,8 +724,7 @@ __global__ void __launch_bounds__(kBlock) k_query_occluded(DeviceScene S, const 
         best = lh.t;
     }
     int hi = -1, ho = -1;
-    TraceStats st{};
-    out[i] = traverse<true, false>(S, o, d, best, hi, ho, inst, obj, stk + threadIdx.x, kBlock, st) ? 1 : 0;
+    out[i] = trace_ray<true>(S, o, d, best, hi, ho, inst, obj, stk) ? 1 : 0;
 }
 
 __global__ void k_query_thinfilm(DeviceScene S, int mat, const float *q, const float *pts, int n, float *out) {
@@ -668,15 +777,7 @@ int tree_stack_depth(const lambda_scene_view &v, int root, int leaf_limit, bool 
     return deepest;
 }
 
-struct HostSceneCopy {
-    std::vector<float4> node_lo, node_hi, obj_a, obj_b, obj_c, obj_n, obj_s, inst;
-    std::vector<int> obj_mat;
-    std::vector<int4> inst_i, mat, xtex;
-    std::vector<float4> stex;
-    std::vector<float2> cspec;
-    std::vector<int2> light_io;
-    std::vector<float> cie;
-};
+
 
 void validate(const lambda_scene_view &v) {
     require(v.num_nodes > 0 && v.num_objects > 0 && v.num_instances > 0, "empty scene");
@@ -757,8 +858,9 @@ void validate(const lambda_scene_view &v) {
 }  // namespace
 
 struct lambda_hip_scene {
-    DevBuf<float4> node_lo, node_hi, obj_a, obj_b, obj_c, obj_n, obj_s, inst, stex;
-    DevBuf<int> obj_mat, props, perm, tlas_rank, obj_rank;
+    DevBuf<float4> wide, obj, obj_s, inst, stex;
+    DevBuf<int> inst_root, obj_mat, props, perm, tlas_rank, obj_rank;
+    int stack_need = 0;     // deepest traversal stack of the scene
     DevBuf<float4> worley_points;
     DevBuf<int4> worley_box, worley_dim;
     DevBuf<int4> inst_i, mat, xtex;
@@ -774,50 +876,15 @@ namespace {
 
 void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g) {
     validate(v);
-    const size_t N = size_t(v.num_nodes), O = size_t(v.num_objects), I = size_t(v.num_instances);
-    std::vector<float4> lo(N), hi(N);
-    for (size_t i = 0; i < N; i++) {
-        const float *b = v.node_bounds + 6 * i;
-        int idx = v.node_ints[2 * i], cnt = v.node_ints[2 * i + 1];
-        float fi, fc;
-        std::memcpy(&fi, &idx, 4);
-        std::memcpy(&fc, &cnt, 4);
-        lo[i] = make_float4(b[0], b[1], b[2], fi);
-        hi[i] = make_float4(b[3], b[4], b[5], fc);
-    }
-    std::vector<float4> oa(O), ob(O), oc(O), on(O), os(4 * O);
+    const size_t O = size_t(v.num_objects), I = size_t(v.num_instances);
+    WideBuilder wb(v);
+    g.stack_need = wb.run();
+    std::vector<float4> ob4 = pack_objects(v), os = pack_shading(v);
     std::vector<int> om(O);
-    for (size_t i = 0; i < O; i++) {
-        const float *g0 = v.object_geometry + 12 * i;
-        const float *sh = v.object_shading + 15 * i;
-        int type = v.object_type[i];
-        float ft;
-        std::memcpy(&ft, &type, 4);
-        if (type == LAMBDA_OBJECT_SPHERE) {
-            oa[i] = make_float4(g0[0], g0[1], g0[2], g0[3]);
-            ob[i] = make_float4(0, 0, 0, ft);
-            oc[i] = make_float4(0, 0, 0, 0);
-            on[i] = make_float4(0, 0, 0, 0);
-        } else {
-            oa[i] = make_float4(g0[0], g0[1], g0[2], 0);
-            ob[i] = make_float4(g0[3], g0[4], g0[5], ft);
-            oc[i] = make_float4(g0[6], g0[7], g0[8], 0);
-            on[i] = make_float4(g0[9], g0[10], g0[11], 0);
-        }
-        os[4 * i + 0] = make_float4(sh[0], sh[1], sh[2], sh[3]);
-        os[4 * i + 1] = make_float4(sh[4], sh[5], sh[6], sh[7]);
-        os[4 * i + 2] = make_float4(sh[8], sh[9], sh[10], sh[11]);
-        os[4 * i + 3] = make_float4(sh[12], sh[13], sh[14], 0);
-        om[i] = v.object_material[i];
-    }
-    std::vector<float4> im(8 * I);
-    std::vector<int4> ii(I);
-    for (size_t i = 0; i < I; i++) {
-        const float *m = v.instance_matrices + 32 * i;
-        for (int r = 0; r < 8; r++) im[8 * i + r] = make_float4(m[4 * r], m[4 * r + 1], m[4 * r + 2], m[4 * r + 3]);
-        const int *q = v.instance_ints + 4 * i;
-        ii[i] = make_int4(q[0], q[1], q[2], q[3]);
-    }
+    for (size_t i = 0; i < O; i++) om[i] = v.object_material[i];
+    std::vector<float4> im;
+    std::vector<int4> ii;
+    pack_instances(v, im, ii);
     std::vector<int4> mt(size_t(v.num_materials));
     g.material_type.resize(size_t(v.num_materials));
     for (int i = 0; i < v.num_materials; i++) {
@@ -851,12 +918,9 @@ void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g) {
     for (int i = 0; i < kBins; i++)
         for (int k = 0; k < 3; k++) cie[3 * i + k] = float(kCie1931Xyz[i][k]);
 
-    g.node_lo.upload(lo);
-    g.node_hi.upload(hi);
-    g.obj_a.upload(oa);
-    g.obj_b.upload(ob);
-    g.obj_c.upload(oc);
-    g.obj_n.upload(on);
+    g.wide.upload(wb.nodes);
+    g.inst_root.upload(wb.inst_root);
+    g.obj.upload(ob4);
     g.obj_s.upload(os);
     g.obj_mat.upload(om);
     g.inst.upload(im);
@@ -872,36 +936,8 @@ void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g) {
     g.cie.upload(cie);
     g.perm.upload(kPerlinPermutation, 256);
     {
-        // Replay the reference's depth-first order (left pushed before right,
-        // right popped first; scene/scene.h:112-119) without culling and number
-        // the leaves in the order it meets them.  Traversal breaks exact
-        // distance ties by these ranks.
-        std::vector<int> tr(I, 0x7fffffff), orank(O, 0x7fffffff);
-        auto replay = [&](int root, auto &&on_leaf) {
-            std::vector<int> st{root};
-            while (!st.empty()) {
-                int n = st.back();
-                st.pop_back();
-                int idx = v.node_ints[2 * n], cnt = v.node_ints[2 * n + 1];
-                if (cnt > 0) on_leaf(idx, cnt);
-                else {
-                    st.push_back(n + 1);
-                    st.push_back(idx);
-                }
-            }
-        };
-        int r = 0;
-        replay(0, [&](int idx, int) { tr[size_t(idx)] = r++; });
-        std::vector<char> seen(size_t(v.num_nodes), 0);
-        for (size_t i = 0; i < I; i++) {
-            int root = v.instance_ints[4 * i + 2];
-            if (seen[size_t(root)]) continue;
-            seen[size_t(root)] = 1;
-            int k = 0;
-            replay(root, [&](int idx, int cnt) {
-                for (int j = 0; j < cnt; j++) orank[size_t(idx + j)] = k++;
-            });
-        }
+        std::vector<int> tr, orank;
+        reference_ranks(v, tr, orank);
         g.tlas_rank.upload(tr);
         g.obj_rank.upload(orank);
     }
@@ -946,12 +982,9 @@ void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g) {
     }
 
     DeviceScene &d = g.ds;
-    d.node_lo = g.node_lo.p;
-    d.node_hi = g.node_hi.p;
-    d.obj_a = g.obj_a.p;
-    d.obj_b = g.obj_b.p;
-    d.obj_c = g.obj_c.p;
-    d.obj_n = g.obj_n.p;
+    d.wide = g.wide.p;
+    d.inst_root = g.inst_root.p;
+    d.obj = g.obj.p;
     d.obj_s = g.obj_s.p;
     d.obj_mat = g.obj_mat.p;
     d.inst = g.inst.p;
@@ -1009,6 +1042,7 @@ struct RenderBuffers {
     DevBuf<int> qa, qb, done, ctr, order;
     DevBuf<double> fb;
     DevBuf<float> rgb;
+    DevBuf<int2> spill;
 };
 
 void validate_params(const lambda_render_params &p) {
@@ -1026,6 +1060,14 @@ void validate_params(const lambda_render_params &p) {
 }
 
 int grid_for(long long n) { return int(std::max(1LL, (n + kBlock - 1) / kBlock)); }
+
+// Global stack area for launches of up to `items` threads: entries beyond
+// the LDS ring, [entry][thread]; none when the scene never needs them.
+void alloc_spill(const lambda_hip_scene &g, long long items, DevBuf<int2> &spill) {
+    if (g.stack_need <= kRing) return;
+    const size_t threads = size_t(grid_for(items)) * kBlock;
+    spill.alloc(size_t(g.stack_need - kRing) * threads);
+}
 
 struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
@@ -1080,6 +1122,7 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
     B.fb.alloc(size_t(F.npix) * 3);
     B.rgb.alloc(size_t(F.npix) * 3);
     B.order.upload(tile_order(p.width, p.height));
+    alloc_spill(g, 2LL * ns, B.spill);
 
     Paths P;
     P.o = B.o.p; P.d = B.d.p; P.lam = B.lam.p; P.beta = B.beta.p; P.L = B.L.p;
@@ -1123,10 +1166,38 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
     const long long per_slot = F.mode == LAMBDA_RNG_REFERENCE ? F.spp : (F.total_work + ns - 1) / ns + 1;
     const long long max_iters = (long long)(F.depth + 2) * per_slot + 16;
     long long iters = 0;
+    // Below this many live paths the rest of the render runs in one
+    // persistent launch (per-thread chains instead of one pass per bounce).
+    const long long tail = p.tail_paths < 0 ? 0 : p.tail_paths > 0 ? p.tail_paths : kAutoTailPaths;
     while (n_closest > 0 || n_shadow > 0 || n_done > 0) {
         if (++iters > max_iters) throw Failure{LAMBDA_EHIP, "bounce loop did not drain"};
+        if ((long long)n_closest + n_done < tail && iters > 1) {
+            // settle pending occlusion rays and ended samples, then run every
+            // live slot to its end
+            if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
+            k_trace<<<grid_for(n_shadow), kBlock, 0, st>>>(g.ds, P, 0, n_shadow, B.spill.p);
+            if (stats) HIP_OK(hipEventRecord(ev[0].b, st));
+            k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done);
+            if (stats) HIP_OK(hipEventRecord(ev[3].a, st));
+            k_persist<<<grid_for((long long)n_closest + n_done), kBlock, 0, st>>>(g.ds, F, P, n_closest, B.spill.p);
+            if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(host_ctr, P.ctr, kCtrCount * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            if (host_ctr[kCtrError]) throw Failure{LAMBDA_EHIP, "device reported an inconsistent hit record"};
+            if (stats) {
+                stats->shadow_rays += n_shadow;
+                for (int k : {0, 3}) {
+                    float ms = 0;
+                    HIP_OK(hipEventElapsedTime(&ms, ev[k].a, ev[k].b));
+                    stats->kernel_ms[k] += ms;
+                    stats->launches[k]++;
+                }
+            }
+            break;
+        }
         if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
-        k_trace<<<grid_for((long long)n_closest + n_shadow), kBlock, 0, st>>>(g.ds, P, n_closest, n_shadow);
+        k_trace<<<grid_for((long long)n_closest + n_shadow), kBlock, 0, st>>>(g.ds, P, n_closest, n_shadow, B.spill.p);
         if (stats) {
             HIP_OK(hipEventRecord(ev[0].b, st));
             HIP_OK(hipEventRecord(ev[2].a, st));
@@ -1251,7 +1322,9 @@ int lambda_hip_closest_hit(lambda_hip_scene *scene, const float *rays, int32_t n
         r.upload(rays, size_t(n) * 6);
         DevBuf<lambda_hit> h;
         h.alloc(size_t(n));
-        k_query_closest<<<grid_for(n), kBlock>>>(scene->ds, r.p, n, h.p);
+        DevBuf<int2> spill;
+        alloc_spill(*scene, n, spill);
+        k_query_closest<<<grid_for(n), kBlock>>>(scene->ds, r.p, n, h.p, spill.p);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpy(hits, h.p, size_t(n) * sizeof(lambda_hit), hipMemcpyDeviceToHost));
     });
@@ -1278,7 +1351,9 @@ int lambda_hip_occluded(lambda_hip_scene *scene, const float *rays, const int32_
         li.upload(light_instance, size_t(n));
         lo.upload(light_object, size_t(n));
         out.alloc(size_t(n));
-        k_query_occluded<<<grid_for(n), kBlock>>>(scene->ds, r.p, li.p, lo.p, n, out.p);
+        DevBuf<int2> spill;
+        alloc_spill(*scene, n, spill);
+        k_query_occluded<<<grid_for(n), kBlock>>>(scene->ds, r.p, li.p, lo.p, n, out.p, spill.p);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpy(occluded, out.p, size_t(n) * sizeof(int), hipMemcpyDeviceToHost));
     });

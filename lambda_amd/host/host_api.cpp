@@ -239,6 +239,7 @@ int lambda_scene_get_params(const lambda_scene *scene, lambda_render_params *p) 
     p->seed = 42;
     p->rng_mode = LAMBDA_RNG_REFERENCE;
     p->path_slots = 0;
+    p->tail_paths = 0;
     return LAMBDA_OK;
 }
 
