@@ -10,14 +10,16 @@ materials, 4 quad emitters and a constant environment; 1920x1080, depth 10,
 region of each step ends when the RGB image is back in host memory.
 
 Printed JSON line (rank 0, one GPU):
-  value        paths/s over the K timed renders
-  roofline     the trace kernel (closest-hit and occlusion traversal):
-               algorithmic bytes per launch (oracle-counted node, instance and
-               primitive tests per ray of each kind on a strided pixel sample,
-               times the rays of each kind per launch) over its mean launch
-               time from HIP events, against 8 TB/s
+  value        paths/s over the K timed renders (reference PCG32 streams)
+  roofline     the kernel that traces the frame's rays (k_persist in the
+               default persistent schedule, k_trace with one launch per
+               bounce): algorithmic bytes per launch (oracle-counted node,
+               instance and primitive tests per ray of each kind on a strided
+               pixel sample, times the rays of each kind the launch traced)
+               over its mean launch time from HIP events, against 8 TB/s
   cpu_baseline the CPU oracle (bit-identical restatement of the reference
-               CPU path) on a band of rows at reduced spp, all host threads
+               CPU path) on a band of rows, all host threads
+  throughput_streams  the same frame with independent streams per sample
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -59,34 +61,33 @@ def make_workload(name: str, scene_dir: str):
 
 
 def cpu_baseline(scene, seconds: float, threads: int) -> dict:
-    """Times the CPU oracle on a band of rows, calibrated to roughly `seconds`
-    of work: the band grows to the full frame at 4 spp, then the spp grows
-    (16, 64); paths/s is independent of spp to first order."""
+    """Times the CPU oracle on the full frame at the largest square spp (up
+    to the workload's) whose estimated time fits `seconds`, from a 1-spp
+    calibration render; below 1 spp it renders a band of rows.  paths/s of
+    this integrator does not depend on spp to first order."""
     import oracle
-    p = scene.render_params()
-    mid = p.height // 2
-    rows, root = 2, 2
-    while True:
-        spp = root * root
-        q = scene.render_params(samples=spp, sqrt_samples=root)
-        r0 = max(0, mid - rows // 2)
-        r1 = min(p.height, r0 + rows)
-        t0 = time.perf_counter()
-        oracle.render(scene, q, threads=threads, rows=(r0, r1))
-        t = time.perf_counter() - t0
-        if t >= seconds / 4:
-            break
-        if r1 - r0 < p.height:
-            rows = min(p.height, max(rows * 2, int(rows * seconds / max(t, 1e-3) / 2)))
-        elif root < 8 and root * root < p.samples:
-            root *= 2
-        else:
-            break
-    paths = (r1 - r0) * p.width * spp
+    p = scene.params
+    q = scene.render_params(samples=1, sqrt_samples=1)
+    t0 = time.perf_counter()
+    oracle.render(scene, q, threads=threads)
+    t1 = time.perf_counter() - t0
+    root = 1
+    while (root + 1) ** 2 <= p.samples and t1 * (root + 1) ** 2 <= seconds:
+        root += 1
+    rows = p.height
+    if t1 > seconds:
+        rows = max(1, int(p.height * seconds / t1))
+    spp = root * root
+    q = scene.render_params(samples=spp, sqrt_samples=root)
+    r0 = (p.height - rows) // 2
+    t0 = time.perf_counter()
+    oracle.render(scene, q, threads=threads, rows=(r0, r0 + rows))
+    t = time.perf_counter() - t0
+    paths = rows * p.width * spp
     return {"value": paths / t, "unit": "paths/s", "cores": threads, "kind": "port",
             "sample": "rows %d-%d of %d (%d pixels) at %d spp, %.1f s; CPU oracle, bit-identical restatement of "
                       "the reference CPU integrator, OpenMP guided schedule over rows" % (
-                          r0, r1, p.height, (r1 - r0) * p.width, spp, t)}
+                          r0, r0 + rows, p.height, rows * p.width, spp, t)}
 
 
 def copy_probe(nbytes: int) -> None:
@@ -118,11 +119,14 @@ def main() -> int:
     ap.add_argument("--workload", default="instanced_mesh", choices=["instanced_mesh", "closed_box", "thinfilm_grid"])
     ap.add_argument("--rng", default="reference", choices=["reference", "throughput"])
     ap.add_argument("--path-slots", type=int, default=0)
+    ap.add_argument("--cost-order", type=int, default=0, help="start pixels longest first (0 automatic, 1 on, -1 off)")
     ap.add_argument("--tail-paths", type=int, default=0,
                     help="live paths below which the render finishes in one persistent launch (0 automatic, -1 never)")
     ap.add_argument("--scene-dir", default=os.environ.get("LAMBDA_SCENE_DIR", "/tmp/lambda_bench"))
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--throughput-steps", type=int, default=2,
+                    help="also time this many renders with throughput streams (0 to skip)")
     ap.add_argument("--stats-pixels", type=int, default=20000, help="pixels sampled to count traversal work")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (a plain copy kernel of known traffic, "
@@ -166,20 +170,33 @@ def main() -> int:
 
     for i in range(args.warmup):
         t0 = time.perf_counter()
-        dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths)
+        dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order)
         log("warmup %d: %.3f s" % (i, time.perf_counter() - t0))
 
     t0 = time.perf_counter()
     for i in range(args.steps):
-        img = dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths)
+        img = dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order)
     elapsed = time.perf_counter() - t0
     value = paths_per_step * args.steps / elapsed
     log("timed %d steps: %.3f s -> %.4g paths/s" % (args.steps, elapsed, value))
     finite = bool(np.all(np.isfinite(img)))
 
     # Kernel timing from HIP events on the render stream.
-    _, st = dev.render_stats(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths)
+    _, st = dev.render_stats(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order)
     log("kernel ms: " + ", ".join("%s %.1f" % (k, st[k + "_ms"]) for k in ("trace", "shade", "finalize", "persist")))
+
+    # The same frame with independent streams per (pixel, sample): same
+    # estimator, different random numbers, no per-pixel sample chains.
+    thr = None
+    if mode == lambda_amd.LAMBDA_RNG_REFERENCE and args.throughput_steps > 0:
+        dev.render(rng_mode=lambda_amd.LAMBDA_RNG_THROUGHPUT)
+        t0 = time.perf_counter()
+        for _ in range(args.throughput_steps):
+            dev.render(rng_mode=lambda_amd.LAMBDA_RNG_THROUGHPUT)
+        t_thr = (time.perf_counter() - t0) / args.throughput_steps
+        thr = {"value": paths_per_step / t_thr, "unit": "paths/s", "ms_per_step": t_thr * 1e3,
+               "steps": args.throughput_steps}
+        log("throughput streams: %.4g paths/s" % thr["value"])
 
     # Algorithmic bytes per closest-hit ray, counted by the oracle on a
     # strided pixel sample (all samples of each sampled pixel).
@@ -190,17 +207,20 @@ def main() -> int:
     t_stats = time.perf_counter() - t0
     b_closest = oracle.ray_bytes(ps, "closest")
     b_shadow = oracle.ray_bytes(ps, "shadow")
-    # k_trace handles both ray kinds in one launch
-    launches = max(1, st["trace_launches"])
+    # The dominant kernel traces every ray of the frame: k_persist (the
+    # persistent schedule, one launch) or k_trace (one launch per bounce);
+    # each handles both ray kinds.
+    kern = "persist" if st["persist_ms"] >= st["trace_ms"] else "trace"
+    launches = max(1, st[kern + "_launches"])
     bytes_total = b_closest * st["closest_rays"] + b_shadow * st["shadow_rays"]
     bytes_per_launch = bytes_total / launches
-    avg_launch_s = st["trace_ms"] / 1e3 / launches
+    avg_launch_s = st[kern + "_ms"] / 1e3 / launches
     achieved = bytes_per_launch / avg_launch_s / 1e9
     log("traversal work from %d sampled pixels (%.1f s): %.0f B per closest ray, %.0f B per shadow ray" % (
         ps["paths"] // max(1, p.samples), t_stats, b_closest, b_shadow))
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "trace_traffic_%s.json" % cfg["workload"])
+    tpath = os.path.join(ROOT, "profiles", "%s_traffic_%s.json" % (kern, cfg["workload"]))
     if os.path.exists(tpath):
         with open(tpath) as fh:
             tj = json.load(fh)
@@ -230,12 +250,13 @@ def main() -> int:
         "data": "synthetic (generated scene, seeded)",
         "config": cfg,
         "roofline": {
-            "bound": "hbm", "kernel": "k_trace", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "bound": "hbm", "kernel": "k_" + kern, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "bytes_per_closest_ray": b_closest, "bytes_per_shadow_ray": b_shadow, "launches": launches,
             "avg_launch_ms": avg_launch_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,
+        "throughput_streams": thr,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         "kernels_ms": {k: st[k + "_ms"] for k in ("trace", "shade", "finalize", "persist")},
         "rays": {"closest": st["closest_rays"], "shadow": st["shadow_rays"], "iterations": st["iterations"]},

@@ -184,6 +184,9 @@ typedef struct lambda_render_params {
     int32_t tail_paths;    /* once fewer paths than this are live, one persistent
                               launch runs each remaining path to its end instead
                               of one launch per bounce; 0 = automatic, -1 = never */
+    int32_t cost_order;    /* start pixels in decreasing order of a one-sample
+                              cost estimate (reference streams; the image does
+                              not change): 0 = automatic, 1 = on, -1 = off */
 } lambda_render_params;
 
 /*

@@ -137,6 +137,7 @@ struct Paths {
     float4 *s_o, *s_d, *s_c;  // shadow: origin+tmax, direction, contribution
     int4 *s_i;                // shadow: slot, light instance, light object
     int *ctr;
+    unsigned long long *rays; // rays traced by the persistent kernel: closest, occlusion
     double *fb;               // XYZ sums, [pixel][3]
     const int *order;         // pixel visiting order (8x8 tiles)
 };
@@ -146,6 +147,7 @@ struct Frame {
     float lmin, lmax, lrange;
     uint64_t seed;
     long long total_work;     // npix * spp (throughput mode)
+    int *cost;                // when set: per-pixel sum of path segments
 };
 
 // New camera sample `sample` of `pixel` into slot s (render/renderer.h:71-82,
@@ -189,8 +191,13 @@ __device__ __forceinline__ Pcg load_rng(const Paths &P, int s) {
 // global area of the launch, [entry][thread] (allocated by the host when the
 // scene's worst-case depth exceeds kRing).
 constexpr int kRing = 16;
-// Live paths below which a render switches to the persistent schedule.
-constexpr long long kAutoTailPaths = 65536;
+// Waves per SIMD the persistent kernel is compiled for (its registers are
+// capped so that this many fit; measured best on gfx950).
+constexpr int kPersistWaves = 3;
+// Automatic schedule: live paths below which a render switches to the
+// persistent schedule (above any slot count: persistent from the first pass
+// on).  Slots default to the number of persistent threads resident at once.
+constexpr long long kAutoTailPaths = 1LL << 40;
 using Stack = TraversalStack<kRing>;
 
 __device__ __forceinline__ Stack make_stack(int *s_code, float *s_dist, int2 *spill) {
@@ -210,18 +217,42 @@ __device__ __forceinline__ bool trace_ray(const DeviceScene &S, V3 o, V3 d, floa
     return lam::traverse<kAny, kRing>(S, o, d, best, hit_inst, hit_obj, skip_inst, skip_obj, stk);
 }
 
-// First sample of every slot.  Reference mode: slot s owns the pixel of
-// rank s for all its samples.  Throughput mode: slot s takes work item s.
+// Reference mode: the pixel of rank `rank` in the visiting order starts its
+// first sample in slot s with PCG32 stream (seed, pixel index)
+// (render/renderer.h:47-49); the slot keeps the pixel for all its samples.
+__device__ __forceinline__ void start_pixel(const DeviceScene &S, const Frame &F, const Paths &P, int s, int rank) {
+    const int pixel = P.order[rank];
+    Pcg rng = pcg_seed(F.seed, uint64_t(pixel));
+    camera_sample(S, F, P, s, pixel, 0, rng);
+    store_rng(P, s, rng);
+}
+
+// Throughput mode: work item w = sample * npix + rank starts in slot s with
+// its own PCG32 stream (seed, w).
+__device__ __forceinline__ void start_work(const DeviceScene &S, const Frame &F, const Paths &P, int s, long long w) {
+    int rank = int(w % F.npix);
+    int sample = int(w / F.npix);
+    int pixel = P.order[rank];
+    Pcg rng = pcg_seed(F.seed, uint64_t(w));
+    camera_sample(S, F, P, s, pixel, sample, rng);
+    store_rng(P, s, rng);
+}
+
+// Work units: pixels (reference mode) or (pixel, sample) items.
+__device__ __forceinline__ long long work_units(const Frame &F) {
+    return F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
+}
+
+__device__ __forceinline__ void start_unit(const DeviceScene &S, const Frame &F, const Paths &P, int s, long long w) {
+    if (F.mode == LAMBDA_RNG_REFERENCE) start_pixel(S, F, P, s, int(w));
+    else start_work(S, F, P, s, w);
+}
+
+// First work unit of every slot: slot s takes unit s.
 __global__ void __launch_bounds__(kBlock) k_start(DeviceScene S, Frame F, Paths P, int nslots) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
-    long long w = s;
-    int rank = F.mode == LAMBDA_RNG_REFERENCE ? s : int(w % F.npix);
-    int sample = F.mode == LAMBDA_RNG_REFERENCE ? 0 : int(w / F.npix);
-    int pixel = P.order[rank];
-    Pcg rng = pcg_seed(F.seed, F.mode == LAMBDA_RNG_REFERENCE ? uint64_t(pixel) : uint64_t(w));
-    camera_sample(S, F, P, s, pixel, sample, rng);
-    store_rng(P, s, rng);
+    start_unit(S, F, P, s, s);
     P.q_in[s] = s;
 }
 
@@ -538,6 +569,7 @@ __device__ __forceinline__ void sample_xyz(const DeviceScene &S, const Frame &F,
 // pixel's PCG32 stream (render/renderer.h:69-91); returns true when it did.
 __device__ __forceinline__ bool retire_sample(const DeviceScene &S, const Frame &F, const Paths &P, int s) {
     int4 info = P.info[s];
+    if (F.cost) atomicAdd(F.cost + info.x, info.z);
     float x, y, z;
     sample_xyz(S, F, P.L[s], P.lam[s], x, y, z);
     double *px = P.fb + size_t(info.x) * 3;
@@ -552,17 +584,6 @@ __device__ __forceinline__ bool retire_sample(const DeviceScene &S, const Frame 
     }
     atomicAdd(&P.ctr[kCtrPixels], 1);
     return false;
-}
-
-// Throughput mode: work item w = sample * npix + rank starts in slot s with
-// its own PCG32 stream (seed, w).
-__device__ __forceinline__ void start_work(const DeviceScene &S, const Frame &F, const Paths &P, int s, long long w) {
-    int rank = int(w % F.npix);
-    int sample = int(w / F.npix);
-    int pixel = P.order[rank];
-    Pcg rng = pcg_seed(F.seed, uint64_t(w));
-    camera_sample(S, F, P, s, pixel, sample, rng);
-    store_rng(P, s, rng);
 }
 
 // Retires the samples that ended in the previous shade pass (their
@@ -580,28 +601,27 @@ __global__ void __launch_bounds__(kBlock) k_finalize(DeviceScene S, Frame F, Pat
     if (q < n_done) {
         s = P.done[q];
         keep = retire_sample(S, F, P, s);
-        need_work = F.mode != LAMBDA_RNG_REFERENCE;
+        need_work = !keep;
     }
-    if (F.mode != LAMBDA_RNG_REFERENCE) {
-        int w = wave_push(&P.ctr[kCtrWork], need_work);
-        if (need_work && w < F.total_work) {
-            start_work(S, F, P, s, w);
-            keep = true;
-        }
+    // a slot whose unit is finished takes the next one
+    int w = wave_push(&P.ctr[kCtrWork], need_work);
+    if (need_work && w < work_units(F)) {
+        start_unit(S, F, P, s, w);
+        keep = true;
     }
     int pos = wave_push(&P.ctr[kCtrNext], keep);
     if (keep) P.q_out[pos] = s;
 }
 
-// Persistent per-thread schedule for the last paths of a render: each
-// thread takes one live slot and runs it to the end -- closest hit, shading,
-// the occlusion ray at once, retirement, the slot's next sample -- without
-// waiting for other paths.  Threads [0, n_in) take q_in (paths that need
-// their closest hit); the next P.ctr[kCtrNext] threads take q_out (samples
-// just started by k_finalize).  No occlusion ray or ended sample may be
-// pending when it starts.  In throughput mode threads keep taking work items
-// until none are left.
-__global__ void __launch_bounds__(kBlock) k_persist(DeviceScene S, Frame F, Paths P, int n_in, int2 *spill) {
+// Persistent per-thread schedule: each thread takes one live slot and runs
+// it -- closest hit, shading, the occlusion ray at once, retirement, the
+// slot's next sample, then further work units -- without waiting for other
+// paths.  Threads [0, n_in) take q_in (paths that need their closest hit);
+// the next P.ctr[kCtrNext] threads take q_out (samples just started by
+// k_finalize).  No occlusion ray or ended sample may be pending when it
+// starts.  Threads exit when no work unit is left.
+__global__ void __launch_bounds__(kBlock, kPersistWaves) k_persist(DeviceScene S, Frame F, Paths P, int n_in,
+                                                                 int2 *spill) {
     __shared__ int s_code[kRing * kBlock];
     __shared__ float s_dist[kRing * kBlock];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -609,7 +629,9 @@ __global__ void __launch_bounds__(kBlock) k_persist(DeviceScene S, Frame F, Path
     if (t >= n_in + n_new) return;
     const int s = t < n_in ? P.q_in[t] : P.q_out[t - n_in];
     Stack stk = make_stack(s_code, s_dist, spill);
+    unsigned n_closest = 0, n_shadow = 0;
     while (true) {
+        n_closest++;
         {
             float4 o = P.o[s], d = P.d[s];
             float best = kMax;
@@ -622,6 +644,7 @@ __global__ void __launch_bounds__(kBlock) k_persist(DeviceScene S, Frame F, Path
         ShadowRay sh;
         const bool ended = shade_slot(S, F, P, s, push, sh);
         if (push) {
+            n_shadow++;
             float best = sh.o.w;
             int hi = -1, ho = -1;
             stk.sp = stk.lo = 0;
@@ -636,11 +659,12 @@ __global__ void __launch_bounds__(kBlock) k_persist(DeviceScene S, Frame F, Path
         }
         if (!ended) continue;
         if (retire_sample(S, F, P, s)) continue;
-        if (F.mode == LAMBDA_RNG_REFERENCE) break;
         const int w = atomicAdd(&P.ctr[kCtrWork], 1);
-        if (w >= F.total_work) break;
-        start_work(S, F, P, s, w);
+        if (w >= work_units(F)) break;
+        start_unit(S, F, P, s, w);
     }
+    atomicAdd(&P.rays[0], (unsigned long long)n_closest);
+    atomicAdd(&P.rays[1], (unsigned long long)n_shadow);
 }
 
 // Mean XYZ over samples to linear sRGB (core/utils.h:230), in double.
@@ -654,6 +678,8 @@ __global__ void k_resolve(Frame F, const double *fb, float *rgb) {
     rgb[3 * size_t(p) + 1] = float(-x * double(float(0.9689307)) + y * double(float(1.8757561)) + z * double(float(0.0415175)));
     rgb[3 * size_t(p) + 2] = float(x * double(float(0.0557101)) - y * double(float(0.2040211)) + z * double(float(1.0569959)));
 }
+
+__global__ void k_set_i(int *p, int v) { *p = v; }
 
 __global__ void k_fill_d(double *p, size_t n) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1040,6 +1066,7 @@ struct RenderBuffers {
     DevBuf<int4> info, s_i;
     DevBuf<int2> hit;
     DevBuf<int> qa, qb, done, ctr, order;
+    DevBuf<unsigned long long> rays;
     DevBuf<double> fb;
     DevBuf<float> rgb;
     DevBuf<int2> spill;
@@ -1054,8 +1081,10 @@ void validate_params(const lambda_render_params &p) {
     if (p.rng_mode != LAMBDA_RNG_REFERENCE && p.rng_mode != LAMBDA_RNG_THROUGHPUT)
         throw Failure{LAMBDA_EINVAL, "invalid rng_mode"};
     if ((long long)p.width * p.height > (1LL << 28)) throw Failure{LAMBDA_EINVAL, "image too large"};
-    if (p.rng_mode == LAMBDA_RNG_THROUGHPUT && (long long)p.width * p.height * p.samples >= (1LL << 31))
-        throw Failure{LAMBDA_EINVAL, "throughput mode supports fewer than 2^31 samples per render"};
+    // work-unit counters are 32-bit and may run past the last unit by the
+    // number of slots (at most 2^28)
+    if (p.rng_mode == LAMBDA_RNG_THROUGHPUT && (long long)p.width * p.height * p.samples >= (1LL << 31) - (1LL << 28))
+        throw Failure{LAMBDA_EINVAL, "throughput mode supports fewer than 2^31 - 2^28 samples per render"};
     if ((long long)p.width * p.height * p.samples > (1LL << 40)) throw Failure{LAMBDA_EINVAL, "too many samples"};
 }
 
@@ -1077,17 +1106,17 @@ struct EventPair {
     }
 };
 
-struct StreamGuard {
-    hipStream_t s = nullptr;
-    ~StreamGuard() {
-        if (s) (void)hipStreamDestroy(s);
-    }
-};
 
-void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_rgb, volatile int32_t *progress,
-                 lambda_render_stats *stats = nullptr) {
-    auto wall0 = std::chrono::steady_clock::now();
-    validate_params(p);
+// Threads of k_persist resident on the device at once.
+long long resident_persist_threads() {
+    int dev = 0, cus = 0, blocks = 0;
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_persist, kBlock, 0));
+    return (long long)std::max(1, cus) * std::max(1, blocks) * kBlock;
+}
+
+Frame make_frame(const lambda_render_params &p) {
     Frame F;
     F.width = p.width;
     F.height = p.height;
@@ -1101,143 +1130,208 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
     F.lrange = p.lambda_max - p.lambda_min;
     F.seed = p.seed;
     F.total_work = (long long)F.npix * F.spp;
+    F.cost = nullptr;
+    return F;
+}
 
-    long long slots;
-    if (F.mode == LAMBDA_RNG_REFERENCE) {
-        slots = F.npix;
-    } else {
-        long long want = p.path_slots > 0 ? p.path_slots : (1LL << 21);
-        slots = std::min<long long>(want, F.total_work);
-        slots = std::max<long long>(slots, 1);
-    }
-    if (slots > (1LL << 28)) throw Failure{LAMBDA_EINVAL, "too many path slots"};
-    const int ns = int(slots);
-
+// Device buffers, stream and counters of one render.
+struct Renderer {
+    lambda_hip_scene &g;
     RenderBuffers B;
-    B.o.alloc(ns); B.d.alloc(ns); B.lam.alloc(ns); B.beta.alloc(ns); B.L.alloc(ns);
-    B.s_o.alloc(ns); B.s_d.alloc(ns); B.s_c.alloc(ns); B.s_i.alloc(ns);
-    B.rng.alloc(ns); B.info.alloc(ns); B.hit.alloc(ns);
-    B.qa.alloc(ns); B.qb.alloc(ns); B.done.alloc(ns);
-    B.ctr.alloc(kCtrCount);
-    B.fb.alloc(size_t(F.npix) * 3);
-    B.rgb.alloc(size_t(F.npix) * 3);
-    B.order.upload(tile_order(p.width, p.height));
-    alloc_spill(g, 2LL * ns, B.spill);
-
-    Paths P;
-    P.o = B.o.p; P.d = B.d.p; P.lam = B.lam.p; P.beta = B.beta.p; P.L = B.L.p;
-    P.rng = B.rng.p; P.info = B.info.p; P.hit = B.hit.p;
-    P.q_in = B.qa.p; P.q_out = B.qb.p; P.done = B.done.p;
-    P.s_o = B.s_o.p; P.s_d = B.s_d.p; P.s_c = B.s_c.p; P.s_i = B.s_i.p;
-    P.ctr = B.ctr.p; P.fb = B.fb.p; P.order = B.order.p;
-
+    Paths P{};
+    int ns = 0;
     int *host_ctr = nullptr;
-    HIP_OK(hipHostMalloc((void **)&host_ctr, kCtrCount * sizeof(int), hipHostMallocDefault));
-    struct PinGuard {
-        int *p;
-        ~PinGuard() { if (p) (void)hipHostFree(p); }
-    } pin{host_ctr};
-
-    StreamGuard sg;
-    HIP_OK(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
-    hipStream_t st = sg.s;
+    hipStream_t st = nullptr;
     EventPair ev[4];
-    if (stats) {
-        std::memset(stats, 0, sizeof(*stats));
-        stats->path_slots = ns;
-        for (auto &e : ev) {
-            HIP_OK(hipEventCreate(&e.a));
-            HIP_OK(hipEventCreate(&e.b));
-        }
+
+    Renderer(lambda_hip_scene &scene, const Frame &F, int slots) : g(scene), ns(slots) {
+        B.o.alloc(ns); B.d.alloc(ns); B.lam.alloc(ns); B.beta.alloc(ns); B.L.alloc(ns);
+        B.s_o.alloc(ns); B.s_d.alloc(ns); B.s_c.alloc(ns); B.s_i.alloc(ns);
+        B.rng.alloc(ns); B.info.alloc(ns); B.hit.alloc(ns);
+        B.qa.alloc(ns); B.qb.alloc(ns); B.done.alloc(ns);
+        B.ctr.alloc(kCtrCount);
+        B.rays.alloc(2);
+        B.fb.alloc(size_t(F.npix) * 3);
+        B.rgb.alloc(size_t(F.npix) * 3);
+        B.order.upload(tile_order(F.width, F.height));
+        alloc_spill(g, 2LL * ns, B.spill);
+        P.o = B.o.p; P.d = B.d.p; P.lam = B.lam.p; P.beta = B.beta.p; P.L = B.L.p;
+        P.rng = B.rng.p; P.info = B.info.p; P.hit = B.hit.p;
+        P.q_in = B.qa.p; P.q_out = B.qb.p; P.done = B.done.p;
+        P.s_o = B.s_o.p; P.s_d = B.s_d.p; P.s_c = B.s_c.p; P.s_i = B.s_i.p;
+        P.ctr = B.ctr.p; P.rays = B.rays.p; P.fb = B.fb.p; P.order = B.order.p;
+        HIP_OK(hipHostMalloc((void **)&host_ctr, kCtrCount * sizeof(int), hipHostMallocDefault));
+        HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     }
-    k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
-    k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
-    k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns);
-    HIP_OK(hipGetLastError());
-    // Work counter for throughput mode starts after the slots' first items.
-    if (F.mode != LAMBDA_RNG_REFERENCE) {
-        int start = ns;
-        HIP_OK(hipMemcpyAsync(P.ctr + kCtrWork, &start, sizeof(int), hipMemcpyHostToDevice, st));
+    ~Renderer() {
+        if (host_ctr) (void)hipHostFree(host_ctr);
+        if (st) (void)hipStreamDestroy(st);
     }
-    int n_closest = ns, n_shadow = 0, n_done = 0;
-    const long long total_units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
-    // Every pass advances each live path by one segment; a slot runs at most
-    // depth + 1 segments per sample it owns, plus one pass to retire it.
-    const long long per_slot = F.mode == LAMBDA_RNG_REFERENCE ? F.spp : (F.total_work + ns - 1) / ns + 1;
-    const long long max_iters = (long long)(F.depth + 2) * per_slot + 16;
-    long long iters = 0;
-    // Below this many live paths the rest of the render runs in one
-    // persistent launch (per-thread chains instead of one pass per bounce).
-    const long long tail = p.tail_paths < 0 ? 0 : p.tail_paths > 0 ? p.tail_paths : kAutoTailPaths;
-    while (n_closest > 0 || n_shadow > 0 || n_done > 0) {
-        if (++iters > max_iters) throw Failure{LAMBDA_EHIP, "bounce loop did not drain"};
-        if ((long long)n_closest + n_done < tail && iters > 1) {
-            // settle pending occlusion rays and ended samples, then run every
-            // live slot to its end
-            if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
-            k_trace<<<grid_for(n_shadow), kBlock, 0, st>>>(g.ds, P, 0, n_shadow, B.spill.p);
-            if (stats) HIP_OK(hipEventRecord(ev[0].b, st));
-            k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done);
-            if (stats) HIP_OK(hipEventRecord(ev[3].a, st));
-            k_persist<<<grid_for((long long)n_closest + n_done), kBlock, 0, st>>>(g.ds, F, P, n_closest, B.spill.p);
-            if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
-            HIP_OK(hipGetLastError());
-            HIP_OK(hipMemcpyAsync(host_ctr, P.ctr, kCtrCount * sizeof(int), hipMemcpyDeviceToHost, st));
-            HIP_OK(hipStreamSynchronize(st));
-            if (host_ctr[kCtrError]) throw Failure{LAMBDA_EHIP, "device reported an inconsistent hit record"};
-            if (stats) {
-                stats->shadow_rays += n_shadow;
-                for (int k : {0, 3}) {
-                    float ms = 0;
-                    HIP_OK(hipEventElapsedTime(&ms, ev[k].a, ev[k].b));
-                    stats->kernel_ms[k] += ms;
-                    stats->launches[k]++;
-                }
-            }
-            break;
-        }
-        if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
-        k_trace<<<grid_for((long long)n_closest + n_shadow), kBlock, 0, st>>>(g.ds, P, n_closest, n_shadow, B.spill.p);
-        if (stats) {
-            HIP_OK(hipEventRecord(ev[0].b, st));
-            HIP_OK(hipEventRecord(ev[2].a, st));
-        }
-        k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done);
-        if (stats) {
-            HIP_OK(hipEventRecord(ev[2].b, st));
-            HIP_OK(hipEventRecord(ev[1].a, st));
-        }
-        k_shade<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, n_closest);
-        if (stats) HIP_OK(hipEventRecord(ev[1].b, st));
+
+    void sync_counters() {
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(host_ctr, P.ctr, kCtrCount * sizeof(int), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
         if (host_ctr[kCtrError]) throw Failure{LAMBDA_EHIP, "device reported an inconsistent hit record"};
+    }
+
+    // Renders F into the FP64 framebuffer: one launch per bounce while at
+    // least `tail` paths are live, then one persistent launch.
+    void run(const Frame &F, long long tail, volatile int32_t *progress, lambda_render_stats *stats) {
+        const long long units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
         if (stats) {
-            stats->iterations++;
-            stats->closest_rays += n_closest;
-            stats->shadow_rays += n_shadow;
-            for (int k = 0; k < 3; k++) {
-                float ms = 0;
-                HIP_OK(hipEventElapsedTime(&ms, ev[k].a, ev[k].b));
-                stats->kernel_ms[k] += ms;
-                stats->launches[k]++;
+            for (auto &e : ev) {
+                if (!e.a) HIP_OK(hipEventCreate(&e.a));
+                if (!e.b) HIP_OK(hipEventCreate(&e.b));
             }
         }
-        n_closest = host_ctr[kCtrNext];
-        n_shadow = host_ctr[kCtrShadow];
-        n_done = host_ctr[kCtrDone];
-        std::swap(P.q_in, P.q_out);
-        if (progress) {
-            long long done = host_ctr[kCtrPixels];
-            if (F.mode != LAMBDA_RNG_REFERENCE) done = done * F.npix / std::max<long long>(1, total_units);
-            *progress = int32_t(std::min<long long>(done, F.npix));
+        auto timed = [&](int k) {
+            float ms = 0;
+            HIP_OK(hipEventElapsedTime(&ms, ev[k].a, ev[k].b));
+            stats->kernel_ms[k] += ms;
+            stats->launches[k]++;
+        };
+        k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
+        k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
+        k_fill_d<<<1, kBlock, 0, st>>>(reinterpret_cast<double *>(P.rays), 2);
+        k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns);
+        // The work counter starts after the slots' first units.
+        k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrWork, ns);
+        HIP_OK(hipGetLastError());
+        int n_closest = ns, n_shadow = 0, n_done = 0;
+        // Every pass advances each live path by one segment; a slot runs at
+        // most depth + 1 segments per sample it owns, plus one pass to retire.
+        const long long per_slot = F.mode == LAMBDA_RNG_REFERENCE ? F.spp * ((units + ns - 1) / ns + 1)
+                                                                  : (F.total_work + ns - 1) / ns + 1;
+        const long long max_iters = (long long)(F.depth + 2) * per_slot + 16;
+        long long iters = 0;
+        while (n_closest > 0 || n_shadow > 0 || n_done > 0) {
+            if (++iters > max_iters) throw Failure{LAMBDA_EHIP, "bounce loop did not drain"};
+            if ((long long)n_closest + n_done < tail) {
+                // settle pending occlusion rays and ended samples, then run
+                // every live slot, and the units still to start, to the end
+                if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
+                k_trace<<<grid_for(n_shadow), kBlock, 0, st>>>(g.ds, P, 0, n_shadow, B.spill.p);
+                if (stats) HIP_OK(hipEventRecord(ev[0].b, st));
+                k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done);
+                if (stats) HIP_OK(hipEventRecord(ev[3].a, st));
+                k_persist<<<grid_for((long long)n_closest + n_done), kBlock, 0, st>>>(g.ds, F, P, n_closest, B.spill.p);
+                if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
+                sync_counters();
+                if (stats) {
+                    unsigned long long r[2];
+                    HIP_OK(hipMemcpy(r, P.rays, sizeof r, hipMemcpyDeviceToHost));
+                    stats->closest_rays += (long long)r[0];
+                    stats->shadow_rays += n_shadow + (long long)r[1];
+                    timed(0);
+                    timed(3);
+                }
+                break;
+            }
+            if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
+            k_trace<<<grid_for((long long)n_closest + n_shadow), kBlock, 0, st>>>(g.ds, P, n_closest, n_shadow, B.spill.p);
+            if (stats) {
+                HIP_OK(hipEventRecord(ev[0].b, st));
+                HIP_OK(hipEventRecord(ev[2].a, st));
+            }
+            k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done);
+            if (stats) {
+                HIP_OK(hipEventRecord(ev[2].b, st));
+                HIP_OK(hipEventRecord(ev[1].a, st));
+            }
+            k_shade<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, n_closest);
+            if (stats) HIP_OK(hipEventRecord(ev[1].b, st));
+            sync_counters();
+            if (stats) {
+                stats->iterations++;
+                stats->closest_rays += n_closest;
+                stats->shadow_rays += n_shadow;
+                for (int k = 0; k < 3; k++) timed(k);
+            }
+            n_closest = host_ctr[kCtrNext];
+            n_shadow = host_ctr[kCtrShadow];
+            n_done = host_ctr[kCtrDone];
+            std::swap(P.q_in, P.q_out);
+            if (progress) {
+                long long done = host_ctr[kCtrPixels];
+                if (F.mode != LAMBDA_RNG_REFERENCE) done = done * F.npix / std::max<long long>(1, units);
+                *progress = int32_t(std::min<long long>(done, F.npix));
+            }
         }
     }
-    k_resolve<<<grid_for(F.npix), kBlock, 0, st>>>(F, P.fb, B.rgb.p);
+
+    // Visiting order by cost: a one-sample render with throughput streams
+    // counts each pixel's path segments; 8x8 tiles are then started in
+    // decreasing order of their summed count (longest first, so the last
+    // pixels to start are short), pixels within a tile in row order.  The order changes when a pixel
+    // is rendered, never what it is rendered with.
+    void order_by_cost(const Frame &F) {
+        Frame C = F;
+        C.mode = LAMBDA_RNG_THROUGHPUT;
+        C.spp = 1;
+        C.sqrt_spp = 1;
+        C.total_work = F.npix;
+        C.seed = F.seed ^ 0x9e3779b97f4a7c15ull;
+        DevBuf<int> cost;
+        cost.alloc(size_t(F.npix));
+        k_fill_i<<<grid_for(F.npix), kBlock, 0, st>>>(cost.p, F.npix);
+        C.cost = cost.p;
+        run(C, 1LL << 40, nullptr, nullptr);
+        std::vector<int> c(size_t(F.npix));
+        HIP_OK(hipMemcpyAsync(c.data(), cost.p, c.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        // tiles of 8x8 pixels (the visiting order's unit), by summed cost
+        const std::vector<int> tiles = tile_order(F.width, F.height);
+        std::vector<std::pair<long long, int>> tc;  // (-cost, first index in tiles)
+        for (size_t i = 0; i < tiles.size();) {
+            const int tx = (tiles[i] % F.width) / 8, ty = (tiles[i] / F.width) / 8;
+            size_t j = i;
+            long long sum = 0;
+            while (j < tiles.size() && (tiles[j] % F.width) / 8 == tx && (tiles[j] / F.width) / 8 == ty)
+                sum += c[size_t(tiles[j++])];
+            tc.emplace_back(-sum, int(i));
+            i = j;
+        }
+        std::stable_sort(tc.begin(), tc.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        std::vector<int> order;
+        order.reserve(tiles.size());
+        for (const auto &t : tc) {
+            const int tx = (tiles[size_t(t.second)] % F.width) / 8, ty = (tiles[size_t(t.second)] / F.width) / 8;
+            for (size_t j = size_t(t.second);
+                 j < tiles.size() && (tiles[j] % F.width) / 8 == tx && (tiles[j] / F.width) / 8 == ty; j++)
+                order.push_back(tiles[j]);
+        }
+        HIP_OK(hipMemcpyAsync(B.order.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        HIP_OK(hipStreamSynchronize(st));
+    }
+};
+
+void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_rgb, volatile int32_t *progress,
+                 lambda_render_stats *stats = nullptr) {
+    auto wall0 = std::chrono::steady_clock::now();
+    validate_params(p);
+    const Frame F = make_frame(p);
+    // Slots: paths in flight.  Reference mode: at most one per pixel (a slot
+    // owns a pixel for all its samples).
+    const long long units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
+    long long slots = p.path_slots > 0 ? p.path_slots : resident_persist_threads();
+    slots = std::max<long long>(1, std::min<long long>(slots, units));
+    if (slots > (1LL << 28)) throw Failure{LAMBDA_EINVAL, "too many path slots"};
+    // Below this many live paths the rest of the render runs in one
+    // persistent launch (per-thread chains instead of one pass per bounce).
+    const long long tail = p.tail_paths < 0 ? 0 : p.tail_paths > 0 ? p.tail_paths : kAutoTailPaths;
+    Renderer R(g, F, int(slots));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->path_slots = R.ns;
+    }
+    const bool by_cost = p.cost_order > 0 || (p.cost_order == 0 && F.mode == LAMBDA_RNG_REFERENCE && R.ns < F.npix);
+    if (by_cost) R.order_by_cost(F);
+    R.run(F, tail, progress, stats);
+    k_resolve<<<grid_for(F.npix), kBlock, 0, R.st>>>(F, R.P.fb, R.B.rgb.p);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(out_rgb, B.rgb.p, size_t(F.npix) * 3 * sizeof(float), hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
+    HIP_OK(hipMemcpyAsync(out_rgb, R.B.rgb.p, size_t(F.npix) * 3 * sizeof(float), hipMemcpyDeviceToHost, R.st));
+    HIP_OK(hipStreamSynchronize(R.st));
     if (progress) *progress = F.npix;
     if (stats)
         stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();

@@ -240,6 +240,7 @@ int lambda_scene_get_params(const lambda_scene *scene, lambda_render_params *p) 
     p->rng_mode = LAMBDA_RNG_REFERENCE;
     p->path_slots = 0;
     p->tail_paths = 0;
+    p->cost_order = 0;
     return LAMBDA_OK;
 }
 

@@ -217,3 +217,13 @@ def test_persistent_tail_throughput_mode(devices):
     a = devices["box"].render(tail_paths=-1, **kw)
     b = devices["box"].render(tail_paths=1 << 30, **kw)
     np.testing.assert_allclose(b, a, rtol=1e-6, atol=1e-7)
+
+
+def test_fewer_slots_than_pixels_reference_mode(devices):
+    """Reference streams with fewer slots than pixels: a slot that finishes
+    its pixel takes the next one, and each pixel still consumes its own
+    stream sample after sample, so the image is bit-identical."""
+    a = devices["mesh"].render(tail_paths=-1, cost_order=-1)
+    for slots, tail, order in ((1000, -1, -1), (1000, 0, 0), (777, 1 << 30, -1), (777, 1 << 30, 1), (1000, -1, 1)):
+        b = devices["mesh"].render(path_slots=slots, tail_paths=tail, cost_order=order)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (slots, tail, order)
