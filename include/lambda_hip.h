@@ -276,6 +276,46 @@ int lambda_hip_thinfilm_reflectance(lambda_hip_scene *scene, int32_t material, c
                                     const float *points, int32_t n, float *out,
                                     char *err, size_t err_len);
 
+/*
+ * Path-guiding network (no reference counterpart: the reference samples
+ * directions from the BSDF only).  A mixture of 8 von Mises-Fisher lobes
+ * over the next direction, whose parameters come from an MLP of (position,
+ * outgoing direction, wavelength): each of 6 features in [0, 1] is encoded
+ * as sin/cos of 2^b pi x, b = 0..7 (96 inputs), 4 hidden layers of 64 ReLU
+ * units, 40 outputs = 8 x {mean xyz (normalised), kappa (softplus),
+ * weight (softmax logit)}.  BF16 weights and activations on the matrix
+ * cores with FP32 accumulation; FP32 master weights; Adam.
+ *
+ * Parameters: for each layer l = 0..4, W_l[K_l][64] row-major then
+ * b_l[64]; K_0 = 96, K_1..4 = 64; output columns 40..63 are padding.
+ * Training sample: {features[6], direction[3], weight}; the loss of a
+ * sample is -weight * log q(direction).
+ */
+#define LAMBDA_GUIDE_PARAMS 22848
+#define LAMBDA_GUIDE_FEATURES 6
+#define LAMBDA_GUIDE_OUTPUTS 40
+#define LAMBDA_GUIDE_SAMPLE 10
+
+typedef struct lambda_guide lambda_guide;
+
+/* New network: He-uniform weights from PCG32 stream (seed, 1), zero biases. */
+int lambda_hip_guide_create(uint64_t seed, lambda_guide **out, char *err, size_t err_len);
+void lambda_hip_guide_destroy(lambda_guide *guide);
+int lambda_hip_guide_get_params(lambda_guide *guide, float *params, char *err, size_t err_len);
+/* Replaces the parameters and restarts Adam. */
+int lambda_hip_guide_set_params(lambda_guide *guide, const float *params, char *err, size_t err_len);
+/* out[i][0..39] = raw network outputs for features[i][0..5]. */
+int lambda_hip_guide_forward(lambda_guide *guide, const float *features, int32_t n, float *out,
+                             char *err, size_t err_len);
+/* Sum over the n samples of the gradient of the loss (grads[PARAMS]) and of
+ * the loss itself; the parameters do not change. */
+int lambda_hip_guide_gradient(lambda_guide *guide, const float *samples, int32_t n, float *grads, double *loss,
+                              char *err, size_t err_len);
+/* Adam steps (rate lr) over consecutive batches of `batch` samples; *loss
+ * receives the summed loss of the samples seen. */
+int lambda_hip_guide_train(lambda_guide *guide, const float *samples, int32_t n, int32_t batch, float lr,
+                           double *loss, char *err, size_t err_len);
+
 /* Device name and compute-unit count of the current HIP device. */
 int lambda_hip_device_info(char *name, size_t name_len, int32_t *compute_units,
                            char *err, size_t err_len);

@@ -217,3 +217,65 @@ def device_info() -> tuple[str, int]:
     err = C.create_string_buffer(4096)
     _check(_abi.hip().lambda_hip_device_info(name, 256, C.byref(cu), err, 4096), err)
     return name.value.decode(), cu.value
+
+
+GUIDE_PARAMS = 22848
+GUIDE_OUTPUTS = 40
+
+
+class Guide:
+    """The path-guiding network on the GPU (lambda_hip_guide_*)."""
+
+    def __init__(self, seed: int = 1):
+        self._lib = _abi.hip()
+        self._h = C.c_void_p()
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_guide_create(seed, C.byref(self._h), err, 4096), err)
+
+    def close(self):
+        if self._h:
+            self._lib.lambda_hip_guide_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(GUIDE_PARAMS, np.float32)
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_guide_get_params(self._h, _abi.ptr(out, C.c_float), err, 4096), err)
+        return out
+
+    def set_params(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.shape == (GUIDE_PARAMS,)
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_guide_set_params(self._h, _abi.ptr(p, C.c_float), err, 4096), err)
+
+    def forward(self, features: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, 6)
+        out = np.zeros((len(f), GUIDE_OUTPUTS), np.float32)
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_guide_forward(self._h, _abi.ptr(f, C.c_float), len(f),
+                                                  _abi.ptr(out, C.c_float), err, 4096), err)
+        return out
+
+    def gradient(self, samples: np.ndarray):
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 10)
+        g = np.zeros(GUIDE_PARAMS, np.float32)
+        loss = C.c_double()
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_guide_gradient(self._h, _abi.ptr(s, C.c_float), len(s), _abi.ptr(g, C.c_float),
+                                                   C.byref(loss), err, 4096), err)
+        return g, loss.value
+
+    def train(self, samples: np.ndarray, batch: int, lr: float = 1e-3) -> float:
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 10)
+        loss = C.c_double()
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_guide_train(self._h, _abi.ptr(s, C.c_float), len(s), batch, lr,
+                                                C.byref(loss), err, 4096), err)
+        return loss.value

@@ -127,6 +127,16 @@ def hip():
         lib.lambda_hip_thinfilm_reflectance.argtypes = [vp, C.c_int32, _f32p, _f32p, C.c_int32, _f32p,
                                                         C.c_char_p, C.c_size_t]
         lib.lambda_hip_device_info.argtypes = [C.c_char_p, C.c_size_t, _i32p, C.c_char_p, C.c_size_t]
+        f64p = C.POINTER(C.c_double)
+        lib.lambda_hip_guide_create.argtypes = [C.c_uint64, C.POINTER(vp), C.c_char_p, C.c_size_t]
+        lib.lambda_hip_guide_destroy.argtypes = [vp]
+        lib.lambda_hip_guide_destroy.restype = None
+        lib.lambda_hip_guide_get_params.argtypes = [vp, _f32p, C.c_char_p, C.c_size_t]
+        lib.lambda_hip_guide_set_params.argtypes = [vp, _f32p, C.c_char_p, C.c_size_t]
+        lib.lambda_hip_guide_forward.argtypes = [vp, _f32p, C.c_int32, _f32p, C.c_char_p, C.c_size_t]
+        lib.lambda_hip_guide_gradient.argtypes = [vp, _f32p, C.c_int32, _f32p, f64p, C.c_char_p, C.c_size_t]
+        lib.lambda_hip_guide_train.argtypes = [vp, _f32p, C.c_int32, C.c_int32, C.c_float, f64p, C.c_char_p,
+                                               C.c_size_t]
         _hip = lib
     return _hip
 

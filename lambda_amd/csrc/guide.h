@@ -1,0 +1,165 @@
+// Path-guiding distribution: a mixture of von Mises-Fisher lobes on the
+// sphere whose parameters come from a small MLP of (position, outgoing
+// direction, wavelength).  Host and device code (the CPU checks include it).
+//
+// Network: input = 6 scalars, each frequency-encoded with 8 bands as
+// sin(2^b pi x), cos(2^b pi x) (96 values); 4 hidden layers of 64 ReLU units;
+// output 64 values of which the first 40 are 8 lobes x {m.xyz, kappa_raw,
+// weight_logit}; the other 24 are padding kept at zero.
+//
+// Lobe k: mean mu_k = m_k / |m_k|, concentration kappa_k = softplus(kappa_raw)
+// clamped to [kKappaMin, kKappaMax], weight pi_k = softmax(weight_logit)_k.
+// Density of a lobe: vMF(w) = kappa / (2 pi (1 - exp(-2 kappa))) *
+// exp(kappa (mu.w - 1)), which tends to 1 / (4 pi) as kappa -> 0.
+#pragma once
+
+#include "dmath.h"
+
+namespace lam {
+
+constexpr int kGuideScalars = 6;
+constexpr int kGuideBands = 8;
+constexpr int kGuideIn = kGuideScalars * kGuideBands * 2;  // 96
+constexpr int kGuideHidden = 64;
+constexpr int kGuideLayers = 5;                            // 4 hidden + output
+constexpr int kGuideOut = 64;                              // padded output width
+constexpr int kLobes = 8;
+constexpr int kLobeOut = 5 * kLobes;                       // 40 used outputs
+constexpr float kKappaMin = 1e-3f;
+constexpr float kKappaMax = 2000.0f;
+
+// Parameter layout (fp32): for each layer l, W_l[K_l][64] row-major then
+// b_l[64]; K_0 = 96, K_1..4 = 64.
+LAM_FN int guide_layer_in(int l) { return l == 0 ? kGuideIn : kGuideHidden; }
+LAM_FN int guide_w_offset(int l) { return l == 0 ? 0 : kGuideIn * 64 + 64 + (l - 1) * (64 * 64 + 64); }
+LAM_FN int guide_b_offset(int l) { return guide_w_offset(l) + guide_layer_in(l) * 64; }
+constexpr int kGuideParams = kGuideIn * 64 + 64 + 4 * (64 * 64 + 64);  // 22848
+
+// Raw features of one query, each mapped to [0, 1]: position in the scene
+// box, outgoing direction as (theta / pi, (phi + pi) / 2pi), wavelength over
+// 360-830 nm.
+struct GuideFeatures {
+    float f[kGuideScalars];
+};
+
+LAM_FN GuideFeatures guide_features(V3 p, V3 lo, V3 hi, V3 wo, float lambda) {
+    GuideFeatures g;
+    g.f[0] = (p.x - lo.x) / fmaxf(hi.x - lo.x, 1e-6f);
+    g.f[1] = (p.y - lo.y) / fmaxf(hi.y - lo.y, 1e-6f);
+    g.f[2] = (p.z - lo.z) / fmaxf(hi.z - lo.z, 1e-6f);
+    g.f[3] = acosf(fminf(fmaxf(wo.y, -1.0f), 1.0f)) / kPi;
+    g.f[4] = (atan2f(wo.z, wo.x) + kPi) / (2 * kPi);
+    g.f[5] = (lambda - 360.0f) / 470.0f;
+    return g;
+}
+
+// Encoded input value i (0..95): scalar i / 16, band (i % 16) / 2, sin for
+// even i, cos for odd i.
+LAM_FN float guide_encode(const GuideFeatures &g, int i) {
+    const int s = i >> 4, b = (i & 15) >> 1;
+    const float x = g.f[s] * float(1 << b) * kPi;
+    return (i & 1) ? cosf(x) : sinf(x);
+}
+
+struct Lobe {
+    V3 mu;
+    float kappa, norm, weight;  // norm = kappa / (2 pi (1 - exp(-2 kappa)))
+};
+
+LAM_FN float softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
+
+LAM_FN float vmf_norm(float kappa) { return kappa / (2 * kPi * -expm1f(-2 * kappa)); }
+
+// Lobes from the 40 raw outputs.
+LAM_FN void guide_lobes(const float *o, Lobe (&L)[kLobes]) {
+    float mx = -INFINITY;
+    for (int k = 0; k < kLobes; k++) mx = fmaxf(mx, o[5 * k + 4]);
+    float sum = 0;
+    for (int k = 0; k < kLobes; k++) {
+        const float *q = o + 5 * k;
+        V3 m = v3(q[0], q[1], q[2]);
+        float l = fmaxf(len(m), 1e-6f);
+        L[k].mu = m * (1 / l);
+        L[k].kappa = fminf(fmaxf(softplus(q[3]), kKappaMin), kKappaMax);
+        L[k].norm = vmf_norm(L[k].kappa);
+        L[k].weight = expf(q[4] - mx);
+        sum += L[k].weight;
+    }
+    for (int k = 0; k < kLobes; k++) L[k].weight /= sum;
+}
+
+LAM_FN float guide_pdf(const Lobe (&L)[kLobes], V3 w) {
+    float p = 0;
+    for (int k = 0; k < kLobes; k++) p += L[k].weight * L[k].norm * expf(L[k].kappa * (dot(L[k].mu, w) - 1));
+    return p;
+}
+
+// Orthonormal frame around a unit vector (Duff et al., "Building an
+// orthonormal basis, revisited", JCGT 2017).
+LAM_FN void onb(V3 n, V3 &t, V3 &b) {
+    float s = copysignf(1.0f, n.z);
+    float a = -1 / (s + n.z);
+    float c = n.x * n.y * a;
+    t = v3(1 + s * n.x * n.x * a, s * c, -s * n.x);
+    b = v3(c, s + n.y * n.y * a, -n.y);
+}
+
+// Sample of one vMF lobe from two uniforms (W. Jakob, "Numerically stable
+// sampling of the von Mises Fisher distribution on S^2", 2012):
+// cos theta = 1 + log(1 + u1 (exp(-2 kappa) - 1)) / kappa.
+LAM_FN V3 vmf_sample(const Lobe &L, float u1, float u2) {
+    float w = 1 + log1pf(u1 * expm1f(-2 * L.kappa)) / L.kappa;
+    w = fminf(fmaxf(w, -1.0f), 1.0f);
+    float v = sqrtf(fmaxf(0.0f, 1 - w * w));
+    float phi = 2 * kPi * u2;
+    V3 t, b;
+    onb(L.mu, t, b);
+    return normalize(t * (v * cosf(phi)) + b * (v * sinf(phi)) + L.mu * w);
+}
+
+// Sample of the mixture: u0 picks the lobe, u1 and u2 sample it.
+LAM_FN V3 guide_sample(const Lobe (&L)[kLobes], float u0, float u1, float u2) {
+    int k = 0;
+    float c = L[0].weight;
+    while (k < kLobes - 1 && u0 >= c) c += L[++k].weight;
+    return vmf_sample(L[k], u1, u2);
+}
+
+// Loss of one training sample, -c log q(w), and its gradient with respect to
+// the 40 raw outputs (d[40]); returns the loss.
+LAM_FN float guide_loss_grad(const float *o, V3 w, float c, float *d) {
+    Lobe L[kLobes];
+    guide_lobes(o, L);
+    float comp[kLobes];
+    float q = 0;
+    for (int k = 0; k < kLobes; k++) {
+        comp[k] = L[k].norm * expf(L[k].kappa * (dot(L[k].mu, w) - 1));
+        q += L[k].weight * comp[k];
+    }
+    q = fmaxf(q, 1e-30f);
+    for (int k = 0; k < kLobes; k++) {
+        const float *r = o + 5 * k;
+        float *g = d + 5 * k;
+        const float resp = L[k].weight * comp[k] / q;  // posterior of lobe k
+        // d(-c log q)/d logit_k = -c (resp_k - pi_k)
+        g[4] = -c * (resp - L[k].weight);
+        // d log vMF / d kappa = 1/kappa - 2 exp(-2k)/(1 - exp(-2k)) + (mu.w - 1)
+        const float kap = L[k].kappa;
+        const float e = expf(-2 * kap);
+        const float dlog_dk = 1 / kap - 2 * e / (-expm1f(-2 * kap)) + (dot(L[k].mu, w) - 1);
+        const float sp = softplus(r[3]);
+        const float dk_draw = (sp > kKappaMin && sp < kKappaMax) ? 1 / (1 + expf(-r[3])) : 0.0f;
+        g[3] = -c * resp * dlog_dk * dk_draw;
+        // d log vMF / d mu = kappa w; mu = m / |m|: d/dm = (I - mu mu^T) / |m|
+        V3 m = v3(r[0], r[1], r[2]);
+        const float l = fmaxf(len(m), 1e-6f);
+        V3 gm = w * kap;
+        gm = (gm - L[k].mu * dot(L[k].mu, gm)) * (1 / l);
+        g[0] = -c * resp * gm.x;
+        g[1] = -c * resp * gm.y;
+        g[2] = -c * resp * gm.z;
+    }
+    return -c * logf(q);
+}
+
+}  // namespace lam

@@ -1,0 +1,374 @@
+// The guiding network on the GPU: BF16 weights and activations on MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation), FP32 master weights, Adam.
+// Included inside lambda_hip.hip's anonymous namespace, after its error and
+// buffer helpers (Failure, HIP_OK, DevBuf).
+//
+// Every kernel works on 16-row tiles, one per wave: a layer is
+// C[16 x 64] = A[16 x K] B[K x 64] as 4 column tiles x K/32 MFMA steps.
+// Operand layout of the MFMA (checked on gfx950): lane l holds
+// A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j = 0..7, and
+// receives C[4 (l / 16) + r][l % 16], r = 0..3.
+//
+// Weights are kept twice in BF16: w_nk[l] = W_l^T (n-major, the B operand of
+// the forward product) and w_kn[l] = W_l (k-major, the B operand of the
+// backward product dH = dZ W^T); both are rewritten from the FP32 master
+// copy after every Adam step.
+#pragma once
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kGuideRowsPerWave = 16;
+constexpr int kGuideWaves = 4;
+constexpr int kGuideRowsPerBlock = kGuideRowsPerWave * kGuideWaves;  // 64
+// LDS row strides (elements), padded by 8 to spread banks.
+constexpr int kLdX = kGuideIn + 8;
+constexpr int kLdH = kGuideHidden + 8;
+// bf16 weight elements per layout: sum over layers of K_l * 64.
+constexpr int kGuideWeightsBf = kGuideIn * 64 + 4 * 64 * 64;
+
+__host__ __device__ inline int guide_wbf_offset(int l) { return l == 0 ? 0 : kGuideIn * 64 + (l - 1) * 64 * 64; }
+
+struct GuideDev {
+    const float *params;     // fp32 master: W_l[K][64], b_l[64] per layer
+    const __bf16 *w_nk;      // W_l^T [64][K_l] per layer
+    const __bf16 *w_kn;      // W_l [K_l][64] per layer
+};
+
+// 8 consecutive bf16 from LDS or global memory (16-byte aligned).
+__device__ __forceinline__ bf16x8 load8(const __bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+
+// 8 bf16 with stride `stride` (elements) from LDS.
+__device__ __forceinline__ bf16x8 load8_strided(const __bf16 *p, int stride) {
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = p[j * stride];
+    return v;
+}
+
+// acc[nt] += A[16 x K] (LDS, row-major, stride lda) * B, B given n-major in
+// Bt[n][k] (stride ldb), for the 4 column tiles nt.
+__device__ __forceinline__ void gemm_rows16(const __bf16 *A, int lda, const __bf16 *Bt, int ldb, int K,
+                                            f32x4 (&acc)[4]) {
+    const int l = threadIdx.x & 63;
+    const int r = l & 15, q = l >> 4;
+    for (int kk = 0; kk < K; kk += 32) {
+        const bf16x8 a = load8(A + r * lda + kk + 8 * q);
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            const bf16x8 b = load8(Bt + (nt * 16 + r) * ldb + kk + 8 * q);
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nt], 0, 0, 0);
+        }
+    }
+}
+
+// Encodes a wave's 16 feature rows into X[16][kLdX] (bf16); rows >= n are 0.
+__device__ __forceinline__ void encode_rows(const float *feat, int fstride, int row0, int n, __bf16 *X) {
+    const int l = threadIdx.x & 63;
+    for (int e = l; e < kGuideRowsPerWave * kGuideIn; e += 64) {
+        const int r = e / kGuideIn, i = e % kGuideIn;
+        float v = 0;
+        if (row0 + r < n) {
+            GuideFeatures g;
+            const float *f = feat + size_t(row0 + r) * fstride;
+#pragma unroll
+            for (int s = 0; s < kGuideScalars; s++) g.f[s] = f[s];
+            v = guide_encode(g, i);
+        }
+        X[r * kLdX + i] = __bf16(v);
+    }
+}
+
+// One layer for a wave's 16 rows: H_out = act(H_in W_l + b_l).  relu: hidden
+// layers; otherwise the fp32 result goes to out (16 x 64, stride 64).
+__device__ __forceinline__ void layer_rows16(const GuideDev &G, int layer, const __bf16 *Hin, int ldin,
+                                             __bf16 *Hout, float *out) {
+    const int l = threadIdx.x & 63;
+    const int K = guide_layer_in(layer);
+    f32x4 acc[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) acc[nt] = f32x4{0, 0, 0, 0};
+    gemm_rows16(Hin, ldin, G.w_nk + guide_wbf_offset(layer), K, K, acc);
+    const float *bias = G.params + guide_b_offset(layer);
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        const int c = nt * 16 + (l & 15);
+        const float bc = bias[c];
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            const int row = 4 * (l >> 4) + rr;
+            const float v = acc[nt][rr] + bc;
+            if (Hout) Hout[row * kLdH + c] = __bf16(fmaxf(v, 0.0f));
+            else out[row * 64 + c] = v;
+        }
+    }
+}
+
+// Inference: out[i][0..39] for feature rows feat[i][0..5], i < n.  Each
+// wave takes 16 rows, each block 64.
+__global__ void __launch_bounds__(256) k_guide_forward(GuideDev G, const float *feat, int fstride, int n,
+                                                       float *out, int ostride) {
+    __shared__ __attribute__((aligned(16))) __bf16 sX[kGuideRowsPerBlock * kLdX];
+    __shared__ __attribute__((aligned(16))) __bf16 sH[2][kGuideRowsPerBlock * kLdH];
+    __shared__ float sO[kGuideRowsPerBlock * 64];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int wr = w * kGuideRowsPerWave;
+    const int row0 = blockIdx.x * kGuideRowsPerBlock + wr;
+    encode_rows(feat, fstride, row0, n, sX + wr * kLdX);
+    __syncthreads();
+    layer_rows16(G, 0, sX + wr * kLdX, kLdX, sH[0] + wr * kLdH, nullptr);
+    __syncthreads();
+    layer_rows16(G, 1, sH[0] + wr * kLdH, kLdH, sH[1] + wr * kLdH, nullptr);
+    __syncthreads();
+    layer_rows16(G, 2, sH[1] + wr * kLdH, kLdH, sH[0] + wr * kLdH, nullptr);
+    __syncthreads();
+    layer_rows16(G, 3, sH[0] + wr * kLdH, kLdH, sH[1] + wr * kLdH, nullptr);
+    __syncthreads();
+    layer_rows16(G, 4, sH[1] + wr * kLdH, kLdH, nullptr, sO + wr * 64);
+    __syncthreads();
+    for (int e = l; e < kGuideRowsPerWave * kLobeOut; e += 64) {
+        const int r = e / kLobeOut, c = e % kLobeOut;
+        if (row0 + r < n) out[size_t(row0 + r) * ostride + c] = sO[(wr + r) * 64 + c];
+    }
+}
+
+// Training sample: features[6], direction[3], weight c.
+constexpr int kGuideSample = 10;
+
+// Gradient of sum_i -c_i log q(w_i) over samples [0, n) into grads (fp32,
+// parameter layout, accumulated with atomics) and the loss into *loss.
+// Each block takes 64 rows: forward with stored activations, the loss
+// gradient per row, then backward through the five layers.  Weight
+// gradients dW_l = H_{l-1}^T dZ_l sum over the block's rows on the MFMA
+// (16 x 16 tiles shared among the block's waves).
+__global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float *samples, int n, float *grads,
+                                                        double *loss) {
+    __shared__ __attribute__((aligned(16))) __bf16 sX[kGuideRowsPerBlock * kLdX];
+    __shared__ __attribute__((aligned(16))) __bf16 sH[4][kGuideRowsPerBlock * kLdH];   // H1..H4
+    __shared__ __attribute__((aligned(16))) __bf16 sD[2][kGuideRowsPerBlock * kLdH];   // dZ ping-pong
+    __shared__ float sO[kGuideRowsPerBlock * 64];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int r16 = l & 15, q = l >> 4;
+    const int row0 = blockIdx.x * kGuideRowsPerBlock;
+    const int wr = w * kGuideRowsPerWave;  // this wave's first row in the block
+
+    // forward
+    encode_rows(samples, kGuideSample, row0 + wr, n, sX + wr * kLdX);
+    __syncthreads();
+    layer_rows16(G, 0, sX + wr * kLdX, kLdX, sH[0] + wr * kLdH, nullptr);
+    __syncthreads();
+    layer_rows16(G, 1, sH[0] + wr * kLdH, kLdH, sH[1] + wr * kLdH, nullptr);
+    __syncthreads();
+    layer_rows16(G, 2, sH[1] + wr * kLdH, kLdH, sH[2] + wr * kLdH, nullptr);
+    __syncthreads();
+    layer_rows16(G, 3, sH[2] + wr * kLdH, kLdH, sH[3] + wr * kLdH, nullptr);
+    __syncthreads();
+    layer_rows16(G, 4, sH[3] + wr * kLdH, kLdH, nullptr, sO + wr * 64);
+    __syncthreads();
+
+    // loss gradient per row (one lane per row), dZ_4 in bf16, padding zero
+    float lsum = 0;
+    if (threadIdx.x < kGuideRowsPerBlock) {
+        const int r = threadIdx.x;
+        float d[kLobeOut];
+#pragma unroll
+        for (int c = 0; c < kLobeOut; c++) d[c] = 0;
+        if (row0 + r < n) {
+            const float *sm = samples + size_t(row0 + r) * kGuideSample;
+            lsum = guide_loss_grad(sO + r * 64, v3(sm[6], sm[7], sm[8]), sm[9], d);
+        }
+#pragma unroll
+        for (int c = 0; c < 64; c++) sD[0][r * kLdH + c] = __bf16(c < kLobeOut ? d[c] : 0.0f);
+    }
+    // wave 0 holds all 64 rows' losses
+    if (w == 0) {
+        for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+        if (l == 0) atomicAdd(loss, double(lsum));
+    }
+    __syncthreads();
+
+    int cur = 0;
+    for (int layer = kGuideLayers - 1; layer >= 0; layer--) {
+        const int K = guide_layer_in(layer);
+        const __bf16 *Hprev = layer == 0 ? sX : sH[layer - 1];
+        const int ldp = layer == 0 ? kLdX : kLdH;
+        const __bf16 *dZ = sD[cur];
+        float *gW = grads + guide_w_offset(layer);
+        float *gb = grads + guide_b_offset(layer);
+        // dW_l[K x 64] = H_prev^T dZ over the block's 64 rows: tiles (kt, nt)
+        const int ktiles = K / 16;
+        for (int t = w; t < ktiles * 4; t += kGuideWaves) {
+            const int kt = t >> 2, nt = t & 3;
+            f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int rr = 0; rr < kGuideRowsPerBlock; rr += 32) {
+                // A = H_prev^T: A[i][rows] = H_prev[rows][kt*16 + i]
+                const bf16x8 a = load8_strided(Hprev + (rr + 8 * q) * ldp + kt * 16 + r16, ldp);
+                // B = dZ: B[rows][j] = dZ[rows][nt*16 + j]
+                const bf16x8 b = load8_strided(dZ + (rr + 8 * q) * kLdH + nt * 16 + r16, kLdH);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int k = kt * 16 + 4 * q + rr, c = nt * 16 + r16;
+                atomicAdd(gW + k * 64 + c, acc[rr]);
+            }
+        }
+        // db_l = column sums of dZ
+        if (threadIdx.x < 64) {
+            float s = 0;
+            for (int r = 0; r < kGuideRowsPerBlock; r++) s += float(dZ[r * kLdH + threadIdx.x]);
+            atomicAdd(gb + threadIdx.x, s);
+        }
+        if (layer > 0) {
+            // dH_prev[16 x 64] = dZ[16 x 64] W_l^T for this wave's rows; the
+            // B operand W_l^T[n][k] is read n-major from W_l (k-major)
+            f32x4 acc[4];
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) acc[nt] = f32x4{0, 0, 0, 0};
+            gemm_rows16(dZ + wr * kLdH, kLdH, G.w_kn + guide_wbf_offset(layer), 64, 64, acc);
+            // dZ_prev = dH_prev where H_prev > 0
+            __bf16 *dn = sD[cur ^ 1];
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) {
+                const int c = nt * 16 + r16;
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) {
+                    const int row = wr + 4 * q + rr;
+                    const float h = float(Hprev[row * ldp + c]);
+                    dn[row * kLdH + c] = __bf16(h > 0 ? acc[nt][rr] : 0.0f);
+                }
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+}
+
+// Adam step over all parameters with gradient scale `scale` (1 / batch
+// weight); clears the gradients and rewrites the BF16 weight copies.
+__global__ void k_guide_adam(float *params, float *grads, float *m, float *v, __bf16 *w_nk, __bf16 *w_kn,
+                             float lr, float b1, float b2, float eps, float c1, float c2, float scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kGuideParams) return;
+    const float g = grads[i] * scale;
+    grads[i] = 0;
+    const float mi = b1 * m[i] + (1 - b1) * g;
+    const float vi = b2 * v[i] + (1 - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    float p = params[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    params[i] = p;
+    // weight element? find its layer
+    for (int layer = 0; layer < kGuideLayers; layer++) {
+        const int wo = guide_w_offset(layer), K = guide_layer_in(layer);
+        if (i >= wo && i < wo + K * 64) {
+            const int k = (i - wo) / 64, c = (i - wo) % 64;
+            w_kn[guide_wbf_offset(layer) + k * 64 + c] = __bf16(p);
+            w_nk[guide_wbf_offset(layer) + c * K + k] = __bf16(p);
+        }
+    }
+}
+
+// Writes the BF16 copies from the master weights.
+__global__ void k_guide_pack(const float *params, __bf16 *w_nk, __bf16 *w_kn) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kGuideParams) return;
+    const float p = params[i];
+    for (int layer = 0; layer < kGuideLayers; layer++) {
+        const int wo = guide_w_offset(layer), K = guide_layer_in(layer);
+        if (i >= wo && i < wo + K * 64) {
+            const int k = (i - wo) / 64, c = (i - wo) % 64;
+            w_kn[guide_wbf_offset(layer) + k * 64 + c] = __bf16(p);
+            w_nk[guide_wbf_offset(layer) + c * K + k] = __bf16(p);
+        }
+    }
+}
+
+// Host side of the network: parameters, Adam state and the BF16 copies.
+struct GuideNet {
+    DevBuf<float> params, grads, m, v;
+    DevBuf<__bf16> w_nk, w_kn;
+    DevBuf<double> loss;
+    int step = 0;
+    float lr = 1e-3f;
+
+    GuideDev dev() const { return GuideDev{params.p, w_nk.p, w_kn.p}; }
+
+    void alloc() {
+        params.alloc(kGuideParams);
+        grads.alloc(kGuideParams);
+        m.alloc(kGuideParams);
+        v.alloc(kGuideParams);
+        w_nk.alloc(kGuideWeightsBf);
+        w_kn.alloc(kGuideWeightsBf);
+        loss.alloc(1);
+    }
+
+    // He-uniform weights from a seeded PCG32, zero biases; the output layer
+    // starts small (near-uniform lobes) and its padding columns stay zero.
+    void init(uint64_t seed) {
+        alloc();
+        std::vector<float> p(kGuideParams, 0.0f);
+        Pcg rng = pcg_seed(seed, 1);
+        for (int layer = 0; layer < kGuideLayers; layer++) {
+            const int K = guide_layer_in(layer);
+            const float a = layer == kGuideLayers - 1 ? 0.1f * sqrtf(6.0f / K) : sqrtf(6.0f / K);
+            for (int k = 0; k < K; k++)
+                for (int c = 0; c < 64; c++) {
+                    float u = rng.next();
+                    p[size_t(guide_w_offset(layer) + k * 64 + c)] =
+                        (layer == kGuideLayers - 1 && c >= kLobeOut) ? 0.0f : a * (2 * u - 1);
+                }
+        }
+        set(p.data(), nullptr);
+        reset_adam(nullptr);
+    }
+
+    void reset_adam(hipStream_t st) {
+        HIP_OK(hipMemsetAsync(m.p, 0, kGuideParams * sizeof(float), st));
+        HIP_OK(hipMemsetAsync(v.p, 0, kGuideParams * sizeof(float), st));
+        HIP_OK(hipMemsetAsync(grads.p, 0, kGuideParams * sizeof(float), st));
+        HIP_OK(hipStreamSynchronize(st));
+        step = 0;
+    }
+
+    void set(const float *host, hipStream_t st) {
+        HIP_OK(hipMemcpyAsync(params.p, host, kGuideParams * sizeof(float), hipMemcpyHostToDevice, st));
+        k_guide_pack<<<(kGuideParams + 255) / 256, 256, 0, st>>>(params.p, w_nk.p, w_kn.p);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(st));
+    }
+
+    void forward(const float *feat, int fstride, int n, float *out, int ostride, hipStream_t st) const {
+        if (n <= 0) return;
+        const int blocks = (n + kGuideRowsPerBlock - 1) / kGuideRowsPerBlock;
+        k_guide_forward<<<blocks, 256, 0, st>>>(dev(), feat, fstride, n, out, ostride);
+        HIP_OK(hipGetLastError());
+    }
+
+    // Accumulates the gradient of samples [0, n) into grads.
+    void gradient(const float *samples, int n, hipStream_t st) {
+        if (n <= 0) return;
+        k_guide_gradient<<<(n + kGuideRowsPerBlock - 1) / kGuideRowsPerBlock, 256, 0, st>>>(dev(), samples, n,
+                                                                                            grads.p, loss.p);
+        HIP_OK(hipGetLastError());
+    }
+
+    void adam(float scale, hipStream_t st) {
+        step++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1 - powf(b1, float(step)), c2 = 1 - powf(b2, float(step));
+        k_guide_adam<<<(kGuideParams + 255) / 256, 256, 0, st>>>(params.p, grads.p, m.p, v.p, w_nk.p, w_kn.p, lr, b1,
+                                                                 b2, 1e-8f, c1, c2, scale);
+        HIP_OK(hipGetLastError());
+    }
+
+    // Minibatch Adam over samples [0, n): steps of `batch` samples.
+    void train(const float *samples, int n, int batch, hipStream_t st) {
+        for (int b0 = 0; b0 + batch <= n; b0 += batch) {
+            gradient(samples + size_t(b0) * kGuideSample, batch, st);
+            adam(1.0f / float(batch), st);
+        }
+    }
+};

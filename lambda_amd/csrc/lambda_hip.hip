@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -30,6 +31,7 @@
 
 #include "lambda_hip.h"
 #include "dscene.h"
+#include "guide.h"
 #include "scene_prep.h"
 
 namespace lam {
@@ -89,6 +91,8 @@ struct DevBuf {
     }
     void upload(const std::vector<T> &v) { upload(v.data(), v.size()); }
 };
+
+#include "guide_mlp.h"
 
 // --------------------------------------------------------- wave utilities
 
@@ -898,6 +902,10 @@ struct lambda_hip_scene {
     std::vector<int> material_type;
 };
 
+struct lambda_guide {
+    GuideNet net;
+};
+
 namespace {
 
 void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g) {
@@ -1468,6 +1476,78 @@ int lambda_hip_thinfilm_reflectance(lambda_hip_scene *scene, int32_t material, c
         k_query_thinfilm<<<grid_for(n), kBlock>>>(scene->ds, material, q.p, p.p, n, o.p);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpy(out, o.p, size_t(n) * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int lambda_hip_guide_create(uint64_t seed, lambda_guide **out, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!out) throw Failure{LAMBDA_EINVAL, "null argument"};
+        check_device();
+        auto g = std::make_unique<lambda_guide>();
+        g->net.init(seed);
+        *out = g.release();
+    });
+}
+
+void lambda_hip_guide_destroy(lambda_guide *guide) { delete guide; }
+
+int lambda_hip_guide_get_params(lambda_guide *guide, float *params, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!guide || !params) throw Failure{LAMBDA_EINVAL, "null argument"};
+        HIP_OK(hipMemcpy(params, guide->net.params.p, kGuideParams * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int lambda_hip_guide_set_params(lambda_guide *guide, const float *params, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!guide || !params) throw Failure{LAMBDA_EINVAL, "null argument"};
+        guide->net.set(params, nullptr);
+        guide->net.reset_adam(nullptr);
+    });
+}
+
+int lambda_hip_guide_forward(lambda_guide *guide, const float *features, int32_t n, float *out, char *err,
+                             size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!guide || n < 0 || (n > 0 && (!features || !out))) throw Failure{LAMBDA_EINVAL, "invalid arguments"};
+        if (n == 0) return;
+        DevBuf<float> f, o;
+        f.upload(features, size_t(n) * kGuideScalars);
+        o.alloc(size_t(n) * kLobeOut);
+        guide->net.forward(f.p, kGuideScalars, n, o.p, kLobeOut, nullptr);
+        HIP_OK(hipMemcpy(out, o.p, size_t(n) * kLobeOut * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int lambda_hip_guide_gradient(lambda_guide *guide, const float *samples, int32_t n, float *grads, double *loss,
+                              char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!guide || n < 0 || !grads || (n > 0 && !samples)) throw Failure{LAMBDA_EINVAL, "invalid arguments"};
+        GuideNet &net = guide->net;
+        DevBuf<float> sm;
+        if (n > 0) sm.upload(samples, size_t(n) * kGuideSample);
+        HIP_OK(hipMemset(net.grads.p, 0, kGuideParams * sizeof(float)));
+        HIP_OK(hipMemset(net.loss.p, 0, sizeof(double)));
+        net.gradient(sm.p, n, nullptr);
+        HIP_OK(hipMemcpy(grads, net.grads.p, kGuideParams * sizeof(float), hipMemcpyDeviceToHost));
+        if (loss) HIP_OK(hipMemcpy(loss, net.loss.p, sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemset(net.grads.p, 0, kGuideParams * sizeof(float)));
+    });
+}
+
+int lambda_hip_guide_train(lambda_guide *guide, const float *samples, int32_t n, int32_t batch, float lr,
+                           double *loss, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!guide || n < 0 || batch <= 0 || (n > 0 && !samples) || !(lr > 0))
+            throw Failure{LAMBDA_EINVAL, "invalid arguments"};
+        GuideNet &net = guide->net;
+        DevBuf<float> sm;
+        if (n > 0) sm.upload(samples, size_t(n) * kGuideSample);
+        HIP_OK(hipMemset(net.loss.p, 0, sizeof(double)));
+        net.lr = lr;
+        net.train(sm.p, n, batch, nullptr);
+        HIP_OK(hipDeviceSynchronize());
+        if (loss) HIP_OK(hipMemcpy(loss, net.loss.p, sizeof(double), hipMemcpyDeviceToHost));
     });
 }
 

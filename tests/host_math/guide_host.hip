@@ -1,0 +1,84 @@
+// Test driver: the guiding distribution of guide.h (encoding, vMF mixture
+// density, sampling, loss gradient) compiled for the host.
+//
+// argv: encode   <features.f32 [n][6]>            <out.f32 [n][96]>
+//       pdf      <outputs.f32 [n][40]> <dirs.f32 [n][3]> <out.f32 [n]>
+//       grad     <outputs.f32 [n][40]> <dirs.f32 [n][3]> <c.f32 [n]> <out.f32 [n][41] = loss, d[40]>
+//       sample   <outputs.f32 [40]> <count> <seed>   <out.f32 [count][3]>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "guide.h"
+
+using namespace lam;
+
+static std::vector<float> read_f32(const char *path) {
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    size_t n = size_t(f.tellg());
+    f.seekg(0);
+    std::vector<float> v(n / 4);
+    f.read(reinterpret_cast<char *>(v.data()), std::streamsize(n));
+    return v;
+}
+
+static void write_f32(const char *path, const std::vector<float> &v) {
+    std::ofstream o(path, std::ios::binary);
+    o.write(reinterpret_cast<const char *>(v.data()), std::streamsize(v.size() * 4));
+}
+
+int main(int argc, char **argv) {
+    if (argc < 4) return 2;
+    const std::string mode = argv[1];
+    if (mode == "encode") {
+        std::vector<float> f = read_f32(argv[2]);
+        const size_t n = f.size() / kGuideScalars;
+        std::vector<float> out(n * kGuideIn);
+        for (size_t r = 0; r < n; r++) {
+            GuideFeatures g;
+            for (int s = 0; s < kGuideScalars; s++) g.f[s] = f[r * kGuideScalars + s];
+            for (int i = 0; i < kGuideIn; i++) out[r * kGuideIn + i] = guide_encode(g, i);
+        }
+        write_f32(argv[3], out);
+    } else if (mode == "pdf" && argc >= 5) {
+        std::vector<float> o = read_f32(argv[2]), w = read_f32(argv[3]);
+        const size_t n = w.size() / 3;
+        std::vector<float> out(n);
+        for (size_t r = 0; r < n; r++) {
+            Lobe L[kLobes];
+            guide_lobes(&o[r * kLobeOut], L);
+            out[r] = guide_pdf(L, v3(w[3 * r], w[3 * r + 1], w[3 * r + 2]));
+        }
+        write_f32(argv[4], out);
+    } else if (mode == "grad" && argc >= 6) {
+        std::vector<float> o = read_f32(argv[2]), w = read_f32(argv[3]), c = read_f32(argv[4]);
+        const size_t n = c.size();
+        std::vector<float> out(n * (kLobeOut + 1));
+        for (size_t r = 0; r < n; r++) {
+            float d[kLobeOut];
+            out[r * (kLobeOut + 1)] = guide_loss_grad(&o[r * kLobeOut], v3(w[3 * r], w[3 * r + 1], w[3 * r + 2]), c[r], d);
+            std::memcpy(&out[r * (kLobeOut + 1) + 1], d, sizeof d);
+        }
+        write_f32(argv[5], out);
+    } else if (mode == "sample" && argc >= 6) {
+        std::vector<float> o = read_f32(argv[2]);
+        const int count = std::atoi(argv[3]);
+        Pcg rng = pcg_seed(uint64_t(std::atoll(argv[4])), 7);
+        Lobe L[kLobes];
+        guide_lobes(o.data(), L);
+        std::vector<float> out(size_t(count) * 3);
+        for (int i = 0; i < count; i++) {
+            float u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
+            V3 d = guide_sample(L, u0, u1, u2);
+            out[3 * size_t(i)] = d.x;
+            out[3 * size_t(i) + 1] = d.y;
+            out[3 * size_t(i) + 2] = d.z;
+        }
+        write_f32(argv[5], out);
+    } else {
+        return 2;
+    }
+    return 0;
+}

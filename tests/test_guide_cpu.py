@@ -1,0 +1,171 @@
+"""The guiding distribution without a GPU: the FP64 oracle pinned by finite
+differences and closed forms, and the device math of guide.h (compiled for
+the host) against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+import guide_oracle as go
+
+
+def random_params(rng, scale=1.0):
+    p = np.zeros(go.N_PARAMS)
+    for l in range(go.LAYERS):
+        k = go.layer_in(l)
+        w = rng.uniform(-1, 1, (k, 64)) * np.sqrt(6 / k) * scale
+        if l == go.LAYERS - 1:
+            w[:, go.N_OUT:] = 0
+        p[go.w_offset(l):go.w_offset(l) + k * 64] = w.ravel()
+        p[go.b_offset(l):go.b_offset(l) + 64] = rng.uniform(-0.1, 0.1, 64)
+    p[go.b_offset(go.LAYERS - 1) + go.N_OUT:go.b_offset(go.LAYERS - 1) + 64] = 0
+    return p
+
+
+def random_dirs(rng, n):
+    w = rng.normal(size=(n, 3))
+    return w / np.linalg.norm(w, axis=1, keepdims=True)
+
+
+def random_samples(rng, n):
+    s = np.zeros((n, 10))
+    s[:, :6] = rng.random((n, 6))
+    s[:, 6:9] = random_dirs(rng, n)
+    s[:, 9] = rng.uniform(0.1, 2.0, n)
+    return s
+
+
+def test_oracle_gradient_matches_finite_differences():
+    """Central differences of the float64 loss against the analytic
+    backpropagation, on parameters of every layer (weights and biases)."""
+    rng = np.random.default_rng(3)
+    p = random_params(rng)
+    s = random_samples(rng, 24)
+    g, _ = go.gradient(p, s)
+    idx = []
+    for l in range(go.LAYERS):
+        k = go.layer_in(l)
+        cols = go.N_OUT if l == go.LAYERS - 1 else 64
+        for _ in range(12):
+            idx.append(go.w_offset(l) + rng.integers(k) * 64 + rng.integers(cols))
+        for _ in range(4):
+            idx.append(go.b_offset(l) + rng.integers(cols))
+    h = 1e-6
+    for i in idx:
+        q = p.copy()
+        q[i] += h
+        lp = go.loss(q, s)
+        q[i] -= 2 * h
+        lm = go.loss(q, s)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-5 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_padding_outputs_have_zero_gradient():
+    rng = np.random.default_rng(4)
+    p = random_params(rng)
+    g, _ = go.gradient(p, random_samples(rng, 16))
+    w4 = g[go.w_offset(4):go.w_offset(4) + 64 * 64].reshape(64, 64)
+    assert np.all(w4[:, go.N_OUT:] == 0)
+    assert np.all(g[go.b_offset(4) + go.N_OUT:go.b_offset(4) + 64] == 0)
+
+
+def sphere_grid(n_theta=720, n_phi=1440):
+    """Midpoint quadrature nodes and weights on the unit sphere."""
+    th = (np.arange(n_theta) + 0.5) * np.pi / n_theta
+    ph = (np.arange(n_phi) + 0.5) * 2 * np.pi / n_phi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    w = np.sin(T) * (np.pi / n_theta) * (2 * np.pi / n_phi)
+    d = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1)
+    return d.reshape(-1, 3), w.ravel()
+
+
+@pytest.mark.parametrize("kscale", [-8.0, 0.0, 3.0, 6.0])
+def test_oracle_mixture_integrates_to_one(kscale):
+    """The vMF mixture is a density: its integral over the sphere is 1 for
+    small (kappa -> 1e-3), moderate and sharp (kappa ~ 400) lobes."""
+    rng = np.random.default_rng(5)
+    o = rng.normal(size=(1, go.N_OUT))
+    o[0, 3::5] = kscale + rng.normal(scale=0.2, size=8)
+    d, w = sphere_grid()
+    q = go.pdf(np.repeat(o, len(d), 0), d)
+    assert abs((q * w).sum() - 1) < 2e-3
+
+
+def test_oracle_small_kappa_is_uniform():
+    o = np.zeros((3, go.N_OUT))
+    o[:, 3::5] = -30  # softplus -> below the clamp: kappa = 1e-3
+    o[:, 0::5] = 1
+    q = go.pdf(o, random_dirs(np.random.default_rng(0), 3))
+    np.testing.assert_allclose(q, 1 / (4 * np.pi), rtol=2e-3)
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    if not os.path.exists("/opt/rocm/bin/hipcc"):
+        pytest.skip("hipcc not available")
+    out = tmp_path_factory.mktemp("guide") / "guide_host"
+    cmd = ["/opt/rocm/bin/hipcc", "-O2", "-ffp-contract=off", "--offload-arch=gfx950",
+           "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "lambda_amd", "csrc"),
+           os.path.join(ROOT, "tests", "host_math", "guide_host.hip"), "-o", str(out)]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return str(out)
+
+
+def run(driver, tmp_path, args, arrays, out_shape):
+    paths = []
+    for k, a in enumerate(arrays):
+        p = tmp_path / ("in%d.f32" % k)
+        np.ascontiguousarray(a, np.float32).tofile(p)
+        paths.append(str(p))
+    op = tmp_path / "out.f32"
+    subprocess.run([driver, args[0], *paths, *args[1:], str(op)], check=True)
+    return np.fromfile(op, np.float32).reshape(out_shape)
+
+
+def test_device_encoding_matches_oracle(driver, tmp_path):
+    f = np.random.default_rng(1).random((500, 6)).astype(np.float32)
+    got = run(driver, tmp_path, ["encode"], [f], (500, 96))
+    # float32 arguments up to 128 pi carry ~3e-5 of rounding; the network
+    # stores its input in bfloat16 (8 significant bits) anyway
+    np.testing.assert_allclose(got, go.encode(f), atol=1e-4)
+
+
+def test_device_pdf_and_gradient_match_oracle(driver, tmp_path):
+    rng = np.random.default_rng(2)
+    n = 400
+    o = rng.normal(size=(n, go.N_OUT)).astype(np.float32)
+    o[:, 3::5] = rng.uniform(-4, 6, (n, 8))
+    w = random_dirs(rng, n).astype(np.float32)
+    c = rng.uniform(0.1, 3, n).astype(np.float32)
+    q = run(driver, tmp_path, ["pdf"], [o, w], (n,))
+    np.testing.assert_allclose(q, go.pdf(o.astype(np.float64), w.astype(np.float64)), rtol=2e-4)
+    got = run(driver, tmp_path, ["grad"], [o, w, c], (n, 41))
+    lref, dref = go.loss_grad(o.astype(np.float64), w.astype(np.float64), c.astype(np.float64))
+    np.testing.assert_allclose(got[:, 0], lref, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(got[:, 1:], dref, rtol=2e-3, atol=2e-3 * np.abs(dref).max())
+
+
+def test_device_sampling_follows_pdf(driver, tmp_path):
+    """Histogram of 400k mixture samples over 12 x 24 sphere cells against
+    the cells' integrated density (oracle quadrature)."""
+    rng = np.random.default_rng(6)
+    o = rng.normal(size=(1, go.N_OUT)).astype(np.float32)
+    o[0, 3::5] = rng.uniform(0, 4, 8)
+    count = 400000
+    d = run(driver, tmp_path, ["sample", str(count), "11"], [o], (count, 3)).astype(np.float64)
+    assert np.all(np.abs(np.linalg.norm(d, axis=1) - 1) < 1e-5)
+    nt, nphi = 12, 24
+    th = np.arccos(np.clip(d[:, 1], -1, 1))
+    ph = np.mod(np.arctan2(d[:, 2], d[:, 0]), 2 * np.pi)
+    hist = np.histogram2d(th, ph, bins=[nt, nphi], range=[[0, np.pi], [0, 2 * np.pi]])[0] / count
+    g, wq = sphere_grid(nt * 40, nphi * 40)
+    q = go.pdf(np.repeat(o.astype(np.float64), len(g), 0), g) * wq
+    cell = q.reshape(nt, 40, nphi, 40).sum(axis=(1, 3))
+    assert abs(cell.sum() - 1) < 1e-3
+    sigma = np.sqrt(cell * (1 - cell) / count) + 1e-7
+    z = np.abs(hist - cell) / sigma
+    assert z.max() < 6, z.max()
