@@ -316,6 +316,39 @@ int lambda_hip_guide_gradient(lambda_guide *guide, const float *samples, int32_t
 int lambda_hip_guide_train(lambda_guide *guide, const float *samples, int32_t n, int32_t batch, float lr,
                            double *loss, char *err, size_t err_len);
 
+/*
+ * Progressive rendering (no reference counterpart): a session accumulates
+ * samples over steps into one FP64 framebuffer, with throughput streams
+ * (fresh PCG32 streams per step).  With a guide, Lambertian vertices sample
+ * the one-sample mixture 0.5 cos/pi + 0.5 q(guide), whose density is the
+ * scatter pdf of both MIS weights, so every step is unbiased; a step can
+ * then train the guide (Adam, batches of 2^14) on up to 2^18 samples of the
+ * radiance its own paths gathered at Lambertian vertices.  Without a guide
+ * the session is the plain tracer.
+ */
+typedef struct lambda_hip_progressive lambda_hip_progressive;
+
+typedef struct lambda_progressive_stats {
+    int64_t samples;        /* samples per pixel accumulated so far */
+    int64_t train_samples;  /* training samples collected by the last step */
+    double render_ms;       /* device time of the last step's rendering */
+    double train_ms;        /* device time of its training (0 if none) */
+    double loss;            /* mean training loss of that training */
+} lambda_progressive_stats;
+
+/* params: size, depth, wavelength range, seed and path_slots are used;
+ * rng_mode and samples are ignored.  guide may be NULL; the session borrows
+ * it (keep it alive until the session is destroyed). */
+int lambda_hip_progressive_create(lambda_hip_scene *scene, const lambda_render_params *params, lambda_guide *guide,
+                                  lambda_hip_progressive **out, char *err, size_t err_len);
+/* Renders `samples` (a perfect square) more samples per pixel; trains the
+ * guide afterwards when train != 0. */
+int lambda_hip_progressive_step(lambda_hip_progressive *session, int32_t samples, int32_t train,
+                                lambda_progressive_stats *stats, char *err, size_t err_len);
+/* out_rgb[W*H*3]: linear sRGB mean over all samples so far. */
+int lambda_hip_progressive_image(lambda_hip_progressive *session, float *out_rgb, char *err, size_t err_len);
+void lambda_hip_progressive_destroy(lambda_hip_progressive *session);
+
 /* Device name and compute-unit count of the current HIP device. */
 int lambda_hip_device_info(char *name, size_t name_len, int32_t *compute_units,
                            char *err, size_t err_len);

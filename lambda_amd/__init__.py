@@ -279,3 +279,62 @@ class Guide:
         _check(self._lib.lambda_hip_guide_train(self._h, _abi.ptr(s, C.c_float), len(s), batch, lr,
                                                 C.byref(loss), err, 4096), err)
         return loss.value
+
+
+class Progressive:
+    """Progressive rendering session (lambda_hip_progressive_*): plain when
+    guide is None, guided otherwise.  The scene and guide must outlive it."""
+
+    def __init__(self, dev: "DeviceScene", guide: "Guide | None" = None, path_slots: int = 0, **overrides):
+        self._lib = _abi.hip()
+        self.dev, self.guide = dev, guide
+        p = dev.scene.render_params(path_slots=path_slots, **overrides)
+        self.width, self.height = p.width, p.height
+        self._h = C.c_void_p()
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_progressive_create(dev._h, C.byref(p), guide._h if guide else None,
+                                                       C.byref(self._h), err, 4096), err)
+
+    def step(self, samples: int = 1, train: bool = True) -> dict:
+        st = _abi.ProgressiveStats()
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_progressive_step(self._h, samples, 1 if train else 0, C.byref(st), err, 4096),
+               err)
+        return {"samples": st.samples, "train_samples": st.train_samples, "render_ms": st.render_ms,
+                "train_ms": st.train_ms, "loss": st.loss}
+
+    def image(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_progressive_image(self._h, _abi.ptr(out, C.c_float), err, 4096), err)
+        return out
+
+    def close(self):
+        if self._h:
+            self._lib.lambda_hip_progressive_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def finite_pixels(*imgs: np.ndarray) -> np.ndarray:
+    """Mask of the pixels whose channels are finite in every image (the
+    reference estimator yields 0/0 in rare pixels when randomFloat() returns
+    1.0; render/renderer.h:179-183)."""
+    m = np.ones(imgs[0].shape[:2], bool)
+    for im in imgs:
+        m &= np.isfinite(im).all(-1)
+    return m
+
+
+def rel_mse(img: np.ndarray, ref: np.ndarray) -> float:
+    """Mean over pixels and channels of (I - R)^2 / (R^2 + 0.01), over the
+    pixels finite in both images."""
+    m = finite_pixels(img, ref)
+    a = img[m].astype(np.float64)
+    r = ref[m].astype(np.float64)
+    return float(np.mean((a - r) ** 2 / (r * r + 0.01)))

@@ -67,6 +67,13 @@ class RenderStats(C.Structure):
     ]
 
 
+class ProgressiveStats(C.Structure):
+    _fields_ = [
+        ("samples", C.c_int64), ("train_samples", C.c_int64), ("render_ms", C.c_double),
+        ("train_ms", C.c_double), ("loss", C.c_double),
+    ]
+
+
 KERNELS = ("trace", "shade", "finalize", "persist")
 
 
@@ -137,6 +144,12 @@ def hip():
         lib.lambda_hip_guide_gradient.argtypes = [vp, _f32p, C.c_int32, _f32p, f64p, C.c_char_p, C.c_size_t]
         lib.lambda_hip_guide_train.argtypes = [vp, _f32p, C.c_int32, C.c_int32, C.c_float, f64p, C.c_char_p,
                                                C.c_size_t]
+        lib.lambda_hip_progressive_create.argtypes = [vp, rp, vp, C.POINTER(vp), C.c_char_p, C.c_size_t]
+        lib.lambda_hip_progressive_step.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(ProgressiveStats),
+                                                    C.c_char_p, C.c_size_t]
+        lib.lambda_hip_progressive_image.argtypes = [vp, _f32p, C.c_char_p, C.c_size_t]
+        lib.lambda_hip_progressive_destroy.argtypes = [vp]
+        lib.lambda_hip_progressive_destroy.restype = None
         _hip = lib
     return _hip
 

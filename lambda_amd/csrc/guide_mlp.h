@@ -104,10 +104,12 @@ __device__ __forceinline__ void layer_rows16(const GuideDev &G, int layer, const
     }
 }
 
-// Inference: out[i][0..39] for feature rows feat[i][0..5], i < n.  Each
-// wave takes 16 rows, each block 64.
+// Inference: out[i][0..39] for feature rows feat[i][0..5], i < n (n read
+// from n_dev when given).  Each wave takes 16 rows, each block 64.
 __global__ void __launch_bounds__(256) k_guide_forward(GuideDev G, const float *feat, int fstride, int n,
-                                                       float *out, int ostride) {
+                                                       const int *n_dev, float *out, int ostride) {
+    if (n_dev) n = *n_dev;
+    if (int(blockIdx.x) * kGuideRowsPerBlock >= n) return;
     __shared__ __attribute__((aligned(16))) __bf16 sX[kGuideRowsPerBlock * kLdX];
     __shared__ __attribute__((aligned(16))) __bf16 sH[2][kGuideRowsPerBlock * kLdH];
     __shared__ float sO[kGuideRowsPerBlock * 64];
@@ -343,7 +345,15 @@ struct GuideNet {
     void forward(const float *feat, int fstride, int n, float *out, int ostride, hipStream_t st) const {
         if (n <= 0) return;
         const int blocks = (n + kGuideRowsPerBlock - 1) / kGuideRowsPerBlock;
-        k_guide_forward<<<blocks, 256, 0, st>>>(dev(), feat, fstride, n, out, ostride);
+        k_guide_forward<<<blocks, 256, 0, st>>>(dev(), feat, fstride, n, nullptr, out, ostride);
+        HIP_OK(hipGetLastError());
+    }
+
+    // Inference for at most `bound` rows, the count read on the device.
+    void forward_dev(const float *feat, const int *n_dev, int bound, float *out, hipStream_t st) const {
+        if (bound <= 0) return;
+        const int blocks = (bound + kGuideRowsPerBlock - 1) / kGuideRowsPerBlock;
+        k_guide_forward<<<blocks, 256, 0, st>>>(dev(), feat, kGuideScalars, 0, n_dev, out, kLobeOut);
         HIP_OK(hipGetLastError());
     }
 

@@ -115,6 +115,7 @@ __device__ __forceinline__ int wave_push(int *counter, bool pred) {
 enum : int {
     kFlagSpecular = 1,   // previous vertex was specular (or the camera)
     kFlagFinished = 2,   // the sample's path has ended
+    kFlagGuidePending = 4,  // a guided vertex awaits the radiance base of its continuation
 };
 
 enum : int {
@@ -124,6 +125,8 @@ enum : int {
     kCtrPixels = 3,      // finished pixels (reference mode) / samples (throughput)
     kCtrError = 4,       // error bits
     kCtrDone = 5,        // samples ended by the last shade pass
+    kCtrGuided = 6,      // paths queued for the guiding network this pass
+    kCtrTrain = 7,       // training samples written since the last training
     kCtrCount = 8
 };
 
@@ -144,6 +147,7 @@ struct Paths {
     unsigned long long *rays; // rays traced by the persistent kernel: closest, occlusion
     double *fb;               // XYZ sums, [pixel][3]
     const int *order;         // pixel visiting order (8x8 tiles)
+    float4 *grec;             // guided vertices, [slot][bounce][3] (guided passes only)
 };
 
 struct Frame {
@@ -152,6 +156,10 @@ struct Frame {
     uint64_t seed;
     long long total_work;     // npix * spp (throughput mode)
     int *cost;                // when set: per-pixel sum of path segments
+    float *train;             // when set: training samples [train_cap][10]
+    int train_cap;
+    int grec_stride;          // guided vertices per slot (depth + 1)
+    float3 box_lo, box_hi;    // scene box for the guiding features
 };
 
 // New camera sample `sample` of `pixel` into slot s (render/renderer.h:71-82,
@@ -308,12 +316,25 @@ struct ShadowRay {
     int4 i;
 };
 
+// Guided vertex record: {features 0-3}, {features 4-5, w.xy}, {w.z,
+// cos / pdf, hero throughput after the vertex, radiance base}; radiance base
+// -2 = no record, -1 = awaiting the next vertex.
+constexpr float kRecNone = -2.0f, kRecPending = -1.0f;
+
 // One iteration body of Renderer::trace (render/renderer.h:102-192) for the
 // path in slot s after its closest-hit query (P.hit[s]).  Updates the slot's
 // state; returns true when the sample's path has ended.  *push_shadow tells
 // whether next-event estimation produced an occlusion ray.
+//
+// kGuided: Lambertian vertices sample the next direction from the one-sample
+// mixture 0.5 cos/pi + 0.5 q (q: the guiding distribution from the network
+// outputs gout); the mixture density is the scatter pdf of both MIS
+// weights (next-event estimation and emitter hits), so the estimator stays
+// unbiased.  The vertex is recorded for training in P.grec.
+template <bool kGuided = false>
 __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F, const Paths &P, int s,
-                                           bool &push_shadow, ShadowRay &sh) {
+                                           bool &push_shadow, ShadowRay &sh, const float *gout = nullptr,
+                                           const float *gfeat = nullptr) {
     push_shadow = false;
     {
         float4 o4 = P.o[s], d4 = P.d[s];
@@ -364,6 +385,10 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     const bool lambertian = mt.x == LAMBDA_MATERIAL_LAMBERTIAN;
                     S4 albedo = s4(0);
                     if (lambertian) albedo = lambertian_albedo(S, mt, h, lam);
+                    Lobe GL[kGuided ? kLobes : 1];
+                    if constexpr (kGuided) {
+                        if (lambertian) guide_lobes(gout, GL);
+                    }
                     if (lambertian && total > 0) {
                         // Next-event estimation (render/renderer.h:133-175).
                         float target = total * rng.next();
@@ -404,6 +429,7 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                                 S4 att = s4(0);
                                 if (dot(h.n, ldir) > 0) att = albedo / kPi;
                                 float sp = ct > 0 ? ct / kPi : 0;
+                                if constexpr (kGuided) sp = 0.5f * sp + 0.5f * guide_pdf(GL, ldir);
                                 float w = power_heuristic(lprob, sp);
                                 S4 c = beta * att * ct * em * w / lprob;
                                 push_shadow = true;
@@ -418,7 +444,16 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     S4 att;
                     V3 nd;
                     if (lambertian) {
-                        nd = random_in_hemisphere(h.n, rng);
+                        if constexpr (kGuided) {
+                            if (rng.next() < 0.5f) {
+                                nd = random_in_hemisphere(h.n, rng);
+                            } else {
+                                float u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
+                                nd = guide_sample(GL, u0, u1, u2);
+                            }
+                        } else {
+                            nd = random_in_hemisphere(h.n, rng);
+                        }
                         att = dot(h.n, nd) <= 0 ? s4(0) : albedo / kPi;
                     } else if (mt.x == LAMBDA_MATERIAL_MIRROR) {
                         nd = reflected(d, h.n);
@@ -473,6 +508,13 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     d = nd;
                     float ct = dot(h.n, d);
                     prev_pdf = lambertian ? (ct > 0 ? ct / kPi : 0) : 0;
+                    if constexpr (kGuided) {
+                        if (lambertian) {
+                            prev_pdf = 0.5f * prev_pdf + 0.5f * guide_pdf(GL, d);
+                            // below the surface the contribution is zero
+                            if (!(ct > 0) || !(prev_pdf > 0)) finished = true;
+                        }
+                    }
                     specular = !lambertian;
                     if (specular) {
                         beta = beta * att;
@@ -489,6 +531,15 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                         }
                     }
                     if (bounce >= F.depth) finished = true;
+                    if constexpr (kGuided) {
+                        if (lambertian && !finished && bounce < F.grec_stride) {
+                            float4 *r = P.grec + (size_t(s) * F.grec_stride + bounce) * 3;
+                            r[0] = make_float4(gfeat[0], gfeat[1], gfeat[2], gfeat[3]);
+                            r[1] = make_float4(gfeat[4], gfeat[5], d.x, d.y);
+                            r[2] = make_float4(d.z, ct / prev_pdf, beta.v[0], kRecPending);
+                            info.w |= kFlagGuidePending;
+                        }
+                    }
                 }
             }
         }
@@ -497,7 +548,9 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
         P.beta[s] = f4(beta);
         P.L[s] = f4(L);
         store_rng(P, s, rng);
-        P.info[s] = make_int4(info.x, info.y, bounce + 1, (specular ? kFlagSpecular : 0) | (finished ? kFlagFinished : 0));
+        P.info[s] = make_int4(info.x, info.y, bounce + 1,
+                              (specular ? kFlagSpecular : 0) | (finished ? kFlagFinished : 0) |
+                                  (info.w & kFlagGuidePending));
         return finished;
     }
 }
@@ -568,12 +621,123 @@ __device__ __forceinline__ void sample_xyz(const DeviceScene &S, const Frame &F,
 // Retires the samples that ended in the previous shade pass (their
 // occlusion rays have been traced since): radiance to the framebuffer, then
 // the slot's next sample, which joins the closest-hit queue.
+// Paths queued for the guiding network in a guided pass.
+struct GuideQueue {
+    int *slot;       // [n] slot of each queued path
+    float *feat;     // [n][6] features
+    float *out;      // [n][40] network outputs
+};
+
+// Guided pass, step 1: every queued path first closes the record of its
+// previous guided vertex (the radiance gathered so far is the base of that
+// vertex's continuation); paths on a Lambertian surface then queue their
+// features for the network, all others are shaded at once.
+__global__ void __launch_bounds__(kBlock) k_shade_guided_queue(DeviceScene S, Frame F, Paths P, int count,
+                                                               GuideQueue Q) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = q < count;
+    bool guided = false, push_shadow = false, path_done = false;
+    int s = 0;
+    ShadowRay sh;
+    GuideFeatures gf;
+    if (active) {
+        s = P.q_in[q];
+        int4 info = P.info[s];
+        if (info.w & kFlagGuidePending) {
+            if (info.z >= 1 && info.z - 1 < F.grec_stride)
+                P.grec[(size_t(s) * F.grec_stride + info.z - 1) * 3 + 2].w = P.L[s].x;
+            info.w &= ~kFlagGuidePending;
+            P.info[s] = info;
+        }
+        if (info.z < F.grec_stride) P.grec[(size_t(s) * F.grec_stride + info.z) * 3 + 2].w = kRecNone;
+        const int2 hit = P.hit[s];
+        if (hit.x >= 0 && S.mat[material_of(S, hit.x, hit.y)].x == LAMBDA_MATERIAL_LAMBERTIAN) {
+            const float4 o4 = P.o[s], d4 = P.d[s];
+            Hit h;
+            if (hit_record(S, hit.x, hit.y, v3(o4), v3(d4), h)) {
+                gf = guide_features(h.p, v3(F.box_lo.x, F.box_lo.y, F.box_lo.z), v3(F.box_hi.x, F.box_hi.y, F.box_hi.z),
+                                    v3(-d4.x, -d4.y, -d4.z), P.lam[s].x);
+                guided = true;
+            }
+        }
+        if (!guided) path_done = shade_slot<false>(S, F, P, s, push_shadow, sh);
+    }
+    const int gp = wave_push(&P.ctr[kCtrGuided], guided);
+    if (guided) {
+        Q.slot[gp] = s;
+#pragma unroll
+        for (int k = 0; k < kGuideScalars; k++) Q.feat[size_t(gp) * kGuideScalars + k] = gf.f[k];
+    }
+    const bool plain = active && !guided;
+    int pos = wave_push(&P.ctr[kCtrShadow], push_shadow);
+    if (push_shadow) {
+        P.s_o[pos] = sh.o;
+        P.s_d[pos] = sh.d;
+        P.s_c[pos] = sh.c;
+        P.s_i[pos] = sh.i;
+    }
+    const bool cont = plain && !path_done;
+    const int pn = wave_push(&P.ctr[kCtrNext], cont);
+    if (cont) P.q_out[pn] = s;
+    const bool ended = plain && path_done;
+    const int pd = wave_push(&P.ctr[kCtrDone], ended);
+    if (ended) P.done[pd] = s;
+}
+
+// Guided pass, step 2: the queued Lambertian paths shade with the network's
+// mixture (P.ctr[kCtrGuided] of them).
+__global__ void __launch_bounds__(kBlock) k_shade_guided(DeviceScene S, Frame F, Paths P, GuideQueue Q) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = q < P.ctr[kCtrGuided];
+    bool push_shadow = false, path_done = false;
+    int s = 0;
+    ShadowRay sh;
+    if (active) {
+        s = Q.slot[q];
+        path_done = shade_slot<true>(S, F, P, s, push_shadow, sh, Q.out + size_t(q) * kLobeOut,
+                                     Q.feat + size_t(q) * kGuideScalars);
+    }
+    int pos = wave_push(&P.ctr[kCtrShadow], push_shadow);
+    if (push_shadow) {
+        P.s_o[pos] = sh.o;
+        P.s_d[pos] = sh.d;
+        P.s_c[pos] = sh.c;
+        P.s_i[pos] = sh.i;
+    }
+    const bool cont = active && !path_done;
+    const int pn = wave_push(&P.ctr[kCtrNext], cont);
+    if (cont) P.q_out[pn] = s;
+    const bool ended = active && path_done;
+    const int pd = wave_push(&P.ctr[kCtrDone], ended);
+    if (ended) P.done[pd] = s;
+}
+
 // Adds the ended sample of slot s to its pixel's XYZ sum.  In reference
 // mode the pixel's next sample then starts in the same slot, continuing the
 // pixel's PCG32 stream (render/renderer.h:69-91); returns true when it did.
 __device__ __forceinline__ bool retire_sample(const DeviceScene &S, const Frame &F, const Paths &P, int s) {
     int4 info = P.info[s];
     if (F.cost) atomicAdd(F.cost + info.x, info.z);
+    if (F.train) {
+        // training samples from the sample's guided vertices: the radiance
+        // that reached vertex b along its sampled direction is the radiance
+        // gathered after it over the throughput after it (hero wavelength)
+        const float Lend = P.L[s].x;
+        for (int b = 0; b < info.z && b < F.grec_stride; b++) {
+            float4 *r = P.grec + (size_t(s) * F.grec_stride + b) * 3;
+            const float4 r2 = r[2];
+            if (!(r2.w >= 0)) continue;
+            r[2].w = kRecNone;
+            const float c = (Lend - r2.w) / r2.z * r2.y;
+            if (!(c > 0) || !isfinite(c)) continue;
+            const int k = atomicAdd(&P.ctr[kCtrTrain], 1);
+            if (k >= F.train_cap) continue;
+            const float4 r0 = r[0], r1 = r[1];
+            float *o = F.train + size_t(k) * kGuideSample;
+            o[0] = r0.x; o[1] = r0.y; o[2] = r0.z; o[3] = r0.w; o[4] = r1.x; o[5] = r1.y;
+            o[6] = r1.z; o[7] = r1.w; o[8] = r2.x; o[9] = c;
+        }
+    }
     float x, y, z;
     sample_xyz(S, F, P.L[s], P.lam[s], x, y, z);
     double *px = P.fb + size_t(info.x) * 3;
@@ -598,6 +762,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(DeviceScene S, Frame F, Pat
     if (q == 0) {
         P.ctr[kCtrShadow] = 0;
         P.ctr[kCtrDone] = 0;
+        P.ctr[kCtrGuided] = 0;
     }
     bool keep = false;
     bool need_work = false;
@@ -891,6 +1056,7 @@ struct lambda_hip_scene {
     DevBuf<float4> wide, obj, obj_s, inst, stex;
     DevBuf<int> inst_root, obj_mat, props, perm, tlas_rank, obj_rank;
     int stack_need = 0;     // deepest traversal stack of the scene
+    float box[6] = {0, 0, 0, 0, 0, 0};  // scene bounds (instance-level root)
     DevBuf<float4> worley_points;
     DevBuf<int4> worley_box, worley_dim;
     DevBuf<int4> inst_i, mat, xtex;
@@ -913,6 +1079,7 @@ void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g) {
     const size_t O = size_t(v.num_objects), I = size_t(v.num_instances);
     WideBuilder wb(v);
     g.stack_need = wb.run();
+    for (int k = 0; k < 6; k++) g.box[k] = v.node_bounds[k];
     std::vector<float4> ob4 = pack_objects(v), os = pack_shading(v);
     std::vector<int> om(O);
     for (size_t i = 0; i < O; i++) om[i] = v.object_material[i];
@@ -1125,7 +1292,7 @@ long long resident_persist_threads() {
 }
 
 Frame make_frame(const lambda_render_params &p) {
-    Frame F;
+    Frame F{};  // every optional pointer (cost, train) starts null
     F.width = p.width;
     F.height = p.height;
     F.depth = p.depth;
@@ -1183,9 +1350,12 @@ struct Renderer {
         if (host_ctr[kCtrError]) throw Failure{LAMBDA_EHIP, "device reported an inconsistent hit record"};
     }
 
-    // Renders F into the FP64 framebuffer: one launch per bounce while at
-    // least `tail` paths are live, then one persistent launch.
-    void run(const Frame &F, long long tail, volatile int32_t *progress, lambda_render_stats *stats) {
+    // Renders F into the FP64 framebuffer (added to it when clear is
+    // false): one launch per bounce while at least `tail` paths are live,
+    // then one persistent launch.  With a guide, every pass runs the
+    // network on the paths at Lambertian vertices (no persistent launch).
+    void run(const Frame &F, long long tail, volatile int32_t *progress, lambda_render_stats *stats,
+             bool clear = true, const GuideNet *guide = nullptr, GuideQueue *Q = nullptr) {
         const long long units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
         if (stats) {
             for (auto &e : ev) {
@@ -1199,9 +1369,10 @@ struct Renderer {
             stats->kernel_ms[k] += ms;
             stats->launches[k]++;
         };
-        k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
+        if (clear) k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
         k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
         k_fill_d<<<1, kBlock, 0, st>>>(reinterpret_cast<double *>(P.rays), 2);
+        if (guide) tail = 0;
         k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns);
         // The work counter starts after the slots' first units.
         k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrWork, ns);
@@ -1247,7 +1418,13 @@ struct Renderer {
                 HIP_OK(hipEventRecord(ev[2].b, st));
                 HIP_OK(hipEventRecord(ev[1].a, st));
             }
-            k_shade<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, n_closest);
+            if (guide) {
+                k_shade_guided_queue<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, n_closest, *Q);
+                guide->forward_dev(Q->feat, P.ctr + kCtrGuided, n_closest, Q->out, st);
+                k_shade_guided<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, *Q);
+            } else {
+                k_shade<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, n_closest);
+            }
             if (stats) HIP_OK(hipEventRecord(ev[1].b, st));
             sync_counters();
             if (stats) {
@@ -1343,6 +1520,46 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
     if (progress) *progress = F.npix;
     if (stats)
         stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+}
+
+}  // namespace
+
+// Progressive rendering with throughput streams: each step adds samples
+// with fresh PCG32 streams to one FP64 framebuffer; with a guide, Lambertian
+// vertices sample the guiding mixture and each step can train the network on
+// the radiance its own paths gathered.
+struct lambda_hip_progressive {
+    lambda_hip_scene *scene = nullptr;
+    lambda_render_params params{};
+    GuideNet *net = nullptr;
+    std::unique_ptr<Renderer> R;
+    DevBuf<int> q_slot;
+    DevBuf<float> q_feat, q_out, train;
+    DevBuf<float4> grec;
+    GuideQueue Q{};
+    long long total = 0;
+    int steps = 0;
+    int train_cap = 1 << 18;
+    int batch = 1 << 14;
+};
+
+namespace {
+
+Frame progressive_frame(const lambda_hip_progressive &pr, int samples, int root) {
+    Frame F = make_frame(pr.params);
+    F.mode = LAMBDA_RNG_THROUGHPUT;
+    F.spp = samples;
+    F.sqrt_spp = root;
+    F.total_work = (long long)F.npix * samples;
+    F.seed = pr.params.seed + 0x9e3779b97f4a7c15ull * uint64_t(pr.steps + 1);
+    F.box_lo = make_float3(pr.scene->box[0], pr.scene->box[1], pr.scene->box[2]);
+    F.box_hi = make_float3(pr.scene->box[3], pr.scene->box[4], pr.scene->box[5]);
+    if (pr.net) {
+        F.train = pr.train.p;
+        F.train_cap = pr.train_cap;
+        F.grec_stride = pr.params.depth + 1;
+    }
+    return F;
 }
 
 template <typename F>
@@ -1550,6 +1767,99 @@ int lambda_hip_guide_train(lambda_guide *guide, const float *samples, int32_t n,
         if (loss) HIP_OK(hipMemcpy(loss, net.loss.p, sizeof(double), hipMemcpyDeviceToHost));
     });
 }
+
+int lambda_hip_progressive_create(lambda_hip_scene *scene, const lambda_render_params *params, lambda_guide *guide,
+                                  lambda_hip_progressive **out, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scene || !params || !out) throw Failure{LAMBDA_EINVAL, "null argument"};
+        lambda_render_params p = *params;
+        p.rng_mode = LAMBDA_RNG_THROUGHPUT;
+        p.samples = 1;
+        p.sqrt_samples = 1;
+        validate_params(p);
+        auto pr = std::make_unique<lambda_hip_progressive>();
+        pr->scene = scene;
+        pr->params = p;
+        pr->net = guide ? &guide->net : nullptr;
+        const Frame F = make_frame(p);
+        long long slots = params->path_slots > 0 ? params->path_slots
+                          : guide ? (1LL << 20) : resident_persist_threads();
+        slots = std::max<long long>(1, std::min<long long>(slots, 1LL << 26));
+        pr->R = std::make_unique<Renderer>(*scene, F, int(slots));
+        if (guide) {
+            const size_t ns = size_t(slots);
+            pr->q_slot.alloc(ns);
+            pr->q_feat.alloc(ns * kGuideScalars);
+            pr->q_out.alloc(ns * kLobeOut);
+            pr->grec.alloc(ns * size_t(p.depth + 1) * 3);
+            pr->train.alloc(size_t(pr->train_cap) * kGuideSample);
+            pr->Q = GuideQueue{pr->q_slot.p, pr->q_feat.p, pr->q_out.p};
+            pr->R->P.grec = pr->grec.p;
+        }
+        *out = pr.release();
+    });
+}
+
+int lambda_hip_progressive_step(lambda_hip_progressive *pr, int32_t samples, int32_t train,
+                                lambda_progressive_stats *stats, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!pr || samples < 1) throw Failure{LAMBDA_EINVAL, "invalid arguments"};
+        int root = int(std::lround(std::sqrt(double(samples))));
+        if (root * root != samples) throw Failure{LAMBDA_EINVAL, "samples per step must be a perfect square"};
+        Renderer &R = *pr->R;
+        const Frame F = progressive_frame(*pr, samples, root);
+        if ((long long)F.npix * samples >= (1LL << 31) - (1LL << 28))
+            throw Failure{LAMBDA_EINVAL, "too many samples in one step"};
+        EventPair ev;
+        HIP_OK(hipEventCreate(&ev.a));
+        HIP_OK(hipEventCreate(&ev.b));
+        HIP_OK(hipEventRecord(ev.a, R.st));
+        R.run(F, pr->net ? 0 : (1LL << 40), nullptr, nullptr, pr->steps == 0, pr->net, pr->net ? &pr->Q : nullptr);
+        HIP_OK(hipEventRecord(ev.b, R.st));
+        HIP_OK(hipEventSynchronize(ev.b));
+        float render_ms = 0;
+        HIP_OK(hipEventElapsedTime(&render_ms, ev.a, ev.b));
+        const long long collected = pr->net ? std::min<long long>(R.host_ctr[kCtrTrain], pr->train_cap) : 0;
+        float train_ms = 0;
+        double loss = 0;
+        if (pr->net && train && collected >= pr->batch) {
+            GuideNet &net = *pr->net;
+            HIP_OK(hipMemsetAsync(net.loss.p, 0, sizeof(double), R.st));
+            HIP_OK(hipEventRecord(ev.a, R.st));
+            net.train(pr->train.p, int(collected), pr->batch, R.st);
+            HIP_OK(hipEventRecord(ev.b, R.st));
+            HIP_OK(hipEventSynchronize(ev.b));
+            HIP_OK(hipEventElapsedTime(&train_ms, ev.a, ev.b));
+            HIP_OK(hipMemcpy(&loss, net.loss.p, sizeof(double), hipMemcpyDeviceToHost));
+            loss /= double(collected / pr->batch * pr->batch);
+        }
+        pr->total += samples;
+        pr->steps++;
+        if (stats) {
+            stats->samples = pr->total;
+            stats->train_samples = collected;
+            stats->render_ms = render_ms;
+            stats->train_ms = train_ms;
+            stats->loss = loss;
+        }
+    });
+}
+
+int lambda_hip_progressive_image(lambda_hip_progressive *pr, float *out_rgb, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!pr || !out_rgb) throw Failure{LAMBDA_EINVAL, "null argument"};
+        if (pr->total == 0) throw Failure{LAMBDA_EINVAL, "no samples rendered yet"};
+        Renderer &R = *pr->R;
+        Frame F = make_frame(pr->params);
+        F.spp = int(std::min<long long>(pr->total, 1LL << 30));
+        k_resolve<<<grid_for(F.npix), kBlock, 0, R.st>>>(F, R.P.fb, R.B.rgb.p);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(out_rgb, R.B.rgb.p, size_t(F.npix) * 3 * sizeof(float), hipMemcpyDeviceToHost, R.st));
+        HIP_OK(hipStreamSynchronize(R.st));
+    });
+}
+
+void lambda_hip_progressive_destroy(lambda_hip_progressive *pr) { delete pr; }
 
 int lambda_hip_device_info(char *name, size_t name_len, int32_t *compute_units, char *err, size_t err_len) {
     return guarded(err, err_len, [&] {
