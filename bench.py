@@ -90,6 +90,90 @@ def cpu_baseline(scene, seconds: float, threads: int) -> dict:
                           r0, r0 + rows, p.height, rows * p.width, spp, t)}
 
 
+def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, warmup: int = 0,
+                   train_steps: int = 1 << 30, epochs: int = 1, batch: int = 1 << 14):
+    """Progressive render until relMSE against ref <= target; returns the
+    render-and-train wall time at which the target was crossed (log-linear
+    interpolation between steps), the samples per pixel then, and the curve.
+    With a guide: `warmup` steps train without adding to the image, then
+    the first `train_steps` kept steps train too.  relMSE evaluation is not
+    timed."""
+    import lambda_amd
+    p = lambda_amd.Progressive(dev, guide)
+    if guide is not None:
+        p.set_training(epochs, batch, 1e-3)
+    elapsed, spp, curve = 0.0, 0, []
+    prev = None
+    hit = None
+    if guide is not None:
+        for _ in range(warmup):
+            t0 = time.perf_counter()
+            p.step(step_spp, train=True, keep=False)
+            elapsed += time.perf_counter() - t0
+    kept = 0
+    while spp < max_spp:
+        t0 = time.perf_counter()
+        p.step(step_spp, train=guide is not None and kept < train_steps)
+        elapsed += time.perf_counter() - t0
+        kept += 1
+        spp += step_spp
+        err = lambda_amd.rel_mse(p.image(), ref)
+        curve.append((round(elapsed, 4), spp, err))
+        if err <= target:
+            if prev is None:
+                hit = (elapsed, spp)
+            else:
+                import math
+                (t1, e1), (t2, e2) = (prev[0], prev[2]), (elapsed, err)
+                f = (math.log(e1) - math.log(target)) / max(math.log(e1) - math.log(e2), 1e-12)
+                hit = (t1 + f * (t2 - t1), spp)
+            break
+        prev = (elapsed, spp, err)
+    p.close()
+    return hit, curve
+
+
+def guided_leg(args, scene_dir: str) -> dict:
+    """Wall time to relMSE 1e-3 against a plain reference of 65536 spp on the
+    room lit through a door gap, plain tracer vs guided (the guide trains on
+    the paths of every step)."""
+    import lambda_amd
+    from lambda_amd import scenes
+    w, h = (int(x) for x in args.guided_res.split("x"))
+    lrd = scenes.write_guiding_room(os.path.join(scene_dir, "guiding_room"), width=w, height=h, depth=16, samples=1)
+    scene = lambda_amd.Scene.load(lrd)
+    dev = lambda_amd.DeviceScene(scene)
+    t0 = time.perf_counter()
+    # the reference's streams are independent of the measured runs
+    ref_session = lambda_amd.Progressive(dev, None, seed=0x5EED0F2EF)
+    for _ in range(args.guided_ref_spp // 256):
+        ref_session.step(256, train=False)
+    ref = ref_session.image()
+    ref_session.close()
+    t_ref = time.perf_counter() - t0
+    log("guided leg: %dx%d reference at %d spp in %.1f s" % (w, h, args.guided_ref_spp, t_ref))
+    plain, plain_curve = time_to_target(dev, None, ref, 1e-3, args.guided_step, args.guided_max_spp)
+    log("guided leg: plain reaches relMSE 1e-3 at %s" % (plain,))
+    guide = lambda_amd.Guide(seed=args.guided_seed)
+    guided, guided_curve = time_to_target(dev, guide, ref, 1e-3, args.guided_step, args.guided_max_spp,
+                                          warmup=args.guided_warmup, train_steps=args.guided_train_steps,
+                                          epochs=args.guided_epochs, batch=args.guided_batch)
+    guide.close()
+    log("guided leg: guided reaches relMSE 1e-3 at %s" % (guided,))
+    dev.close()
+    out = {"workload": "guiding_room", "width": w, "height": h, "depth": 16,
+           "reference_spp": args.guided_ref_spp, "target_rel_mse": 1e-3, "step_spp": args.guided_step,
+           "guided_warmup_steps": args.guided_warmup, "guided_train_steps": min(args.guided_train_steps, 1 << 30),
+           "guided_epochs": args.guided_epochs, "guided_batch": args.guided_batch, "adam_lr": 1e-3,
+           "plain_s": plain[0] if plain else None, "plain_spp": plain[1] if plain else None,
+           "guided_s": guided[0] if guided else None, "guided_spp": guided[1] if guided else None,
+           "plain_curve": plain_curve[::max(1, len(plain_curve) // 12)],
+           "guided_curve": guided_curve[::max(1, len(guided_curve) // 12)]}
+    if plain and guided:
+        out["guided_over_plain_time"] = guided[0] / plain[0]
+    return out
+
+
 def copy_probe(nbytes: int) -> None:
     """One device-to-device hipMemcpy of nbytes through the HIP runtime the
     integrator itself uses (the runtime's copy-buffer kernel reads and writes
@@ -125,6 +209,16 @@ def main() -> int:
     ap.add_argument("--scene-dir", default=os.environ.get("LAMBDA_SCENE_DIR", "/tmp/lambda_bench"))
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain leg")
+    ap.add_argument("--guided-res", default="320x180")
+    ap.add_argument("--guided-ref-spp", type=int, default=65536)
+    ap.add_argument("--guided-step", type=int, default=64)
+    ap.add_argument("--guided-max-spp", type=int, default=65536)
+    ap.add_argument("--guided-seed", type=int, default=1)
+    ap.add_argument("--guided-warmup", type=int, default=48, help="guided steps that train without adding samples")
+    ap.add_argument("--guided-train-steps", type=int, default=0, help="kept guided steps that also train")
+    ap.add_argument("--guided-epochs", type=int, default=4, help="Adam epochs over each step's samples")
+    ap.add_argument("--guided-batch", type=int, default=4096)
     ap.add_argument("--throughput-steps", type=int, default=2,
                     help="also time this many renders with throughput streams (0 to skip)")
     ap.add_argument("--stats-pixels", type=int, default=20000, help="pixels sampled to count traversal work")
@@ -233,6 +327,8 @@ def main() -> int:
         cpu = cpu_baseline(scene, args.cpu_seconds, threads)
         log("CPU baseline %.4g paths/s (%s)" % (cpu["value"], cpu["sample"]))
 
+    guided = None if args.no_guided else guided_leg(args, args.scene_dir)
+
     cfg.update({"rng": args.rng, "path_slots": st["path_slots"], "setup_s": {
         "generate": round(t_gen, 2), "parse_and_bvh": round(t_parse, 2), "upload": round(t_upload, 2)}})
     out = {
@@ -257,6 +353,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "throughput_streams": thr,
+        "guided_vs_plain": guided,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         "kernels_ms": {k: st[k + "_ms"] for k in ("trace", "shade", "finalize", "persist")},
         "rays": {"closest": st["closest_rays"], "shadow": st["shadow_rays"], "iterations": st["iterations"]},

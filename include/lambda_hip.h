@@ -341,12 +341,21 @@ typedef struct lambda_progressive_stats {
  * it (keep it alive until the session is destroyed). */
 int lambda_hip_progressive_create(lambda_hip_scene *scene, const lambda_render_params *params, lambda_guide *guide,
                                   lambda_hip_progressive **out, char *err, size_t err_len);
-/* Renders `samples` (a perfect square) more samples per pixel; trains the
- * guide afterwards when train != 0. */
-int lambda_hip_progressive_step(lambda_hip_progressive *session, int32_t samples, int32_t train,
+/* Step flags. */
+#define LAMBDA_STEP_TRAIN 1    /* train the guide on this step's paths afterwards */
+#define LAMBDA_STEP_DISCARD 2  /* do not add this step's samples to the image */
+
+/* Renders `samples` (a perfect square) more samples per pixel.  Training
+ * samples are a uniform subsample (sized from the previous step) of the
+ * guided vertices whose continuation was traced. */
+int lambda_hip_progressive_step(lambda_hip_progressive *session, int32_t samples, int32_t flags,
                                 lambda_progressive_stats *stats, char *err, size_t err_len);
 /* out_rgb[W*H*3]: linear sRGB mean over all samples so far. */
 int lambda_hip_progressive_image(lambda_hip_progressive *session, float *out_rgb, char *err, size_t err_len);
+/* Training per step: `epochs` passes of Adam (rate lr) over the step's
+ * samples in batches of `batch` (defaults 1, 16384, 1e-3). */
+int lambda_hip_progressive_set_training(lambda_hip_progressive *session, int32_t epochs, int32_t batch, float lr,
+                                        char *err, size_t err_len);
 void lambda_hip_progressive_destroy(lambda_hip_progressive *session);
 
 /* Device name and compute-unit count of the current HIP device. */

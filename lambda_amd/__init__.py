@@ -285,23 +285,30 @@ class Progressive:
     """Progressive rendering session (lambda_hip_progressive_*): plain when
     guide is None, guided otherwise.  The scene and guide must outlive it."""
 
-    def __init__(self, dev: "DeviceScene", guide: "Guide | None" = None, path_slots: int = 0, **overrides):
+    def __init__(self, dev: "DeviceScene", guide: "Guide | None" = None, path_slots: int = 0, seed: int = 42,
+                 **overrides):
         self._lib = _abi.hip()
         self.dev, self.guide = dev, guide
-        p = dev.scene.render_params(path_slots=path_slots, **overrides)
+        p = dev.scene.render_params(path_slots=path_slots, seed=seed, **overrides)
         self.width, self.height = p.width, p.height
         self._h = C.c_void_p()
         err = C.create_string_buffer(4096)
         _check(self._lib.lambda_hip_progressive_create(dev._h, C.byref(p), guide._h if guide else None,
                                                        C.byref(self._h), err, 4096), err)
 
-    def step(self, samples: int = 1, train: bool = True) -> dict:
+    def step(self, samples: int = 1, train: bool = True, keep: bool = True) -> dict:
+        """Renders `samples` more samples per pixel; train: train the guide
+        on them afterwards; keep: add them to the image."""
         st = _abi.ProgressiveStats()
         err = C.create_string_buffer(4096)
-        _check(self._lib.lambda_hip_progressive_step(self._h, samples, 1 if train else 0, C.byref(st), err, 4096),
-               err)
+        flags = (1 if train else 0) | (0 if keep else 2)
+        _check(self._lib.lambda_hip_progressive_step(self._h, samples, flags, C.byref(st), err, 4096), err)
         return {"samples": st.samples, "train_samples": st.train_samples, "render_ms": st.render_ms,
                 "train_ms": st.train_ms, "loss": st.loss}
+
+    def set_training(self, epochs: int = 1, batch: int = 1 << 14, lr: float = 1e-3) -> None:
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_progressive_set_training(self._h, epochs, batch, lr, err, 4096), err)
 
     def image(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 3), np.float32)

@@ -105,32 +105,99 @@ __device__ __forceinline__ void layer_rows16(const GuideDev &G, int layer, const
 }
 
 // Inference: out[i][0..39] for feature rows feat[i][0..5], i < n (n read
-// from n_dev when given).  Each wave takes 16 rows, each block 64.
+// from n_dev when given).  Each wave takes 64 rows (4 tiles of 16) and
+// computes every layer transposed, H_next^T = W^T H^T: the weights are the
+// A operand (16 outputs x 32 inputs per fragment, read once and used for all
+// 4 row tiles) and the activations the B operand (row-major in LDS, 8
+// consecutive inputs of one row per lane), so that each lane receives 4
+// consecutive outputs of one row and stores them with one 8-byte write.
+constexpr int kFwdRowsPerWave = 64;
+constexpr int kFwdRowsPerBlock = kFwdRowsPerWave * kGuideWaves;  // 256
+
+__device__ __forceinline__ void fwd_layer(const GuideDev &G, int layer, const __bf16 *Hin, int ldin, __bf16 *Hout,
+                                          float *out) {
+    const int l = threadIdx.x & 63;
+    const int r16 = l & 15, q = l >> 4;
+    const int K = guide_layer_in(layer);
+    const __bf16 *Wt = G.w_nk + guide_wbf_offset(layer);  // [64 n][K]
+    const float *bias = G.params + guide_b_offset(layer);
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        f32x4 acc[4];
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++) acc[rt] = f32x4{0, 0, 0, 0};
+        for (int kk = 0; kk < K; kk += 32) {
+            // A = W^T[nt*16 + r16][kk + 8q .. +7]
+            const bf16x8 a = load8(Wt + (nt * 16 + r16) * K + kk + 8 * q);
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++) {
+                // B = H^T[kk + 8q + j][row = rt*16 + r16] = H[row][kk + 8q + j]
+                const bf16x8 b = load8(Hin + (rt * 16 + r16) * ldin + kk + 8 * q);
+                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[rt], 0, 0, 0);
+            }
+        }
+        // lane holds H_next[row = rt*16 + r16][n = nt*16 + 4q + i], i = 0..3
+        const int n0 = nt * 16 + 4 * q;
+        const float b0 = bias[n0], b1 = bias[n0 + 1], b2 = bias[n0 + 2], b3 = bias[n0 + 3];
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++) {
+            const int row = rt * 16 + r16;
+            const float v0 = acc[rt][0] + b0, v1 = acc[rt][1] + b1, v2 = acc[rt][2] + b2, v3 = acc[rt][3] + b3;
+            if (Hout) {
+                typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                bf16x4 h;
+                h[0] = __bf16(fmaxf(v0, 0.0f));
+                h[1] = __bf16(fmaxf(v1, 0.0f));
+                h[2] = __bf16(fmaxf(v2, 0.0f));
+                h[3] = __bf16(fmaxf(v3, 0.0f));
+                *reinterpret_cast<bf16x4 *>(Hout + row * kLdH + n0) = h;
+            } else {
+                *reinterpret_cast<float4 *>(out + row * 64 + n0) = make_float4(v0, v1, v2, v3);
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_guide_forward(GuideDev G, const float *feat, int fstride, int n,
                                                        const int *n_dev, float *out, int ostride) {
     if (n_dev) n = *n_dev;
-    if (int(blockIdx.x) * kGuideRowsPerBlock >= n) return;
-    __shared__ __attribute__((aligned(16))) __bf16 sX[kGuideRowsPerBlock * kLdX];
-    __shared__ __attribute__((aligned(16))) __bf16 sH[2][kGuideRowsPerBlock * kLdH];
-    __shared__ float sO[kGuideRowsPerBlock * 64];
+    if (int(blockIdx.x) * kFwdRowsPerBlock >= n) return;
+    // per wave: X [64][kLdX] bf16 (13 KB) reused as the fp32 output [64][64]
+    // (16 KB), and two hidden buffers [64][kLdH] bf16 (9 KB each)
+    __shared__ __attribute__((aligned(16))) unsigned char smem[kGuideWaves][16384 + 2 * kFwdRowsPerWave * kLdH * 2];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int wr = w * kGuideRowsPerWave;
-    const int row0 = blockIdx.x * kGuideRowsPerBlock + wr;
-    encode_rows(feat, fstride, row0, n, sX + wr * kLdX);
+    __bf16 *X = reinterpret_cast<__bf16 *>(smem[w]);
+    float *O = reinterpret_cast<float *>(smem[w]);
+    __bf16 *H0 = reinterpret_cast<__bf16 *>(smem[w] + 16384);
+    __bf16 *H1 = H0 + kFwdRowsPerWave * kLdH;
+    const int row0 = blockIdx.x * kFwdRowsPerBlock + w * kFwdRowsPerWave;
+    // encode 64 rows: lane e handles elements e, e + 64, ...
+    for (int e = l; e < kFwdRowsPerWave * kGuideIn; e += 64) {
+        const int r = e / kGuideIn, i = e % kGuideIn;
+        float v = 0;
+        if (row0 + r < n) {
+            GuideFeatures g;
+            const float *f = feat + size_t(row0 + r) * fstride;
+#pragma unroll
+            for (int s = 0; s < kGuideScalars; s++) g.f[s] = f[s];
+            v = guide_encode(g, i);
+        }
+        X[r * kLdX + i] = __bf16(v);
+    }
     __syncthreads();
-    layer_rows16(G, 0, sX + wr * kLdX, kLdX, sH[0] + wr * kLdH, nullptr);
+    fwd_layer(G, 0, X, kLdX, H0, nullptr);
     __syncthreads();
-    layer_rows16(G, 1, sH[0] + wr * kLdH, kLdH, sH[1] + wr * kLdH, nullptr);
+    fwd_layer(G, 1, H0, kLdH, H1, nullptr);
     __syncthreads();
-    layer_rows16(G, 2, sH[1] + wr * kLdH, kLdH, sH[0] + wr * kLdH, nullptr);
+    fwd_layer(G, 2, H1, kLdH, H0, nullptr);
     __syncthreads();
-    layer_rows16(G, 3, sH[0] + wr * kLdH, kLdH, sH[1] + wr * kLdH, nullptr);
+    fwd_layer(G, 3, H0, kLdH, H1, nullptr);
     __syncthreads();
-    layer_rows16(G, 4, sH[1] + wr * kLdH, kLdH, nullptr, sO + wr * 64);
+    fwd_layer(G, 4, H1, kLdH, nullptr, O);
     __syncthreads();
-    for (int e = l; e < kGuideRowsPerWave * kLobeOut; e += 64) {
+    for (int e = l; e < kFwdRowsPerWave * kLobeOut; e += 64) {
         const int r = e / kLobeOut, c = e % kLobeOut;
-        if (row0 + r < n) out[size_t(row0 + r) * ostride + c] = sO[(wr + r) * 64 + c];
+        if (row0 + r < n) out[size_t(row0 + r) * ostride + c] = O[r * 64 + c];
     }
 }
 
@@ -344,7 +411,7 @@ struct GuideNet {
 
     void forward(const float *feat, int fstride, int n, float *out, int ostride, hipStream_t st) const {
         if (n <= 0) return;
-        const int blocks = (n + kGuideRowsPerBlock - 1) / kGuideRowsPerBlock;
+        const int blocks = (n + kFwdRowsPerBlock - 1) / kFwdRowsPerBlock;
         k_guide_forward<<<blocks, 256, 0, st>>>(dev(), feat, fstride, n, nullptr, out, ostride);
         HIP_OK(hipGetLastError());
     }
@@ -352,7 +419,7 @@ struct GuideNet {
     // Inference for at most `bound` rows, the count read on the device.
     void forward_dev(const float *feat, const int *n_dev, int bound, float *out, hipStream_t st) const {
         if (bound <= 0) return;
-        const int blocks = (bound + kGuideRowsPerBlock - 1) / kGuideRowsPerBlock;
+        const int blocks = (bound + kFwdRowsPerBlock - 1) / kFwdRowsPerBlock;
         k_guide_forward<<<blocks, 256, 0, st>>>(dev(), feat, kGuideScalars, 0, n_dev, out, kLobeOut);
         HIP_OK(hipGetLastError());
     }

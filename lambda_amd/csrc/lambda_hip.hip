@@ -127,7 +127,8 @@ enum : int {
     kCtrDone = 5,        // samples ended by the last shade pass
     kCtrGuided = 6,      // paths queued for the guiding network this pass
     kCtrTrain = 7,       // training samples written since the last training
-    kCtrCount = 8
+    kCtrTrainSeen = 8,   // training candidates met (kept or not)
+    kCtrCount = 16
 };
 
 struct Paths {
@@ -158,6 +159,7 @@ struct Frame {
     int *cost;                // when set: per-pixel sum of path segments
     float *train;             // when set: training samples [train_cap][10]
     int train_cap;
+    float train_keep;         // probability of keeping a training candidate
     int grec_stride;          // guided vertices per slot (depth + 1)
     float3 box_lo, box_hi;    // scene box for the guiding features
 };
@@ -712,6 +714,17 @@ __global__ void __launch_bounds__(kBlock) k_shade_guided(DeviceScene S, Frame F,
     if (ended) P.done[pd] = s;
 }
 
+// 32-bit integer hash of three words (lowbias32 finaliser by C. Wellons).
+__device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t x = a * 0x9e3779b1u ^ (b + 0x7f4a7c15u) * 0x85ebca6bu ^ (c + 0x165667b1u) * 0xc2b2ae35u;
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
 // Adds the ended sample of slot s to its pixel's XYZ sum.  In reference
 // mode the pixel's next sample then starts in the same slot, continuing the
 // pixel's PCG32 stream (render/renderer.h:69-91); returns true when it did.
@@ -730,6 +743,12 @@ __device__ __forceinline__ bool retire_sample(const DeviceScene &S, const Frame 
             r[2].w = kRecNone;
             const float c = (Lend - r2.w) / r2.z * r2.y;
             if (!(c > 0) || !isfinite(c)) continue;
+            // a uniform subsample of the candidates (by a hash of slot,
+            // sample and vertex), so that the buffer does not fill with the
+            // paths that end first
+            atomicAdd(&P.ctr[kCtrTrainSeen], 1);
+            const uint32_t hsh = hash3(uint32_t(s), uint32_t(info.y) * 64u + uint32_t(b), uint32_t(F.seed));
+            if (float(hsh >> 8) * (1.0f / 16777216.0f) >= F.train_keep) continue;
             const int k = atomicAdd(&P.ctr[kCtrTrain], 1);
             if (k >= F.train_cap) continue;
             const float4 r0 = r[0], r1 = r[1];
@@ -1541,6 +1560,9 @@ struct lambda_hip_progressive {
     int steps = 0;
     int train_cap = 1 << 18;
     int batch = 1 << 14;
+    int epochs = 1;
+    double seen_per_sample = 0;  // training candidates per sample per pixel, last step
+    DevBuf<double> scratch_fb;   // framebuffer of steps that are not kept
 };
 
 namespace {
@@ -1558,6 +1580,9 @@ Frame progressive_frame(const lambda_hip_progressive &pr, int samples, int root)
         F.train = pr.train.p;
         F.train_cap = pr.train_cap;
         F.grec_stride = pr.params.depth + 1;
+        // expected candidates from the last step (first step: 4 per path)
+        const double expect = (pr.seen_per_sample > 0 ? pr.seen_per_sample : 4.0 * F.npix) * samples;
+        F.train_keep = float(std::min(1.0, 1.1 * pr.train_cap / std::max(1.0, expect)));
     }
     return F;
 }
@@ -1800,7 +1825,7 @@ int lambda_hip_progressive_create(lambda_hip_scene *scene, const lambda_render_p
     });
 }
 
-int lambda_hip_progressive_step(lambda_hip_progressive *pr, int32_t samples, int32_t train,
+int lambda_hip_progressive_step(lambda_hip_progressive *pr, int32_t samples, int32_t flags,
                                 lambda_progressive_stats *stats, char *err, size_t err_len) {
     return guarded(err, err_len, [&] {
         if (!pr || samples < 1) throw Failure{LAMBDA_EINVAL, "invalid arguments"};
@@ -1813,27 +1838,37 @@ int lambda_hip_progressive_step(lambda_hip_progressive *pr, int32_t samples, int
         EventPair ev;
         HIP_OK(hipEventCreate(&ev.a));
         HIP_OK(hipEventCreate(&ev.b));
+        const bool train = (flags & LAMBDA_STEP_TRAIN) != 0;
+        const bool keep = (flags & LAMBDA_STEP_DISCARD) == 0;
+        double *fb = R.P.fb;
+        if (!keep) {
+            if (!pr->scratch_fb.p) pr->scratch_fb.alloc(size_t(F.npix) * 3);
+            R.P.fb = pr->scratch_fb.p;
+        }
         HIP_OK(hipEventRecord(ev.a, R.st));
-        R.run(F, pr->net ? 0 : (1LL << 40), nullptr, nullptr, pr->steps == 0, pr->net, pr->net ? &pr->Q : nullptr);
+        R.run(F, pr->net ? 0 : (1LL << 40), nullptr, nullptr, !keep || pr->total == 0, pr->net,
+              pr->net ? &pr->Q : nullptr);
         HIP_OK(hipEventRecord(ev.b, R.st));
+        R.P.fb = fb;
         HIP_OK(hipEventSynchronize(ev.b));
         float render_ms = 0;
         HIP_OK(hipEventElapsedTime(&render_ms, ev.a, ev.b));
         const long long collected = pr->net ? std::min<long long>(R.host_ctr[kCtrTrain], pr->train_cap) : 0;
+        if (pr->net) pr->seen_per_sample = double(R.host_ctr[kCtrTrainSeen]) / samples;
         float train_ms = 0;
         double loss = 0;
         if (pr->net && train && collected >= pr->batch) {
             GuideNet &net = *pr->net;
             HIP_OK(hipMemsetAsync(net.loss.p, 0, sizeof(double), R.st));
             HIP_OK(hipEventRecord(ev.a, R.st));
-            net.train(pr->train.p, int(collected), pr->batch, R.st);
+            for (int e = 0; e < pr->epochs; e++) net.train(pr->train.p, int(collected), pr->batch, R.st);
             HIP_OK(hipEventRecord(ev.b, R.st));
             HIP_OK(hipEventSynchronize(ev.b));
             HIP_OK(hipEventElapsedTime(&train_ms, ev.a, ev.b));
             HIP_OK(hipMemcpy(&loss, net.loss.p, sizeof(double), hipMemcpyDeviceToHost));
-            loss /= double(collected / pr->batch * pr->batch);
+            loss /= double(collected / pr->batch * pr->batch) * pr->epochs;
         }
-        pr->total += samples;
+        if (keep) pr->total += samples;
         pr->steps++;
         if (stats) {
             stats->samples = pr->total;
@@ -1856,6 +1891,17 @@ int lambda_hip_progressive_image(lambda_hip_progressive *pr, float *out_rgb, cha
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(out_rgb, R.B.rgb.p, size_t(F.npix) * 3 * sizeof(float), hipMemcpyDeviceToHost, R.st));
         HIP_OK(hipStreamSynchronize(R.st));
+    });
+}
+
+int lambda_hip_progressive_set_training(lambda_hip_progressive *pr, int32_t epochs, int32_t batch, float lr,
+                                        char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!pr || epochs < 1 || batch < kGuideRowsPerBlock || batch > pr->train_cap || !(lr > 0))
+            throw Failure{LAMBDA_EINVAL, "invalid training settings"};
+        pr->epochs = epochs;
+        pr->batch = batch;
+        if (pr->net) pr->net->lr = lr;
     });
 }
 
