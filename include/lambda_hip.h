@@ -317,6 +317,40 @@ int lambda_hip_guide_train(lambda_guide *guide, const float *samples, int32_t n,
                            double *loss, char *err, size_t err_len);
 
 /*
+ * Batched serving (no reference counterpart: the reference renders one
+ * scene per process).  Many render jobs share one persistent launch: each
+ * job is a scene and render settings with reference streams (its image is
+ * bit-identical to lambda_hip_scene_render of the same settings), admitted
+ * at its arrival time (milliseconds after the first batch starts, on the
+ * device clock).  Jobs are admitted in arrival order while their buffers
+ * (36 bytes per pixel) fit hbm_budget (0 = no limit); the rest form later
+ * launches.  Latency = the time the job's last pixel finished - arrival.
+ */
+typedef struct lambda_job {
+    int32_t scene;                  /* index into the scenes array */
+    lambda_render_params params;    /* rng_mode is ignored (reference streams) */
+    double arrival_ms;
+    float *out_rgb;                 /* W*H*3 linear sRGB, written on return */
+} lambda_job;
+
+typedef struct lambda_job_result {
+    double latency_ms;
+    double finish_ms;               /* after the first batch started */
+    int32_t batch;                  /* launch that served the job */
+} lambda_job_result;
+
+typedef struct lambda_serve_stats {
+    int32_t batches;
+    int64_t paths;                  /* sum over jobs of W*H*spp */
+    int64_t closest_rays, shadow_rays;
+    double wall_ms;                 /* first batch start to the last result */
+} lambda_serve_stats;
+
+int lambda_hip_serve(lambda_hip_scene *const *scenes, int32_t num_scenes, const lambda_job *jobs, int32_t num_jobs,
+                     int64_t hbm_budget, lambda_job_result *results, lambda_serve_stats *stats, char *err,
+                     size_t err_len);
+
+/*
  * Progressive rendering (no reference counterpart): a session accumulates
  * samples over steps into one FP64 framebuffer, with throughput streams
  * (fresh PCG32 streams per step).  With a guide, Lambertian vertices sample

@@ -345,3 +345,33 @@ def rel_mse(img: np.ndarray, ref: np.ndarray) -> float:
     a = img[m].astype(np.float64)
     r = ref[m].astype(np.float64)
     return float(np.mean((a - r) ** 2 / (r * r + 0.01)))
+
+
+def serve(devices: "list[DeviceScene]", jobs: "list[dict]", hbm_budget: int = 0):
+    """Renders many jobs in batched persistent launches (lambda_hip_serve).
+    jobs: dicts {scene: index into devices, arrival_ms, and render-parameter
+    overrides (width, height, depth, samples, sqrt_samples, lambda_min,
+    lambda_max, seed)}.  Returns (images, results, stats)."""
+    lib = _abi.hip()
+    handles = (C.c_void_p * len(devices))(*[d._h for d in devices])
+    arr = (_abi.Job * len(jobs))()
+    images = []
+    for k, jd in enumerate(jobs):
+        jd = dict(jd)
+        sc = jd.pop("scene")
+        arrival = float(jd.pop("arrival_ms", 0.0))
+        p = devices[sc].scene.render_params(**jd)
+        img = np.zeros((p.height, p.width, 3), np.float32)
+        images.append(img)
+        arr[k].scene = sc
+        arr[k].params = p
+        arr[k].arrival_ms = arrival
+        arr[k].out_rgb = _abi.ptr(img, C.c_float)
+    res = (_abi.JobResult * len(jobs))()
+    st = _abi.ServeStats()
+    err = C.create_string_buffer(4096)
+    _check(lib.lambda_hip_serve(handles, len(devices), arr, len(jobs), hbm_budget, res, C.byref(st), err, 4096), err)
+    results = [{"latency_ms": r.latency_ms, "finish_ms": r.finish_ms, "batch": r.batch} for r in res]
+    stats = {"batches": st.batches, "paths": st.paths, "closest_rays": st.closest_rays,
+             "shadow_rays": st.shadow_rays, "wall_ms": st.wall_ms}
+    return images, results, stats

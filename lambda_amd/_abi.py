@@ -67,6 +67,20 @@ class RenderStats(C.Structure):
     ]
 
 
+class Job(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("params", RenderParams), ("arrival_ms", C.c_double),
+                ("out_rgb", C.POINTER(C.c_float))]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("latency_ms", C.c_double), ("finish_ms", C.c_double), ("batch", C.c_int32)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("batches", C.c_int32), ("paths", C.c_int64), ("closest_rays", C.c_int64),
+                ("shadow_rays", C.c_int64), ("wall_ms", C.c_double)]
+
+
 class ProgressiveStats(C.Structure):
     _fields_ = [
         ("samples", C.c_int64), ("train_samples", C.c_int64), ("render_ms", C.c_double),
@@ -144,6 +158,8 @@ def hip():
         lib.lambda_hip_guide_gradient.argtypes = [vp, _f32p, C.c_int32, _f32p, f64p, C.c_char_p, C.c_size_t]
         lib.lambda_hip_guide_train.argtypes = [vp, _f32p, C.c_int32, C.c_int32, C.c_float, f64p, C.c_char_p,
                                                C.c_size_t]
+        lib.lambda_hip_serve.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Job), C.c_int32, C.c_int64,
+                                         C.POINTER(JobResult), C.POINTER(ServeStats), C.c_char_p, C.c_size_t]
         lib.lambda_hip_progressive_create.argtypes = [vp, rp, vp, C.POINTER(vp), C.c_char_p, C.c_size_t]
         lib.lambda_hip_progressive_step.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(ProgressiveStats),
                                                     C.c_char_p, C.c_size_t]

@@ -855,6 +855,102 @@ __global__ void __launch_bounds__(kBlock, kPersistWaves) k_persist(DeviceScene S
     atomicAdd(&P.rays[1], (unsigned long long)n_shadow);
 }
 
+// ------------------------------------------------------ batched serving
+//
+// One persistent launch renders a batch of jobs.  A job is a scene and a
+// frame rendered with reference streams; its units are its pixels, numbered
+// consecutively across the batch in arrival order.  Threads claim units
+// from one counter; a thread that claims a unit of a job that has not
+// arrived yet sleeps until the job's arrival time on the device wall clock.
+// Each unit runs all samples of its pixel, so every job's image equals the
+// image of a single render of the same frame bit for bit.  The thread that
+// finishes a job's last pixel records the finishing time.
+struct ServeJob {
+    Frame F;                        // reference streams; fb below
+    double *fb;                     // XYZ sums of the job, [pixel][3]
+    long long first_unit;           // first unit of the job in the batch
+    unsigned long long arrival;     // wall-clock ticks after the batch start
+    int scene;                      // index into the scene table
+    int units;                      // pixels
+};
+
+struct ServeCtl {
+    unsigned long long t0;          // wall clock at the batch start
+    int next_unit;                  // unit counter
+    int pad;
+};
+
+__global__ void k_serve_clock(ServeCtl *ctl) {
+    ctl->t0 = wall_clock64();
+    ctl->next_unit = 0;
+}
+
+__global__ void __launch_bounds__(kBlock, kPersistWaves) k_serve(const DeviceScene *scenes, const ServeJob *jobs,
+                                                                  int njobs, long long total_units, Paths P,
+                                                                  ServeCtl *ctl, int *done, unsigned long long *finish,
+                                                                  int2 *spill) {
+    __shared__ int s_code[kRing * kBlock];
+    __shared__ float s_dist[kRing * kBlock];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack stk = make_stack(s_code, s_dist, spill);
+    const unsigned long long t0 = ctl->t0;
+    unsigned n_closest = 0, n_shadow = 0;
+    int j = 0;
+    while (true) {
+        const int w = atomicAdd(&ctl->next_unit, 1);
+        if (w >= total_units) break;
+        // claims only grow, so the job index only moves forward
+        while (j + 1 < njobs && w >= jobs[j + 1].first_unit) j++;
+        const ServeJob &J = jobs[j];
+        const unsigned long long due = t0 + J.arrival;
+        while (wall_clock64() < due) __builtin_amdgcn_s_sleep(127);
+        const DeviceScene &S = scenes[J.scene];
+        const Frame &F = J.F;
+        Paths Pj = P;
+        Pj.fb = J.fb;
+        const int pixel = int(w - J.first_unit);
+        {
+            Pcg rng = pcg_seed(F.seed, uint64_t(pixel));
+            camera_sample(S, F, Pj, s, pixel, 0, rng);
+            store_rng(Pj, s, rng);
+        }
+        while (true) {
+            n_closest++;
+            {
+                float4 o = Pj.o[s], d = Pj.d[s];
+                float best = kMax;
+                int hi = -1, ho = -1;
+                stk.sp = stk.lo = 0;
+                trace_ray<false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk);
+                Pj.hit[s] = make_int2(hi, ho);
+            }
+            bool push = false;
+            ShadowRay sh;
+            const bool ended = shade_slot(S, F, Pj, s, push, sh);
+            if (push) {
+                n_shadow++;
+                float best = sh.o.w;
+                int hi = -1, ho = -1;
+                stk.sp = stk.lo = 0;
+                if (!trace_ray<true>(S, v3(sh.o), v3(sh.d), best, hi, ho, sh.i.y, sh.i.z, stk)) {
+                    float4 L = Pj.L[s];
+                    L.x += sh.c.x;
+                    L.y += sh.c.y;
+                    L.z += sh.c.z;
+                    L.w += sh.c.w;
+                    Pj.L[s] = L;
+                }
+            }
+            if (!ended) continue;
+            if (retire_sample(S, F, Pj, s)) continue;
+            break;
+        }
+        if (atomicAdd(&done[j], 1) == J.units - 1) finish[j] = wall_clock64();
+    }
+    atomicAdd(&P.rays[0], (unsigned long long)n_closest);
+    atomicAdd(&P.rays[1], (unsigned long long)n_shadow);
+}
+
 // Mean XYZ over samples to linear sRGB (core/utils.h:230), in double.
 __global__ void k_resolve(Frame F, const double *fb, float *rgb) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1790,6 +1886,162 @@ int lambda_hip_guide_train(lambda_guide *guide, const float *samples, int32_t n,
         net.train(sm.p, n, batch, nullptr);
         HIP_OK(hipDeviceSynchronize());
         if (loss) HIP_OK(hipMemcpy(loss, net.loss.p, sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int lambda_hip_serve(lambda_hip_scene *const *scenes, int32_t num_scenes, const lambda_job *jobs, int32_t num_jobs,
+                     int64_t hbm_budget, lambda_job_result *results, lambda_serve_stats *stats, char *err,
+                     size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scenes || num_scenes < 1 || num_jobs < 0 || (num_jobs > 0 && (!jobs || !results)))
+            throw Failure{LAMBDA_EINVAL, "invalid arguments"};
+        check_device();
+        const auto wall0 = std::chrono::steady_clock::now();
+        std::vector<DeviceScene> table(static_cast<size_t>(num_scenes));
+        int need = 0;
+        for (int k = 0; k < num_scenes; k++) {
+            if (!scenes[k]) throw Failure{LAMBDA_EINVAL, "null scene"};
+            table[size_t(k)] = scenes[k]->ds;
+            need = std::max(need, scenes[k]->stack_need);
+        }
+        // jobs in arrival order
+        std::vector<int> order(static_cast<size_t>(num_jobs));
+        for (int i = 0; i < num_jobs; i++) {
+            const lambda_job &jb = jobs[i];
+            if (jb.scene < 0 || jb.scene >= num_scenes) throw Failure{LAMBDA_EINVAL, "job scene out of range"};
+            if (!jb.out_rgb) throw Failure{LAMBDA_EINVAL, "job without output buffer"};
+            if (!(jb.arrival_ms >= 0 && jb.arrival_ms <= 600000)) throw Failure{LAMBDA_EINVAL, "job arrival out of range"};
+            lambda_render_params p = jb.params;
+            p.rng_mode = LAMBDA_RNG_REFERENCE;
+            validate_params(p);
+            order[size_t(i)] = i;
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return jobs[a].arrival_ms < jobs[b].arrival_ms; });
+        int dev = 0, rate_khz = 0;
+        HIP_OK(hipGetDevice(&dev));
+        HIP_OK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+        const double ticks_per_ms = double(std::max(rate_khz, 1));
+        DevBuf<DeviceScene> dscenes;
+        dscenes.upload(table);
+        const int threads = int(resident_persist_threads());
+        RenderBuffers B;
+        B.o.alloc(size_t(threads)); B.d.alloc(size_t(threads)); B.lam.alloc(size_t(threads));
+        B.beta.alloc(size_t(threads)); B.L.alloc(size_t(threads)); B.rng.alloc(size_t(threads));
+        B.info.alloc(size_t(threads)); B.hit.alloc(size_t(threads));
+        B.ctr.alloc(kCtrCount);
+        B.rays.alloc(2);
+        if (need > kRing) B.spill.alloc(size_t(need - kRing) * size_t(threads));
+        Paths P{};
+        P.o = B.o.p; P.d = B.d.p; P.lam = B.lam.p; P.beta = B.beta.p; P.L = B.L.p;
+        P.rng = B.rng.p; P.info = B.info.p; P.hit = B.hit.p; P.ctr = B.ctr.p; P.rays = B.rays.p;
+        DevBuf<ServeCtl> ctl;
+        ctl.alloc(1);
+        hipStream_t st = nullptr;
+        HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        struct StreamFree {
+            hipStream_t s;
+            ~StreamFree() { if (s) (void)hipStreamDestroy(s); }
+        } sfree{st};
+        k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
+        k_fill_d<<<1, kBlock, 0, st>>>(reinterpret_cast<double *>(P.rays), 2);
+        int batches = 0;
+        double first_start_ms = -1;
+        size_t next = 0;
+        while (next < order.size()) {
+            // admission: jobs in arrival order while their buffers fit the budget
+            std::vector<int> batch;
+            long long bytes = 0;
+            while (next < order.size()) {
+                const lambda_job &jb = jobs[order[next]];
+                const long long px = (long long)jb.params.width * jb.params.height;
+                const long long jbytes = px * (3 * 8 + 3 * 4);
+                if (!batch.empty() && hbm_budget > 0 && bytes + jbytes > hbm_budget) break;
+                bytes += jbytes;
+                batch.push_back(order[next++]);
+            }
+            const double now_ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+            if (first_start_ms < 0) first_start_ms = now_ms;
+            const double batch_start = now_ms - first_start_ms;  // relative to the first batch
+            std::vector<ServeJob> sj(batch.size());
+            long long units = 0, px_total = 0;
+            for (size_t k = 0; k < batch.size(); k++) px_total += (long long)jobs[batch[k]].params.width * jobs[batch[k]].params.height;
+            DevBuf<double> fb;
+            DevBuf<float> rgb;
+            fb.alloc(size_t(px_total) * 3);
+            rgb.alloc(size_t(px_total) * 3);
+            DevBuf<int> done;
+            DevBuf<unsigned long long> finish;
+            done.alloc(batch.size());
+            finish.alloc(batch.size());
+            k_fill_d<<<grid_for(px_total * 3), kBlock, 0, st>>>(fb.p, size_t(px_total) * 3);
+            k_fill_i<<<grid_for((long long)batch.size()), kBlock, 0, st>>>(done.p, int(batch.size()));
+            long long off = 0;
+            for (size_t k = 0; k < batch.size(); k++) {
+                const lambda_job &jb = jobs[batch[k]];
+                lambda_render_params p = jb.params;
+                p.rng_mode = LAMBDA_RNG_REFERENCE;
+                ServeJob &J = sj[k];
+                J.F = make_frame(p);
+                J.fb = fb.p + off * 3;
+                J.first_unit = units;
+                J.arrival = (unsigned long long)(std::max(0.0, jb.arrival_ms - batch_start) * ticks_per_ms);
+                J.scene = jb.scene;
+                J.units = J.F.npix;
+                units += J.units;
+                off += J.F.npix;
+            }
+            if (units >= (1LL << 31) - (1LL << 28)) throw Failure{LAMBDA_EINVAL, "batch too large"};
+            DevBuf<ServeJob> djobs;
+            djobs.upload(sj);
+            k_serve_clock<<<1, 1, 0, st>>>(ctl.p);
+            k_serve<<<threads / kBlock, kBlock, 0, st>>>(dscenes.p, djobs.p, int(sj.size()), units, P, ctl.p, done.p,
+                                                        finish.p, B.spill.p);
+            HIP_OK(hipGetLastError());
+            off = 0;
+            for (size_t k = 0; k < batch.size(); k++) {
+                const ServeJob &J = sj[k];
+                k_resolve<<<grid_for(J.F.npix), kBlock, 0, st>>>(J.F, J.fb, rgb.p + off * 3);
+                off += J.F.npix;
+            }
+            std::vector<float> host_rgb(size_t(px_total) * 3);
+            std::vector<unsigned long long> fin(batch.size());
+            ServeCtl hc;
+            HIP_OK(hipMemcpyAsync(host_rgb.data(), rgb.p, host_rgb.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipMemcpyAsync(fin.data(), finish.p, fin.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipMemcpyAsync(&hc, ctl.p, sizeof hc, hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            int herr = 0;
+            HIP_OK(hipMemcpy(&herr, P.ctr + kCtrError, sizeof(int), hipMemcpyDeviceToHost));
+            if (herr) throw Failure{LAMBDA_EHIP, "device reported an inconsistent hit record"};
+            off = 0;
+            for (size_t k = 0; k < batch.size(); k++) {
+                const lambda_job &jb = jobs[batch[k]];
+                const ServeJob &J = sj[k];
+                std::memcpy(jb.out_rgb, host_rgb.data() + off * 3, size_t(J.F.npix) * 3 * sizeof(float));
+                off += J.F.npix;
+                lambda_job_result &r = results[batch[k]];
+                const double fin_ms = double(fin[k] - hc.t0) / ticks_per_ms;  // after this batch's start
+                r.finish_ms = batch_start + fin_ms;
+                r.latency_ms = r.finish_ms - jb.arrival_ms;
+                r.batch = batches;
+            }
+            batches++;
+        }
+        if (stats) {
+            unsigned long long r[2] = {0, 0};
+            HIP_OK(hipMemcpy(r, P.rays, sizeof r, hipMemcpyDeviceToHost));
+            stats->batches = batches;
+            stats->paths = 0;
+            for (int i = 0; i < num_jobs; i++)
+                stats->paths += (long long)jobs[i].params.width * jobs[i].params.height * jobs[i].params.samples;
+            stats->closest_rays = (long long)r[0];
+            stats->shadow_rays = (long long)r[1];
+            stats->wall_ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count() -
+                std::max(0.0, first_start_ms);
+        }
     });
 }
 
