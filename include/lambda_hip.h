@@ -187,6 +187,10 @@ typedef struct lambda_render_params {
     int32_t cost_order;    /* start pixels in decreasing order of a one-sample
                               cost estimate (reference streams; the image does
                               not change): 0 = automatic, 1 = on, -1 = off */
+    int32_t graph_passes;  /* > 0: one launch per bounce throughout, the bounce
+                              loop captured in a hipGraph of this many passes
+                              (rounded up to even) and replayed until done;
+                              0 = no graph */
 } lambda_render_params;
 
 /*

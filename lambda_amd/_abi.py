@@ -47,7 +47,7 @@ class RenderParams(C.Structure):
         ("samples", C.c_int32), ("sqrt_samples", C.c_int32),
         ("lambda_min", C.c_float), ("lambda_max", C.c_float),
         ("seed", C.c_uint64), ("rng_mode", C.c_int32), ("path_slots", C.c_int32),
-        ("tail_paths", C.c_int32), ("cost_order", C.c_int32),
+        ("tail_paths", C.c_int32), ("cost_order", C.c_int32), ("graph_passes", C.c_int32),
     ]
 
 

@@ -274,10 +274,22 @@ __global__ void __launch_bounds__(kBlock) k_start(DeviceScene S, Frame F, Paths 
 // threads [n_closest, n_closest + n_shadow) test queued occlusion ray
 // t - n_closest and, if the light is visible, add its contribution to the
 // path's radiance.
+// Queue lengths of a pass taken on the device (graph-captured passes):
+// {closest, shadow, done}; a kernel given counts reads its lengths there.
+__global__ void k_pass_begin(const int *ctr, int *counts) {
+    counts[0] = ctr[kCtrNext];
+    counts[1] = ctr[kCtrShadow];
+    counts[2] = ctr[kCtrDone];
+}
+
 __global__ void __launch_bounds__(kBlock) k_trace(DeviceScene S, Paths P, int n_closest, int n_shadow,
-                                                  int2 *spill) {
+                                                  int2 *spill, const int *counts) {
     __shared__ int s_code[kRing * kBlock];
     __shared__ float s_dist[kRing * kBlock];
+    if (counts) {
+        n_closest = counts[0];
+        n_shadow = counts[1];
+    }
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) P.ctr[kCtrNext] = 0;
     Stack stk = make_stack(s_code, s_dist, spill);
@@ -558,7 +570,8 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
 }
 
 // Shading of every queued path.
-__global__ void __launch_bounds__(kBlock) k_shade(DeviceScene S, Frame F, Paths P, int count) {
+__global__ void __launch_bounds__(kBlock) k_shade(DeviceScene S, Frame F, Paths P, int count, const int *counts) {
+    if (counts) count = counts[0];
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     bool active = q < count;
     bool push_shadow = false;
@@ -776,7 +789,9 @@ __device__ __forceinline__ bool retire_sample(const DeviceScene &S, const Frame 
 // Retires the samples that ended in the previous shade pass (their
 // occlusion rays have been traced since): radiance to the framebuffer, then
 // the slot's next sample, which joins the closest-hit queue.
-__global__ void __launch_bounds__(kBlock) k_finalize(DeviceScene S, Frame F, Paths P, int n_done) {
+__global__ void __launch_bounds__(kBlock) k_finalize(DeviceScene S, Frame F, Paths P, int n_done,
+                                                     const int *counts) {
+    if (counts) n_done = counts[2];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q == 0) {
         P.ctr[kCtrShadow] = 0;
@@ -1505,9 +1520,9 @@ struct Renderer {
                 // settle pending occlusion rays and ended samples, then run
                 // every live slot, and the units still to start, to the end
                 if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
-                k_trace<<<grid_for(n_shadow), kBlock, 0, st>>>(g.ds, P, 0, n_shadow, B.spill.p);
+                k_trace<<<grid_for(n_shadow), kBlock, 0, st>>>(g.ds, P, 0, n_shadow, B.spill.p, nullptr);
                 if (stats) HIP_OK(hipEventRecord(ev[0].b, st));
-                k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done);
+                k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done, nullptr);
                 if (stats) HIP_OK(hipEventRecord(ev[3].a, st));
                 k_persist<<<grid_for((long long)n_closest + n_done), kBlock, 0, st>>>(g.ds, F, P, n_closest, B.spill.p);
                 if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
@@ -1523,12 +1538,12 @@ struct Renderer {
                 break;
             }
             if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
-            k_trace<<<grid_for((long long)n_closest + n_shadow), kBlock, 0, st>>>(g.ds, P, n_closest, n_shadow, B.spill.p);
+            k_trace<<<grid_for((long long)n_closest + n_shadow), kBlock, 0, st>>>(g.ds, P, n_closest, n_shadow, B.spill.p, nullptr);
             if (stats) {
                 HIP_OK(hipEventRecord(ev[0].b, st));
                 HIP_OK(hipEventRecord(ev[2].a, st));
             }
-            k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done);
+            k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done, nullptr);
             if (stats) {
                 HIP_OK(hipEventRecord(ev[2].b, st));
                 HIP_OK(hipEventRecord(ev[1].a, st));
@@ -1538,7 +1553,7 @@ struct Renderer {
                 guide->forward_dev(Q->feat, P.ctr + kCtrGuided, n_closest, Q->out, st);
                 k_shade_guided<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, *Q);
             } else {
-                k_shade<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, n_closest);
+                k_shade<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, n_closest, nullptr);
             }
             if (stats) HIP_OK(hipEventRecord(ev[1].b, st));
             sync_counters();
@@ -1557,6 +1572,62 @@ struct Renderer {
                 if (F.mode != LAMBDA_RNG_REFERENCE) done = done * F.npix / std::max<long long>(1, units);
                 *progress = int32_t(std::min<long long>(done, F.npix));
             }
+        }
+    }
+
+    // The pass schedule with the bounce loop captured in a hipGraph: a graph
+    // of `passes` (even) passes -- each k_pass_begin, k_trace, k_finalize,
+    // k_shade with queue lengths read on the device and grids sized for the
+    // slot count -- is replayed until the queues are empty; the host reads
+    // the counters once per replay.  The graph holds kernels only.
+    void run_graph(const Frame &F, int passes, volatile int32_t *progress, lambda_render_stats *stats) {
+        const long long units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
+        passes = std::max(2, passes + (passes & 1));
+        DevBuf<int> counts;
+        counts.alloc(3);
+        k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
+        k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
+        k_fill_d<<<1, kBlock, 0, st>>>(reinterpret_cast<double *>(P.rays), 2);
+        k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns);
+        k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrWork, ns);
+        k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrNext, ns);  // every slot starts in the closest-hit queue
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(st));
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        struct GraphFree {
+            hipGraph_t *g;
+            hipGraphExec_t *e;
+            ~GraphFree() {
+                if (*e) (void)hipGraphExecDestroy(*e);
+                if (*g) (void)hipGraphDestroy(*g);
+            }
+        } gf{&graph, &exec};
+        HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        Paths Q = P;
+        for (int k = 0; k < passes; k++) {
+            k_pass_begin<<<1, 1, 0, st>>>(Q.ctr, counts.p);
+            k_trace<<<grid_for(2LL * ns), kBlock, 0, st>>>(g.ds, Q, 0, 0, B.spill.p, counts.p);
+            k_finalize<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, Q, 0, counts.p);
+            k_shade<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, Q, 0, counts.p);
+            std::swap(Q.q_in, Q.q_out);
+        }
+        HIP_OK(hipStreamEndCapture(st, &graph));
+        HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        const long long per_slot = F.mode == LAMBDA_RNG_REFERENCE ? F.spp * ((units + ns - 1) / ns + 1)
+                                                                  : (F.total_work + ns - 1) / ns + 1;
+        const long long max_replays = ((long long)(F.depth + 2) * per_slot + 16) / passes + 2;
+        for (long long r = 0;; r++) {
+            if (r > max_replays) throw Failure{LAMBDA_EHIP, "bounce loop did not drain"};
+            HIP_OK(hipGraphLaunch(exec, st));
+            sync_counters();
+            if (stats) stats->iterations += passes;
+            if (progress) {
+                long long done = host_ctr[kCtrPixels];
+                if (F.mode != LAMBDA_RNG_REFERENCE) done = done * F.npix / std::max<long long>(1, units);
+                *progress = int32_t(std::min<long long>(done, F.npix));
+            }
+            if (host_ctr[kCtrNext] == 0 && host_ctr[kCtrShadow] == 0 && host_ctr[kCtrDone] == 0) break;
         }
     }
 
@@ -1627,7 +1698,8 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
     }
     const bool by_cost = p.cost_order > 0 || (p.cost_order == 0 && F.mode == LAMBDA_RNG_REFERENCE && R.ns < F.npix);
     if (by_cost) R.order_by_cost(F);
-    R.run(F, tail, progress, stats);
+    if (p.graph_passes > 0) R.run_graph(F, p.graph_passes, progress, stats);
+    else R.run(F, tail, progress, stats);
     k_resolve<<<grid_for(F.npix), kBlock, 0, R.st>>>(F, R.P.fb, R.B.rgb.p);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(out_rgb, R.B.rgb.p, size_t(F.npix) * 3 * sizeof(float), hipMemcpyDeviceToHost, R.st));

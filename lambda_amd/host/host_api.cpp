@@ -241,6 +241,7 @@ int lambda_scene_get_params(const lambda_scene *scene, lambda_render_params *p) 
     p->path_slots = 0;
     p->tail_paths = 0;
     p->cost_order = 0;
+    p->graph_passes = 0;
     return LAMBDA_OK;
 }
 

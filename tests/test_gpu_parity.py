@@ -227,3 +227,20 @@ def test_fewer_slots_than_pixels_reference_mode(devices):
     for slots, tail, order in ((1000, -1, -1), (1000, 0, 0), (777, 1 << 30, -1), (777, 1 << 30, 1), (1000, -1, 1)):
         b = devices["mesh"].render(path_slots=slots, tail_paths=tail, cost_order=order)
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (slots, tail, order)
+
+
+@pytest.mark.parametrize("name", ["box", "mesh"])
+def test_graph_captured_bounce_loop_matches_eager_passes(name, devices):
+    """The bounce loop captured in a hipGraph and replayed (host syncs
+    between replays, eager renders between graph renders) gives the image of
+    the eagerly launched passes bit for bit (reference streams)."""
+    eager = devices[name].render(tail_paths=-1, cost_order=-1)
+    for passes in (2, 6):
+        got = devices[name].render(graph_passes=passes, cost_order=-1)
+        assert np.array_equal(got.view(np.uint32), eager.view(np.uint32)), passes
+        again = devices[name].render(tail_paths=-1, cost_order=-1)
+        assert np.array_equal(again.view(np.uint32), eager.view(np.uint32))
+    prog = np.zeros(1, np.int32)
+    devices[name].render(graph_passes=4, progress=prog)
+    p = devices[name].scene.params
+    assert prog[0] == p.width * p.height
