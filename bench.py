@@ -174,6 +174,38 @@ def guided_leg(args, scene_dir: str) -> dict:
     return out
 
 
+def serve_leg(args, mesh_dev, scene_dir: str) -> dict:
+    """32 seeded jobs, random variants of the closed box, the thin-film grid
+    and the instanced mesh (side 128-1024 pixels, depth 4-16, 16-256 spp in
+    perfect squares, 380-780 or 360-830 nm), Poisson arrivals with a mean
+    gap of 50 ms, served in persistent launches admitted by HBM budget."""
+    import numpy as np
+    import lambda_amd
+    from lambda_amd import scenes
+    box = lambda_amd.Scene.load(scenes.write_closed_box(os.path.join(scene_dir, "closed_box")))
+    film = lambda_amd.Scene.load(scenes.write_thinfilm_grid(os.path.join(scene_dir, "thinfilm_grid")))
+    devs = [lambda_amd.DeviceScene(box), lambda_amd.DeviceScene(film), mesh_dev]
+    rng = np.random.default_rng(args.serve_seed)
+    t, jobs = 0.0, []
+    for _ in range(args.serve_jobs):
+        t += rng.exponential(args.serve_gap_ms)
+        side = int(rng.integers(128, 1025))
+        root = int(rng.integers(4, 17))
+        lo, hi = (380.0, 780.0) if rng.random() < 0.5 else (360.0, 830.0)
+        jobs.append(dict(scene=int(rng.integers(0, 3)), width=side, height=side, depth=int(rng.integers(4, 17)),
+                         samples=root * root, sqrt_samples=root, lambda_min=lo, lambda_max=hi, arrival_ms=t))
+    _, results, st = lambda_amd.serve(devs, jobs, hbm_budget=args.serve_budget)
+    for d in devs[:2]:
+        d.close()
+    lat = np.array([r["latency_ms"] for r in results])
+    return {"jobs": len(jobs), "batches": st["batches"], "paths": st["paths"], "wall_ms": st["wall_ms"],
+            "paths_per_s": st["paths"] / (st["wall_ms"] / 1e3),
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "max": float(lat.max())},
+            "last_arrival_ms": jobs[-1]["arrival_ms"], "hbm_budget": args.serve_budget,
+            "seed": args.serve_seed}
+
+
 def copy_probe(nbytes: int) -> None:
     """One device-to-device hipMemcpy of nbytes through the HIP runtime the
     integrator itself uses (the runtime's copy-buffer kernel reads and writes
@@ -204,12 +236,19 @@ def main() -> int:
     ap.add_argument("--rng", default="reference", choices=["reference", "throughput"])
     ap.add_argument("--path-slots", type=int, default=0)
     ap.add_argument("--cost-order", type=int, default=0, help="start pixels longest first (0 automatic, 1 on, -1 off)")
+    ap.add_argument("--graph-passes", type=int, default=0, help="capture the bounce loop in a hipGraph of this many passes")
+    ap.add_argument("--autotune", action="store_true", help="pick the schedule by timing short renders first")
     ap.add_argument("--tail-paths", type=int, default=0,
                     help="live paths below which the render finishes in one persistent launch (0 automatic, -1 never)")
     ap.add_argument("--scene-dir", default=os.environ.get("LAMBDA_SCENE_DIR", "/tmp/lambda_bench"))
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the batched-serving leg")
+    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-gap-ms", type=float, default=50.0)
+    ap.add_argument("--serve-budget", type=int, default=0, help="HBM bytes admitted per launch (0 = no limit)")
+    ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--guided-res", default="320x180")
     ap.add_argument("--guided-ref-spp", type=int, default=65536)
     ap.add_argument("--guided-step", type=int, default=64)
@@ -260,23 +299,34 @@ def main() -> int:
     t_upload = time.perf_counter() - t0
     mode = lambda_amd.LAMBDA_RNG_REFERENCE if args.rng == "reference" else lambda_amd.LAMBDA_RNG_THROUGHPUT
     p = scene.params
+    tuned = None
+    if args.autotune:
+        from lambda_amd import autotune
+        t0 = time.perf_counter()
+        tuned = autotune.tune(dev, cache=None, rng_mode=mode)
+        log("autotune (%.1f s): %s" % (time.perf_counter() - t0, tuned["best"]))
+        best = tuned["best"]
+        args.path_slots = best.get("path_slots", 0)
+        args.tail_paths = best.get("tail_paths", 0)
+        args.cost_order = best.get("cost_order", 0)
+        args.graph_passes = best.get("graph_passes", 0)
     paths_per_step = p.width * p.height * p.samples
 
     for i in range(args.warmup):
         t0 = time.perf_counter()
-        dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order)
+        dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order, graph_passes=args.graph_passes)
         log("warmup %d: %.3f s" % (i, time.perf_counter() - t0))
 
     t0 = time.perf_counter()
     for i in range(args.steps):
-        img = dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order)
+        img = dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order, graph_passes=args.graph_passes)
     elapsed = time.perf_counter() - t0
     value = paths_per_step * args.steps / elapsed
     log("timed %d steps: %.3f s -> %.4g paths/s" % (args.steps, elapsed, value))
     finite = bool(np.all(np.isfinite(img)))
 
     # Kernel timing from HIP events on the render stream.
-    _, st = dev.render_stats(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order)
+    _, st = dev.render_stats(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order, graph_passes=args.graph_passes)
     log("kernel ms: " + ", ".join("%s %.1f" % (k, st[k + "_ms"]) for k in ("trace", "shade", "finalize", "persist")))
 
     # The same frame with independent streams per (pixel, sample): same
@@ -328,8 +378,18 @@ def main() -> int:
         log("CPU baseline %.4g paths/s (%s)" % (cpu["value"], cpu["sample"]))
 
     guided = None if args.no_guided else guided_leg(args, args.scene_dir)
+    serving = None
+    if not args.no_serve:
+        serving = serve_leg(args, dev, args.scene_dir)
+        log("serving: %d jobs in %d launches, %.4g paths/s, latency p50 %.0f ms p99 %.0f ms" % (
+            serving["jobs"], serving["batches"], serving["paths_per_s"], serving["latency_ms"]["p50"],
+            serving["latency_ms"]["p99"]))
 
-    cfg.update({"rng": args.rng, "path_slots": st["path_slots"], "setup_s": {
+    if tuned:
+        cfg["autotune"] = {"best": tuned["best"], "trial_spp": tuned["trial_spp"],
+                           "table_ms": [round(ms, 2) for _, ms in tuned["table"]]}
+    cfg.update({"rng": args.rng, "path_slots": st["path_slots"], "schedule": {
+        "tail_paths": args.tail_paths, "cost_order": args.cost_order, "graph_passes": args.graph_passes}, "setup_s": {
         "generate": round(t_gen, 2), "parse_and_bvh": round(t_parse, 2), "upload": round(t_upload, 2)}})
     out = {
         "metric": METRIC,
@@ -354,6 +414,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "throughput_streams": thr,
         "guided_vs_plain": guided,
+        "batched_serving": serving,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         "kernels_ms": {k: st[k + "_ms"] for k in ("trace", "shade", "finalize", "persist")},
         "rays": {"closest": st["closest_rays"], "shadow": st["shadow_rays"], "iterations": st["iterations"]},
