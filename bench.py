@@ -206,6 +206,32 @@ def serve_leg(args, mesh_dev, scene_dir: str) -> dict:
             "seed": args.serve_seed}
 
 
+# FLOP per row of the guiding network (2 per multiply-add): forward
+# 96x64 + 3 x 64x64 + 64x64 (output padded to 64), gradient = forward + the
+# backward data and weight products (2x the forward's, layer 0 without the
+# data product).
+GUIDE_FWD_FLOP = 2 * (96 * 64 + 4 * 64 * 64)
+GUIDE_GRAD_FLOP = GUIDE_FWD_FLOP + 2 * (2 * (96 * 64 + 4 * 64 * 64) - 96 * 64)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def guide_kernels(rows_fwd: int = 1 << 20, rows_grad: int = 1 << 16) -> dict:
+    """Throughput of the guiding network's BF16 MFMA kernels against the
+    dense BF16 matrix peak."""
+    import lambda_amd
+    g = lambda_amd.Guide(seed=1)
+    f_ms, _ = g.benchmark(rows_fwd, 20)
+    _, g_ms = g.benchmark(rows_grad, 20)
+    g.close()
+    f_tf = GUIDE_FWD_FLOP * rows_fwd / (f_ms * 1e-3) / 1e12
+    g_tf = GUIDE_GRAD_FLOP * rows_grad / (g_ms * 1e-3) / 1e12
+    return {"bound": "mfma", "unit": "TFLOP/s", "peak": MFMA_BF16_PEAK_TFLOPS,
+            "forward": {"rows": rows_fwd, "ms": f_ms, "achieved": f_tf, "frac": f_tf / MFMA_BF16_PEAK_TFLOPS,
+                        "flop_per_row": GUIDE_FWD_FLOP},
+            "gradient": {"rows": rows_grad, "ms": g_ms, "achieved": g_tf, "frac": g_tf / MFMA_BF16_PEAK_TFLOPS,
+                         "flop_per_row": GUIDE_GRAD_FLOP}}
+
+
 def copy_probe(nbytes: int) -> None:
     """One device-to-device hipMemcpy of nbytes through the HIP runtime the
     integrator itself uses (the runtime's copy-buffer kernel reads and writes
@@ -378,6 +404,10 @@ def main() -> int:
         log("CPU baseline %.4g paths/s (%s)" % (cpu["value"], cpu["sample"]))
 
     guided = None if args.no_guided else guided_leg(args, args.scene_dir)
+    mlp = None if args.no_guided else guide_kernels()
+    if mlp:
+        log("guide MLP: forward %.1f TFLOP/s, gradient %.1f TFLOP/s (BF16 MFMA)" % (
+            mlp["forward"]["achieved"], mlp["gradient"]["achieved"]))
     serving = None
     if not args.no_serve:
         serving = serve_leg(args, dev, args.scene_dir)
@@ -414,6 +444,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "throughput_streams": thr,
         "guided_vs_plain": guided,
+        "guide_mlp_roofline": mlp,
         "batched_serving": serving,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         "kernels_ms": {k: st[k + "_ms"] for k in ("trace", "shade", "finalize", "persist")},

@@ -320,6 +320,12 @@ int lambda_hip_guide_gradient(lambda_guide *guide, const float *samples, int32_t
 int lambda_hip_guide_train(lambda_guide *guide, const float *samples, int32_t n, int32_t batch, float lr,
                            double *loss, char *err, size_t err_len);
 
+/* Times the network's kernels on `rows` device-resident samples: mean
+ * milliseconds per forward pass and per gradient pass over `iters` runs
+ * (HIP events on the null stream).  The parameters do not change. */
+int lambda_hip_guide_benchmark(lambda_guide *guide, int32_t rows, int32_t iters, double *forward_ms,
+                               double *gradient_ms, char *err, size_t err_len);
+
 /*
  * Batched serving (no reference counterpart: the reference renders one
  * scene per process).  Many render jobs share one persistent launch: each

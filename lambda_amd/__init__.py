@@ -272,6 +272,13 @@ class Guide:
                                                    C.byref(loss), err, 4096), err)
         return g, loss.value
 
+    def benchmark(self, rows: int = 1 << 20, iters: int = 10):
+        """(forward ms, gradient ms) per pass over `rows` device-resident rows."""
+        f, g = C.c_double(), C.c_double()
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_guide_benchmark(self._h, rows, iters, C.byref(f), C.byref(g), err, 4096), err)
+        return f.value, g.value
+
     def train(self, samples: np.ndarray, batch: int, lr: float = 1e-3) -> float:
         s = np.ascontiguousarray(samples, np.float32).reshape(-1, 10)
         loss = C.c_double()

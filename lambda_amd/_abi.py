@@ -158,6 +158,7 @@ def hip():
         lib.lambda_hip_guide_gradient.argtypes = [vp, _f32p, C.c_int32, _f32p, f64p, C.c_char_p, C.c_size_t]
         lib.lambda_hip_guide_train.argtypes = [vp, _f32p, C.c_int32, C.c_int32, C.c_float, f64p, C.c_char_p,
                                                C.c_size_t]
+        lib.lambda_hip_guide_benchmark.argtypes = [vp, C.c_int32, C.c_int32, f64p, f64p, C.c_char_p, C.c_size_t]
         lib.lambda_hip_serve.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Job), C.c_int32, C.c_int64,
                                          C.POINTER(JobResult), C.POINTER(ServeStats), C.c_char_p, C.c_size_t]
         lib.lambda_hip_progressive_create.argtypes = [vp, rp, vp, C.POINTER(vp), C.c_char_p, C.c_size_t]

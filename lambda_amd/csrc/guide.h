@@ -61,6 +61,23 @@ LAM_FN float guide_encode(const GuideFeatures &g, int i) {
     return (i & 1) ? cosf(x) : sinf(x);
 }
 
+// All 16 encoded values of scalar s (bands 0..7, sin then cos) from one
+// sincos and the double-angle recurrences sin 2x = 2 sin x cos x,
+// cos 2x = 1 - 2 sin^2 x; the recurrence error (about 1e-5 after 7
+// doublings) is far below the bfloat16 resolution the network stores.
+LAM_FN void guide_encode_scalar(float x, float (&v)[16]) {
+    float sn, cs;
+    sincosf(x * kPi, &sn, &cs);
+#pragma unroll
+    for (int b = 0; b < kGuideBands; b++) {
+        v[2 * b] = sn;
+        v[2 * b + 1] = cs;
+        const float s2 = 2 * sn * cs;
+        cs = 1 - 2 * sn * sn;
+        sn = s2;
+    }
+}
+
 struct Lobe {
     V3 mu;
     float kappa, norm, weight;  // norm = kappa / (2 pi (1 - exp(-2 kappa)))

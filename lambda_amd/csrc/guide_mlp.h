@@ -62,20 +62,30 @@ __device__ __forceinline__ void gemm_rows16(const __bf16 *A, int lda, const __bf
     }
 }
 
-// Encodes a wave's 16 feature rows into X[16][kLdX] (bf16); rows >= n are 0.
-__device__ __forceinline__ void encode_rows(const float *feat, int fstride, int row0, int n, __bf16 *X) {
+// Encodes `rows` feature rows starting at row0 into X[rows][kLdX] (bf16);
+// rows >= n are 0.  A lane encodes one (row, scalar) pair at a time: one
+// sincos, 16 values, two 16-byte stores.
+__device__ __forceinline__ void encode_rows(const float *feat, int fstride, int row0, int n, __bf16 *X,
+                                            int rows = kGuideRowsPerWave) {
     const int l = threadIdx.x & 63;
-    for (int e = l; e < kGuideRowsPerWave * kGuideIn; e += 64) {
-        const int r = e / kGuideIn, i = e % kGuideIn;
-        float v = 0;
+    for (int e = l; e < rows * kGuideScalars; e += 64) {
+        const int r = e / kGuideScalars, sc = e % kGuideScalars;
+        float v[16];
         if (row0 + r < n) {
-            GuideFeatures g;
-            const float *f = feat + size_t(row0 + r) * fstride;
+            guide_encode_scalar(feat[size_t(row0 + r) * fstride + sc], v);
+        } else {
 #pragma unroll
-            for (int s = 0; s < kGuideScalars; s++) g.f[s] = f[s];
-            v = guide_encode(g, i);
+            for (int k = 0; k < 16; k++) v[k] = 0;
         }
-        X[r * kLdX + i] = __bf16(v);
+        bf16x8 lo, hi;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            lo[k] = __bf16(v[k]);
+            hi[k] = __bf16(v[8 + k]);
+        }
+        bf16x8 *dst = reinterpret_cast<bf16x8 *>(X + r * kLdX + sc * 16);
+        dst[0] = lo;
+        dst[1] = hi;
     }
 }
 
@@ -171,19 +181,7 @@ __global__ void __launch_bounds__(256) k_guide_forward(GuideDev G, const float *
     __bf16 *H0 = reinterpret_cast<__bf16 *>(smem[w] + 16384);
     __bf16 *H1 = H0 + kFwdRowsPerWave * kLdH;
     const int row0 = blockIdx.x * kFwdRowsPerBlock + w * kFwdRowsPerWave;
-    // encode 64 rows: lane e handles elements e, e + 64, ...
-    for (int e = l; e < kFwdRowsPerWave * kGuideIn; e += 64) {
-        const int r = e / kGuideIn, i = e % kGuideIn;
-        float v = 0;
-        if (row0 + r < n) {
-            GuideFeatures g;
-            const float *f = feat + size_t(row0 + r) * fstride;
-#pragma unroll
-            for (int s = 0; s < kGuideScalars; s++) g.f[s] = f[s];
-            v = guide_encode(g, i);
-        }
-        X[r * kLdX + i] = __bf16(v);
-    }
+    encode_rows(feat, fstride, row0, n, X, kFwdRowsPerWave);
     __syncthreads();
     fwd_layer(G, 0, X, kLdX, H0, nullptr);
     __syncthreads();

@@ -2231,6 +2231,45 @@ int lambda_hip_progressive_set_training(lambda_hip_progressive *pr, int32_t epoc
 
 void lambda_hip_progressive_destroy(lambda_hip_progressive *pr) { delete pr; }
 
+int lambda_hip_guide_benchmark(lambda_guide *guide, int32_t rows, int32_t iters, double *forward_ms,
+                               double *gradient_ms, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!guide || rows < kGuideRowsPerBlock || iters < 1) throw Failure{LAMBDA_EINVAL, "invalid arguments"};
+        GuideNet &net = guide->net;
+        // device-resident samples: features on a lattice, unit directions, c = 1
+        std::vector<float> host(size_t(rows) * kGuideSample);
+        Pcg rng = pcg_seed(12345, 3);
+        for (int r = 0; r < rows; r++) {
+            float *o = &host[size_t(r) * kGuideSample];
+            for (int k = 0; k < kGuideScalars; k++) o[k] = rng.next();
+            V3 w = normalize(v3(rng.next() - 0.5f, rng.next() - 0.5f, rng.next() - 0.5f + 1e-3f));
+            o[6] = w.x; o[7] = w.y; o[8] = w.z; o[9] = 1.0f;
+        }
+        DevBuf<float> sm, out;
+        sm.upload(host);
+        out.alloc(size_t(rows) * kLobeOut);
+        EventPair ev;
+        HIP_OK(hipEventCreate(&ev.a));
+        HIP_OK(hipEventCreate(&ev.b));
+        net.forward(sm.p, kGuideSample, rows, out.p, kLobeOut, nullptr);
+        HIP_OK(hipEventRecord(ev.a, nullptr));
+        for (int i = 0; i < iters; i++) net.forward(sm.p, kGuideSample, rows, out.p, kLobeOut, nullptr);
+        HIP_OK(hipEventRecord(ev.b, nullptr));
+        HIP_OK(hipEventSynchronize(ev.b));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, ev.a, ev.b));
+        if (forward_ms) *forward_ms = ms / iters;
+        net.gradient(sm.p, rows, nullptr);
+        HIP_OK(hipEventRecord(ev.a, nullptr));
+        for (int i = 0; i < iters; i++) net.gradient(sm.p, rows, nullptr);
+        HIP_OK(hipEventRecord(ev.b, nullptr));
+        HIP_OK(hipEventSynchronize(ev.b));
+        HIP_OK(hipEventElapsedTime(&ms, ev.a, ev.b));
+        if (gradient_ms) *gradient_ms = ms / iters;
+        HIP_OK(hipMemset(net.grads.p, 0, kGuideParams * sizeof(float)));
+    });
+}
+
 int lambda_hip_device_info(char *name, size_t name_len, int32_t *compute_units, char *err, size_t err_len) {
     return guarded(err, err_len, [&] {
         check_device();

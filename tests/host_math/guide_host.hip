@@ -36,11 +36,13 @@ int main(int argc, char **argv) {
         std::vector<float> f = read_f32(argv[2]);
         const size_t n = f.size() / kGuideScalars;
         std::vector<float> out(n * kGuideIn);
-        for (size_t r = 0; r < n; r++) {
-            GuideFeatures g;
-            for (int s = 0; s < kGuideScalars; s++) g.f[s] = f[r * kGuideScalars + s];
-            for (int i = 0; i < kGuideIn; i++) out[r * kGuideIn + i] = guide_encode(g, i);
-        }
+        // the kernels' encoding: one sincos per scalar, double-angle bands
+        for (size_t r = 0; r < n; r++)
+            for (int s = 0; s < kGuideScalars; s++) {
+                float v[16];
+                guide_encode_scalar(f[r * kGuideScalars + s], v);
+                for (int k = 0; k < 16; k++) out[r * kGuideIn + s * 16 + k] = v[k];
+            }
         write_f32(argv[3], out);
     } else if (mode == "pdf" && argc >= 5) {
         std::vector<float> o = read_f32(argv[2]), w = read_f32(argv[3]);

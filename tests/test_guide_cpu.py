@@ -129,8 +129,8 @@ def run(driver, tmp_path, args, arrays, out_shape):
 def test_device_encoding_matches_oracle(driver, tmp_path):
     f = np.random.default_rng(1).random((500, 6)).astype(np.float32)
     got = run(driver, tmp_path, ["encode"], [f], (500, 96))
-    # float32 arguments up to 128 pi carry ~3e-5 of rounding; the network
-    # stores its input in bfloat16 (8 significant bits) anyway
+    # one sincos and 7 double-angle steps carry ~1e-5 of rounding; the
+    # network stores its input in bfloat16 (8 significant bits) anyway
     np.testing.assert_allclose(got, go.encode(f), atol=1e-4)
 
 
