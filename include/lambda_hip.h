@@ -184,9 +184,12 @@ typedef struct lambda_render_params {
     int32_t tail_paths;    /* once fewer paths than this are live, one persistent
                               launch runs each remaining path to its end instead
                               of one launch per bounce; 0 = automatic, -1 = never */
-    int32_t cost_order;    /* start pixels in decreasing order of a one-sample
-                              cost estimate (reference streams; the image does
-                              not change): 0 = automatic, 1 = on, -1 = off */
+    int32_t cost_order;    /* start 8x8 tiles in decreasing order of a cost
+                              estimate from a short pre-pass (the image does
+                              not change): 0 = automatic (on for reference
+                              streams with fewer slots than pixels), -1 = off,
+                              n >= 1 = on with a pre-pass of n samples per
+                              pixel (rounded down to a square, at most 64) */
     int32_t graph_passes;  /* > 0: one launch per bounce throughout, the bounce
                               loop captured in a hipGraph of this many passes
                               (rounded up to even) and replayed until done;
@@ -284,7 +287,7 @@ int lambda_hip_thinfilm_reflectance(lambda_hip_scene *scene, int32_t material, c
  * Path-guiding network (no reference counterpart: the reference samples
  * directions from the BSDF only).  A mixture of 8 von Mises-Fisher lobes
  * over the next direction, whose parameters come from an MLP of (position,
- * outgoing direction, wavelength): each of 6 features in [0, 1] is encoded
+ * direction, wavelength): each of 6 features in [0, 1] is encoded
  * as sin/cos of 2^b pi x, b = 0..7 (96 inputs), 4 hidden layers of 64 ReLU
  * units, 40 outputs = 8 x {mean xyz (normalised), kappa (softplus),
  * weight (softmax logit)}.  BF16 weights and activations on the matrix
@@ -292,6 +295,9 @@ int lambda_hip_thinfilm_reflectance(lambda_hip_scene *scene, int32_t material, c
  *
  * Parameters: for each layer l = 0..4, W_l[K_l][64] row-major then
  * b_l[64]; K_0 = 96, K_1..4 = 64; output columns 40..63 are padding.
+ * The renderer guides Lambertian vertices only, whose integrand does not
+ * depend on the outgoing direction: the direction feature it passes is the
+ * surface normal on the side of the incoming ray.
  * Training sample: {features[6], direction[3], weight}; the loss of a
  * sample is -weight * log q(direction).
  */
@@ -380,9 +386,13 @@ typedef struct lambda_progressive_stats {
     double loss;            /* mean training loss of that training */
 } lambda_progressive_stats;
 
-/* params: size, depth, wavelength range, seed and path_slots are used;
- * rng_mode and samples are ignored.  guide may be NULL; the session borrows
- * it (keep it alive until the session is destroyed). */
+/* params: size, depth, wavelength range, seed, path_slots and tail_paths
+ * are used; rng_mode and samples are ignored.  A guided session evaluates
+ * the network inside its persistent kernel (each wave runs it for its own
+ * 64 paths on the matrix cores); tail_paths = -1 selects one launch per
+ * bounce with the network as a kernel of its own instead.  guide may be
+ * NULL; the session borrows it (keep it alive until the session is
+ * destroyed). */
 int lambda_hip_progressive_create(lambda_hip_scene *scene, const lambda_render_params *params, lambda_guide *guide,
                                   lambda_hip_progressive **out, char *err, size_t err_len);
 /* Step flags. */
@@ -397,9 +407,13 @@ int lambda_hip_progressive_step(lambda_hip_progressive *session, int32_t samples
 /* out_rgb[W*H*3]: linear sRGB mean over all samples so far. */
 int lambda_hip_progressive_image(lambda_hip_progressive *session, float *out_rgb, char *err, size_t err_len);
 /* Training per step: `epochs` passes of Adam (rate lr) over the step's
- * samples in batches of `batch` (defaults 1, 16384, 1e-3). */
+ * samples in batches of `batch` (defaults 1, 16384, 1e-3).  Vertices with
+ * bounce index below guided_bounces are guided (0 = the default of 4; a
+ * value above the path depth guides every vertex); with fewer guided
+ * bounces than the depth, each batch of paths runs its guided bounces as
+ * passes and the rest in one persistent launch. */
 int lambda_hip_progressive_set_training(lambda_hip_progressive *session, int32_t epochs, int32_t batch, float lr,
-                                        char *err, size_t err_len);
+                                        int32_t guided_bounces, char *err, size_t err_len);
 void lambda_hip_progressive_destroy(lambda_hip_progressive *session);
 
 /* Device name and compute-unit count of the current HIP device. */

@@ -290,7 +290,10 @@ class Guide:
 
 class Progressive:
     """Progressive rendering session (lambda_hip_progressive_*): plain when
-    guide is None, guided otherwise.  The scene and guide must outlive it."""
+    guide is None, guided otherwise.  The scene and guide must outlive it.
+    Guided sessions run the network inside the persistent kernel; with
+    tail_paths=-1 they run one launch per bounce with the network as its own
+    kernel instead."""
 
     def __init__(self, dev: "DeviceScene", guide: "Guide | None" = None, path_slots: int = 0, seed: int = 42,
                  **overrides):
@@ -313,9 +316,13 @@ class Progressive:
         return {"samples": st.samples, "train_samples": st.train_samples, "render_ms": st.render_ms,
                 "train_ms": st.train_ms, "loss": st.loss}
 
-    def set_training(self, epochs: int = 1, batch: int = 1 << 14, lr: float = 1e-3) -> None:
+    def set_training(self, epochs: int = 1, batch: int = 1 << 14, lr: float = 1e-3, guided_bounces: int = 0) -> None:
+        """Adam epochs per step, batch, rate, and the vertices guided: those
+        with a bounce index below guided_bounces (0 = the default of 4; a
+        value above the depth guides every vertex)."""
         err = C.create_string_buffer(4096)
-        _check(self._lib.lambda_hip_progressive_set_training(self._h, epochs, batch, lr, err, 4096), err)
+        _check(self._lib.lambda_hip_progressive_set_training(self._h, epochs, batch, lr, guided_bounces, err, 4096),
+               err)
 
     def image(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 3), np.float32)

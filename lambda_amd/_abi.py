@@ -165,8 +165,8 @@ def hip():
         lib.lambda_hip_progressive_step.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(ProgressiveStats),
                                                     C.c_char_p, C.c_size_t]
         lib.lambda_hip_progressive_image.argtypes = [vp, _f32p, C.c_char_p, C.c_size_t]
-        lib.lambda_hip_progressive_set_training.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_char_p,
-                                                            C.c_size_t]
+        lib.lambda_hip_progressive_set_training.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_int32,
+                                                            C.c_char_p, C.c_size_t]
         lib.lambda_hip_progressive_destroy.argtypes = [vp]
         lib.lambda_hip_progressive_destroy.restype = None
         _hip = lib

@@ -1,6 +1,6 @@
 // Path-guiding distribution: a mixture of von Mises-Fisher lobes on the
-// sphere whose parameters come from a small MLP of (position, outgoing
-// direction, wavelength).  Host and device code (the CPU checks include it).
+// sphere whose parameters come from a small MLP of (position, direction,
+// wavelength).  Host and device code (the CPU checks include it).
 //
 // Network: input = 6 scalars, each frequency-encoded with 8 bands as
 // sin(2^b pi x), cos(2^b pi x) (96 values); 4 hidden layers of 64 ReLU units;
@@ -36,19 +36,22 @@ LAM_FN int guide_b_offset(int l) { return guide_w_offset(l) + guide_layer_in(l) 
 constexpr int kGuideParams = kGuideIn * 64 + 64 + 4 * (64 * 64 + 64);  // 22848
 
 // Raw features of one query, each mapped to [0, 1]: position in the scene
-// box, outgoing direction as (theta / pi, (phi + pi) / 2pi), wavelength over
-// 360-830 nm.
+// box, a direction as (theta / pi, (phi + pi) / 2pi), wavelength over
+// 360-830 nm.  The renderer guides Lambertian vertices only, where the
+// integrand L_i cos / pi does not depend on the outgoing direction, and
+// passes the surface normal (on the side of the incoming ray) as the
+// direction: the cosine factor of the learned product depends on it.
 struct GuideFeatures {
     float f[kGuideScalars];
 };
 
-LAM_FN GuideFeatures guide_features(V3 p, V3 lo, V3 hi, V3 wo, float lambda) {
+LAM_FN GuideFeatures guide_features(V3 p, V3 lo, V3 hi, V3 dir, float lambda) {
     GuideFeatures g;
     g.f[0] = (p.x - lo.x) / fmaxf(hi.x - lo.x, 1e-6f);
     g.f[1] = (p.y - lo.y) / fmaxf(hi.y - lo.y, 1e-6f);
     g.f[2] = (p.z - lo.z) / fmaxf(hi.z - lo.z, 1e-6f);
-    g.f[3] = acosf(fminf(fmaxf(wo.y, -1.0f), 1.0f)) / kPi;
-    g.f[4] = (atan2f(wo.z, wo.x) + kPi) / (2 * kPi);
+    g.f[3] = acosf(fminf(fmaxf(dir.y, -1.0f), 1.0f)) / kPi;
+    g.f[4] = (atan2f(dir.z, dir.x) + kPi) / (2 * kPi);
     g.f[5] = (lambda - 360.0f) / 470.0f;
     return g;
 }
@@ -70,6 +73,22 @@ LAM_FN void guide_encode_scalar(float x, float (&v)[16]) {
     sincosf(x * kPi, &sn, &cs);
 #pragma unroll
     for (int b = 0; b < kGuideBands; b++) {
+        v[2 * b] = sn;
+        v[2 * b + 1] = cs;
+        const float s2 = 2 * sn * cs;
+        cs = 1 - 2 * sn * sn;
+        sn = s2;
+    }
+}
+
+// Half of the encoding of one scalar: bands 4h .. 4h + 3 (h = 0 or 1), sin
+// then cos per band, from one sincos of 2^(4h) pi x and three double-angle
+// steps (what one lane of the inference kernel feeds the matrix core).
+LAM_FN void guide_encode_half(float x, int h, float (&v)[8]) {
+    float sn, cs;
+    sincosf(x * (h ? 16.0f * kPi : kPi), &sn, &cs);
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
         v[2 * b] = sn;
         v[2 * b + 1] = cs;
         const float s2 = 2 * sn * cs;
@@ -140,6 +159,43 @@ LAM_FN V3 guide_sample(const Lobe (&L)[kLobes], float u0, float u1, float u2) {
     float c = L[0].weight;
     while (k < kLobes - 1 && u0 >= c) c += L[++k].weight;
     return vmf_sample(L[k], u1, u2);
+}
+
+// Compact form of the mixture at one vertex, 5 floats per lobe (40 in all,
+// the size of the raw outputs): kappa mu (xyz), log(weight norm) - kappa, and
+// the cumulative weight through the lobe, so that
+//   q(w) = sum_k exp(c_k + (kappa mu)_k . w)
+// takes one exp per lobe and sampling needs no other state.  c may alias o.
+LAM_FN void guide_compact(const float *o, float *c) {
+    Lobe L[kLobes];
+    guide_lobes(o, L);
+    float cum = 0;
+#pragma unroll
+    for (int k = 0; k < kLobes; k++) {
+        cum += L[k].weight;
+        c[5 * k + 0] = L[k].kappa * L[k].mu.x;
+        c[5 * k + 1] = L[k].kappa * L[k].mu.y;
+        c[5 * k + 2] = L[k].kappa * L[k].mu.z;
+        c[5 * k + 3] = logf(L[k].weight * L[k].norm) - L[k].kappa;
+        c[5 * k + 4] = cum;
+    }
+}
+
+LAM_FN float guide_pdf_c(const float *c, V3 w) {
+    float p = 0;
+#pragma unroll
+    for (int k = 0; k < kLobes; k++) p += expf(c[5 * k + 3] + (c[5 * k] * w.x + c[5 * k + 1] * w.y + c[5 * k + 2] * w.z));
+    return p;
+}
+
+LAM_FN V3 guide_sample_c(const float *c, float u0, float u1, float u2) {
+    int k = 0;
+    while (k < kLobes - 1 && u0 >= c[5 * k + 4]) k++;
+    Lobe L;
+    const V3 km = v3(c[5 * k], c[5 * k + 1], c[5 * k + 2]);
+    L.kappa = len(km);
+    L.mu = km * (1 / L.kappa);
+    return vmf_sample(L, u1, u2);
 }
 
 // Loss of one training sample, -c log q(w), and its gradient with respect to

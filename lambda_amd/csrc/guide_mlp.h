@@ -114,89 +114,125 @@ __device__ __forceinline__ void layer_rows16(const GuideDev &G, int layer, const
     }
 }
 
-// Inference: out[i][0..39] for feature rows feat[i][0..5], i < n (n read
-// from n_dev when given).  Each wave takes 64 rows (4 tiles of 16) and
-// computes every layer transposed, H_next^T = W^T H^T: the weights are the
-// A operand (16 outputs x 32 inputs per fragment, read once and used for all
-// 4 row tiles) and the activations the B operand (row-major in LDS, 8
-// consecutive inputs of one row per lane), so that each lane receives 4
-// consecutive outputs of one row and stores them with one 8-byte write.
+// Inference works on 64 rows per wave (4 waves per block).
 constexpr int kFwdRowsPerWave = 64;
 constexpr int kFwdRowsPerBlock = kFwdRowsPerWave * kGuideWaves;  // 256
 
-__device__ __forceinline__ void fwd_layer(const GuideDev &G, int layer, const __bf16 *Hin, int ldin, __bf16 *Hout,
-                                          float *out) {
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x4 relu_bf16x4(f32x4 v, float4 b) {
+    bf16x4 h;
+    h[0] = __bf16(fmaxf(v[0] + b.x, 0.0f));
+    h[1] = __bf16(fmaxf(v[1] + b.y, 0.0f));
+    h[2] = __bf16(fmaxf(v[2] + b.z, 0.0f));
+    h[3] = __bf16(fmaxf(v[3] + b.w, 0.0f));
+    return h;
+}
+
+__device__ __forceinline__ bf16x8 cat_bf16x4(bf16x4 a, bf16x4 b) {
+    return bf16x8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+}
+
+// The network for the 64 rows of one wave with the activations in
+// registers (no LDS): every layer is computed transposed, H_next^T = W^T H^T,
+// so that lane (r16 = l % 16, q = l / 16) receives H_next[rt*16 + r16][nt*16 +
+// 4q + i], i = 0..3, for the 4 row tiles rt and 4 column tiles nt.  The next
+// layer contracts over k in the order those registers already hold: its
+// k-step m takes from lane (r16, q) the 8 values H[row][32m + 4q + 0..3] and
+// H[row][32m + 16 + 4q + 0..3], and the weight fragment is read in the same
+// order.  Layer 0 encodes its inputs in place: lane (r16, q) needs, at k-step
+// m, bands 4 (q & 1) .. +3 of scalar 2m + (q >> 1) of each of its rows.
+// Rows >= nvalid read zero features and are not written; out[row][0..39].
+__device__ __forceinline__ void mlp_rows64(const GuideDev &G, const float *feat, int fstride, int nvalid, float *out,
+                                           int ostride) {
     const int l = threadIdx.x & 63;
     const int r16 = l & 15, q = l >> 4;
-    const int K = guide_layer_in(layer);
-    const __bf16 *Wt = G.w_nk + guide_wbf_offset(layer);  // [64 n][K]
-    const float *bias = G.params + guide_b_offset(layer);
+    f32x4 acc[4][4];
+    bf16x8 hb[2][4];
+    // layer 0: K = 96 in natural order, k = 32m + 8q + j; the inputs of one
+    // k-step are encoded just before its products
+    {
+        const __bf16 *Wt = G.w_nk + guide_wbf_offset(0);  // [64 n][96]
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-        f32x4 acc[4];
+        for (int nt = 0; nt < 4; nt++)
 #pragma unroll
-        for (int rt = 0; rt < 4; rt++) acc[rt] = f32x4{0, 0, 0, 0};
-        for (int kk = 0; kk < K; kk += 32) {
-            // A = W^T[nt*16 + r16][kk + 8q .. +7]
-            const bf16x8 a = load8(Wt + (nt * 16 + r16) * K + kk + 8 * q);
+            for (int rt = 0; rt < 4; rt++) acc[nt][rt] = f32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const int sc = 2 * m + (q >> 1);
+            bf16x8 xb[4];
 #pragma unroll
             for (int rt = 0; rt < 4; rt++) {
-                // B = H^T[kk + 8q + j][row = rt*16 + r16] = H[row][kk + 8q + j]
-                const bf16x8 b = load8(Hin + (rt * 16 + r16) * ldin + kk + 8 * q);
-                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[rt], 0, 0, 0);
+                const int row = rt * 16 + r16;
+                const float x = row < nvalid ? feat[size_t(row) * fstride + sc] : 0.0f;
+                float v[8];
+                guide_encode_half(x, q & 1, v);
+#pragma unroll
+                for (int j = 0; j < 8; j++) xb[rt][j] = __bf16(v[j]);
+            }
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) {
+                const bf16x8 a = load8(Wt + (nt * 16 + r16) * kGuideIn + 32 * m + 8 * q);
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++)
+                    acc[nt][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xb[rt], acc[nt][rt], 0, 0, 0);
             }
         }
-        // lane holds H_next[row = rt*16 + r16][n = nt*16 + 4q + i], i = 0..3
-        const int n0 = nt * 16 + 4 * q;
-        const float b0 = bias[n0], b1 = bias[n0 + 1], b2 = bias[n0 + 2], b3 = bias[n0 + 3];
+    }
+    // hidden layers 1..3 and the output layer 4 (its last column tile is
+    // padding and is skipped)
+#pragma unroll
+    for (int layer = 1; layer < kGuideLayers; layer++) {
+        const float *bprev = G.params + guide_b_offset(layer - 1);
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            const float4 b0 = *reinterpret_cast<const float4 *>(bprev + 32 * m + 4 * q);
+            const float4 b1 = *reinterpret_cast<const float4 *>(bprev + 32 * m + 16 + 4 * q);
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++)
+                hb[m][rt] = cat_bf16x4(relu_bf16x4(acc[2 * m][rt], b0), relu_bf16x4(acc[2 * m + 1][rt], b1));
+        }
+        const __bf16 *Wt = G.w_nk + guide_wbf_offset(layer);  // [64 n][64]
+        const int ntiles = layer == kGuideLayers - 1 ? 3 : 4;
+#pragma unroll
+        for (int nt = 0; nt < ntiles; nt++) {
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++) acc[nt][rt] = f32x4{0, 0, 0, 0};
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                const __bf16 *w = Wt + (nt * 16 + r16) * kGuideHidden + 32 * m + 4 * q;
+                const bf16x8 a = cat_bf16x4(*reinterpret_cast<const bf16x4 *>(w),
+                                            *reinterpret_cast<const bf16x4 *>(w + 16));
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++)
+                    acc[nt][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hb[m][rt], acc[nt][rt], 0, 0, 0);
+            }
+        }
+    }
+    const float *bout = G.params + guide_b_offset(kGuideLayers - 1);
+#pragma unroll
+    for (int nt = 0; nt < 3; nt++) {
+        const int c0 = nt * 16 + 4 * q;
+        if (c0 >= kLobeOut) continue;
+        const float4 b = *reinterpret_cast<const float4 *>(bout + c0);
 #pragma unroll
         for (int rt = 0; rt < 4; rt++) {
             const int row = rt * 16 + r16;
-            const float v0 = acc[rt][0] + b0, v1 = acc[rt][1] + b1, v2 = acc[rt][2] + b2, v3 = acc[rt][3] + b3;
-            if (Hout) {
-                typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-                bf16x4 h;
-                h[0] = __bf16(fmaxf(v0, 0.0f));
-                h[1] = __bf16(fmaxf(v1, 0.0f));
-                h[2] = __bf16(fmaxf(v2, 0.0f));
-                h[3] = __bf16(fmaxf(v3, 0.0f));
-                *reinterpret_cast<bf16x4 *>(Hout + row * kLdH + n0) = h;
-            } else {
-                *reinterpret_cast<float4 *>(out + row * 64 + n0) = make_float4(v0, v1, v2, v3);
-            }
+            if (row < nvalid)
+                *reinterpret_cast<float4 *>(out + size_t(row) * ostride + c0) =
+                    make_float4(acc[nt][rt][0] + b.x, acc[nt][rt][1] + b.y, acc[nt][rt][2] + b.z, acc[nt][rt][3] + b.w);
         }
     }
 }
 
-__global__ void __launch_bounds__(256) k_guide_forward(GuideDev G, const float *feat, int fstride, int n,
-                                                       const int *n_dev, float *out, int ostride) {
+// Inference for n rows (n read from n_dev when given): 64 rows per wave,
+// registers only.  ostride must be a multiple of 4.
+__global__ void __launch_bounds__(256, 2) k_guide_forward(GuideDev G, const float *feat, int fstride, int n,
+                                                          const int *n_dev, float *out, int ostride) {
     if (n_dev) n = *n_dev;
-    if (int(blockIdx.x) * kFwdRowsPerBlock >= n) return;
-    // per wave: X [64][kLdX] bf16 (13 KB) reused as the fp32 output [64][64]
-    // (16 KB), and two hidden buffers [64][kLdH] bf16 (9 KB each)
-    __shared__ __attribute__((aligned(16))) unsigned char smem[kGuideWaves][16384 + 2 * kFwdRowsPerWave * kLdH * 2];
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    __bf16 *X = reinterpret_cast<__bf16 *>(smem[w]);
-    float *O = reinterpret_cast<float *>(smem[w]);
-    __bf16 *H0 = reinterpret_cast<__bf16 *>(smem[w] + 16384);
-    __bf16 *H1 = H0 + kFwdRowsPerWave * kLdH;
-    const int row0 = blockIdx.x * kFwdRowsPerBlock + w * kFwdRowsPerWave;
-    encode_rows(feat, fstride, row0, n, X, kFwdRowsPerWave);
-    __syncthreads();
-    fwd_layer(G, 0, X, kLdX, H0, nullptr);
-    __syncthreads();
-    fwd_layer(G, 1, H0, kLdH, H1, nullptr);
-    __syncthreads();
-    fwd_layer(G, 2, H1, kLdH, H0, nullptr);
-    __syncthreads();
-    fwd_layer(G, 3, H0, kLdH, H1, nullptr);
-    __syncthreads();
-    fwd_layer(G, 4, H1, kLdH, nullptr, O);
-    __syncthreads();
-    for (int e = l; e < kFwdRowsPerWave * kLobeOut; e += 64) {
-        const int r = e / kLobeOut, c = e % kLobeOut;
-        if (row0 + r < n) out[size_t(row0 + r) * ostride + c] = O[r * 64 + c];
-    }
+    const int row0 = blockIdx.x * kFwdRowsPerBlock + (threadIdx.x >> 6) * kFwdRowsPerWave;
+    if (row0 >= n) return;
+    mlp_rows64(G, feat + size_t(row0) * fstride, fstride, n - row0, out + size_t(row0) * ostride, ostride);
 }
 
 // Training sample: features[6], direction[3], weight c.

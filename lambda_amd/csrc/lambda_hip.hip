@@ -161,6 +161,7 @@ struct Frame {
     int train_cap;
     float train_keep;         // probability of keeping a training candidate
     int grec_stride;          // guided vertices per slot (depth + 1)
+    int guide_bounces;        // vertices with bounce index below this are guided
     float3 box_lo, box_hi;    // scene box for the guiding features
 };
 
@@ -262,11 +263,11 @@ __device__ __forceinline__ void start_unit(const DeviceScene &S, const Frame &F,
     else start_work(S, F, P, s, w);
 }
 
-// First work unit of every slot: slot s takes unit s.
-__global__ void __launch_bounds__(kBlock) k_start(DeviceScene S, Frame F, Paths P, int nslots) {
+// First work unit of every slot: slot s takes unit base + s.
+__global__ void __launch_bounds__(kBlock) k_start(DeviceScene S, Frame F, Paths P, int nslots, long long base) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
-    start_unit(S, F, P, s, s);
+    start_unit(S, F, P, s, base + s);
     P.q_in[s] = s;
 }
 
@@ -340,11 +341,12 @@ constexpr float kRecNone = -2.0f, kRecPending = -1.0f;
 // state; returns true when the sample's path has ended.  *push_shadow tells
 // whether next-event estimation produced an occlusion ray.
 //
-// kGuided: Lambertian vertices sample the next direction from the one-sample
-// mixture 0.5 cos/pi + 0.5 q (q: the guiding distribution from the network
-// outputs gout); the mixture density is the scatter pdf of both MIS
-// weights (next-event estimation and emitter hits), so the estimator stays
-// unbiased.  The vertex is recorded for training in P.grec.
+// kGuided with gout set: a Lambertian vertex samples the next direction from
+// the one-sample mixture 0.5 cos/pi + 0.5 q (q: the guiding distribution in
+// the compact form of guide_compact, features gfeat); the mixture density is the
+// scatter pdf of both MIS weights (next-event estimation and emitter hits),
+// so the estimator stays unbiased.  The vertex is recorded for training in
+// P.grec.  Without gout the vertex is shaded as in the plain tracer.
 template <bool kGuided = false>
 __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F, const Paths &P, int s,
                                            bool &push_shadow, ShadowRay &sh, const float *gout = nullptr,
@@ -353,6 +355,13 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
     {
         float4 o4 = P.o[s], d4 = P.d[s];
         int4 info = P.info[s];
+        if (info.w & kFlagGuidePending) {
+            // the radiance gathered so far is the base of the previous guided
+            // vertex's continuation (never set in unguided renders)
+            if (info.z >= 1 && info.z - 1 < F.grec_stride)
+                P.grec[(size_t(s) * F.grec_stride + info.z - 1) * 3 + 2].w = P.L[s].x;
+            info.w &= ~kFlagGuidePending;
+        }
         int2 hit = P.hit[s];
         S4 lam = s4(P.lam[s]);
         S4 beta = s4(P.beta[s]);
@@ -399,10 +408,7 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     const bool lambertian = mt.x == LAMBDA_MATERIAL_LAMBERTIAN;
                     S4 albedo = s4(0);
                     if (lambertian) albedo = lambertian_albedo(S, mt, h, lam);
-                    Lobe GL[kGuided ? kLobes : 1];
-                    if constexpr (kGuided) {
-                        if (lambertian) guide_lobes(gout, GL);
-                    }
+                    const bool guided = kGuided && lambertian && gout != nullptr;
                     if (lambertian && total > 0) {
                         // Next-event estimation (render/renderer.h:133-175).
                         float target = total * rng.next();
@@ -443,7 +449,9 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                                 S4 att = s4(0);
                                 if (dot(h.n, ldir) > 0) att = albedo / kPi;
                                 float sp = ct > 0 ? ct / kPi : 0;
-                                if constexpr (kGuided) sp = 0.5f * sp + 0.5f * guide_pdf(GL, ldir);
+                                if constexpr (kGuided) {
+                                    if (guided) sp = 0.5f * sp + 0.5f * guide_pdf_c(gout, ldir);
+                                }
                                 float w = power_heuristic(lprob, sp);
                                 S4 c = beta * att * ct * em * w / lprob;
                                 push_shadow = true;
@@ -458,16 +466,15 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     S4 att;
                     V3 nd;
                     if (lambertian) {
+                        bool from_guide = false;
                         if constexpr (kGuided) {
-                            if (rng.next() < 0.5f) {
-                                nd = random_in_hemisphere(h.n, rng);
-                            } else {
+                            if (guided && rng.next() >= 0.5f) {
                                 float u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
-                                nd = guide_sample(GL, u0, u1, u2);
+                                nd = guide_sample_c(gout, u0, u1, u2);
+                                from_guide = true;
                             }
-                        } else {
-                            nd = random_in_hemisphere(h.n, rng);
                         }
+                        if (!from_guide) nd = random_in_hemisphere(h.n, rng);
                         att = dot(h.n, nd) <= 0 ? s4(0) : albedo / kPi;
                     } else if (mt.x == LAMBDA_MATERIAL_MIRROR) {
                         nd = reflected(d, h.n);
@@ -523,8 +530,8 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     float ct = dot(h.n, d);
                     prev_pdf = lambertian ? (ct > 0 ? ct / kPi : 0) : 0;
                     if constexpr (kGuided) {
-                        if (lambertian) {
-                            prev_pdf = 0.5f * prev_pdf + 0.5f * guide_pdf(GL, d);
+                        if (guided) {
+                            prev_pdf = 0.5f * prev_pdf + 0.5f * guide_pdf_c(gout, d);
                             // below the surface the contribution is zero
                             if (!(ct > 0) || !(prev_pdf > 0)) finished = true;
                         }
@@ -546,7 +553,7 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     }
                     if (bounce >= F.depth) finished = true;
                     if constexpr (kGuided) {
-                        if (lambertian && !finished && bounce < F.grec_stride) {
+                        if (guided && !finished && bounce < F.grec_stride) {
                             float4 *r = P.grec + (size_t(s) * F.grec_stride + bounce) * 3;
                             r[0] = make_float4(gfeat[0], gfeat[1], gfeat[2], gfeat[3]);
                             r[1] = make_float4(gfeat[4], gfeat[5], d.x, d.y);
@@ -643,10 +650,9 @@ struct GuideQueue {
     float *out;      // [n][40] network outputs
 };
 
-// Guided pass, step 1: every queued path first closes the record of its
-// previous guided vertex (the radiance gathered so far is the base of that
-// vertex's continuation); paths on a Lambertian surface then queue their
-// features for the network, all others are shaded at once.
+// Guided pass, step 1: paths on a Lambertian surface within the guided
+// bounces queue their features for the network, all others are shaded at
+// once (shading closes the record of a path's previous guided vertex).
 __global__ void __launch_bounds__(kBlock) k_shade_guided_queue(DeviceScene S, Frame F, Paths P, int count,
                                                                GuideQueue Q) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -657,21 +663,16 @@ __global__ void __launch_bounds__(kBlock) k_shade_guided_queue(DeviceScene S, Fr
     GuideFeatures gf;
     if (active) {
         s = P.q_in[q];
-        int4 info = P.info[s];
-        if (info.w & kFlagGuidePending) {
-            if (info.z >= 1 && info.z - 1 < F.grec_stride)
-                P.grec[(size_t(s) * F.grec_stride + info.z - 1) * 3 + 2].w = P.L[s].x;
-            info.w &= ~kFlagGuidePending;
-            P.info[s] = info;
-        }
+        const int4 info = P.info[s];
         if (info.z < F.grec_stride) P.grec[(size_t(s) * F.grec_stride + info.z) * 3 + 2].w = kRecNone;
         const int2 hit = P.hit[s];
-        if (hit.x >= 0 && S.mat[material_of(S, hit.x, hit.y)].x == LAMBDA_MATERIAL_LAMBERTIAN) {
+        if (hit.x >= 0 && info.z < F.guide_bounces &&
+            S.mat[material_of(S, hit.x, hit.y)].x == LAMBDA_MATERIAL_LAMBERTIAN) {
             const float4 o4 = P.o[s], d4 = P.d[s];
             Hit h;
             if (hit_record(S, hit.x, hit.y, v3(o4), v3(d4), h)) {
                 gf = guide_features(h.p, v3(F.box_lo.x, F.box_lo.y, F.box_lo.z), v3(F.box_hi.x, F.box_hi.y, F.box_hi.z),
-                                    v3(-d4.x, -d4.y, -d4.z), P.lam[s].x);
+                                    h.n, P.lam[s].x);
                 guided = true;
             }
         }
@@ -709,8 +710,9 @@ __global__ void __launch_bounds__(kBlock) k_shade_guided(DeviceScene S, Frame F,
     ShadowRay sh;
     if (active) {
         s = Q.slot[q];
-        path_done = shade_slot<true>(S, F, P, s, push_shadow, sh, Q.out + size_t(q) * kLobeOut,
-                                     Q.feat + size_t(q) * kGuideScalars);
+        float *lobes = Q.out + size_t(q) * kLobeOut;
+        guide_compact(lobes, lobes);
+        path_done = shade_slot<true>(S, F, P, s, push_shadow, sh, lobes, Q.feat + size_t(q) * kGuideScalars);
     }
     int pos = wave_push(&P.ctr[kCtrShadow], push_shadow);
     if (push_shadow) {
@@ -749,7 +751,8 @@ __device__ __forceinline__ bool retire_sample(const DeviceScene &S, const Frame 
         // that reached vertex b along its sampled direction is the radiance
         // gathered after it over the throughput after it (hero wavelength)
         const float Lend = P.L[s].x;
-        for (int b = 0; b < info.z && b < F.grec_stride; b++) {
+        const int nb = min(min(info.z, F.grec_stride), F.guide_bounces);
+        for (int b = 0; b < nb; b++) {
             float4 *r = P.grec + (size_t(s) * F.grec_stride + b) * 3;
             const float4 r2 = r[2];
             if (!(r2.w >= 0)) continue;
@@ -865,6 +868,98 @@ __global__ void __launch_bounds__(kBlock, kPersistWaves) k_persist(DeviceScene S
         const int w = atomicAdd(&P.ctr[kCtrWork], 1);
         if (w >= work_units(F)) break;
         start_unit(S, F, P, s, w);
+    }
+    atomicAdd(&P.rays[0], (unsigned long long)n_closest);
+    atomicAdd(&P.rays[1], (unsigned long long)n_shadow);
+}
+
+// Waves per SIMD of the guided persistent kernel (its LDS, stacks plus the
+// network's staging, allows two blocks per CU).
+constexpr int kPersistGuidedWaves = 2;
+
+// The persistent schedule with the guiding network evaluated inside the
+// kernel.  Every thread runs its paths as in k_persist; after each closest
+// hit, the lanes at a guided vertex (Lambertian, bounce index below
+// F.guide_bounces) put their features in the wave's LDS rows, and if any
+// lane has one, the whole wave runs the network for its 64 rows on the
+// matrix cores (mlp_rows64) and each lane shades with its own row of
+// outputs.  The loop ends for the wave together, when no lane has a path.
+__global__ void __launch_bounds__(kBlock, kPersistGuidedWaves) k_persist_guided(DeviceScene S, Frame F, Paths P,
+                                                                                int n_in, int2 *spill, GuideDev G) {
+    __shared__ int s_code[kRing * kBlock];
+    __shared__ float s_dist[kRing * kBlock];
+    // per wave: features [64][6] in, network outputs [64][40] out
+    __shared__ __attribute__((aligned(16))) float s_io[kBlock / 64][64 * kLobeOut];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    float *io = s_io[threadIdx.x >> 6];
+    const int n_new = P.ctr[kCtrNext];
+    bool alive = t < n_in + n_new;
+    const int s = alive ? (t < n_in ? P.q_in[t] : P.q_out[t - n_in]) : 0;
+    Stack stk = make_stack(s_code, s_dist, spill);
+    unsigned n_closest = 0, n_shadow = 0;
+    while (__any(alive)) {
+        bool guided = false;
+        GuideFeatures gf{};
+        if (alive) {
+            n_closest++;
+            float4 o = P.o[s], d = P.d[s];
+            float best = kMax;
+            int hi = -1, ho = -1;
+            stk.sp = stk.lo = 0;
+            trace_ray<false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk);
+            P.hit[s] = make_int2(hi, ho);
+            const int bounce = P.info[s].z;
+            if (bounce < F.grec_stride && bounce < F.guide_bounces)
+                P.grec[(size_t(s) * F.grec_stride + bounce) * 3 + 2].w = kRecNone;
+            if (hi >= 0 && bounce < F.guide_bounces && S.mat[material_of(S, hi, ho)].x == LAMBDA_MATERIAL_LAMBERTIAN) {
+                Hit h;
+                if (hit_record(S, hi, ho, v3(o), v3(d), h)) {
+                    gf = guide_features(h.p, v3(F.box_lo.x, F.box_lo.y, F.box_lo.z),
+                                        v3(F.box_hi.x, F.box_hi.y, F.box_hi.z), h.n, P.lam[s].x);
+                    guided = true;
+                }
+            }
+        }
+        if (__any(guided)) {
+            // every lane of the wave is here (the condition is uniform)
+#pragma unroll
+            for (int k = 0; k < kGuideScalars; k++) io[lane * kGuideScalars + k] = guided ? gf.f[k] : 0.0f;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // reads the features in its first layer, writes [64][40] at the end
+            mlp_rows64(G, io, kGuideScalars, 64, io, kLobeOut);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // each guided lane turns its own row into the compact mixture
+            if (guided) guide_compact(io + lane * kLobeOut, io + lane * kLobeOut);
+        }
+        if (alive) {
+            bool push = false;
+            ShadowRay sh;
+            const bool ended = shade_slot<true>(S, F, P, s, push, sh, guided ? io + lane * kLobeOut : nullptr, gf.f);
+            if (push) {
+                n_shadow++;
+                float best = sh.o.w;
+                int hi = -1, ho = -1;
+                stk.sp = stk.lo = 0;
+                if (!trace_ray<true>(S, v3(sh.o), v3(sh.d), best, hi, ho, sh.i.y, sh.i.z, stk)) {
+                    float4 L = P.L[s];
+                    L.x += sh.c.x;
+                    L.y += sh.c.y;
+                    L.z += sh.c.z;
+                    L.w += sh.c.w;
+                    P.L[s] = L;
+                }
+            }
+            if (ended && !retire_sample(S, F, P, s)) {
+                const int w = atomicAdd(&P.ctr[kCtrWork], 1);
+                if (w < work_units(F)) start_unit(S, F, P, s, w);
+                else alive = false;
+            }
+        }
     }
     atomicAdd(&P.rays[0], (unsigned long long)n_closest);
     atomicAdd(&P.rays[1], (unsigned long long)n_shadow);
@@ -1413,13 +1508,16 @@ struct EventPair {
 
 
 // Threads of k_persist resident on the device at once.
-long long resident_persist_threads() {
+template <typename Kernel>
+long long resident_threads(Kernel kernel) {
     int dev = 0, cus = 0, blocks = 0;
     HIP_OK(hipGetDevice(&dev));
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_persist, kBlock, 0));
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kBlock, 0));
     return (long long)std::max(1, cus) * std::max(1, blocks) * kBlock;
 }
+
+long long resident_persist_threads() { return resident_threads(k_persist); }
 
 Frame make_frame(const lambda_render_params &p) {
     Frame F{};  // every optional pointer (cost, train) starts null
@@ -1482,10 +1580,12 @@ struct Renderer {
 
     // Renders F into the FP64 framebuffer (added to it when clear is
     // false): one launch per bounce while at least `tail` paths are live,
-    // then one persistent launch.  With a guide, every pass runs the
-    // network on the paths at Lambertian vertices (no persistent launch).
+    // then one persistent launch.  With a guide: fused, one launch of the
+    // guided persistent kernel (the network inside it); otherwise every pass
+    // runs the network on the paths at guided vertices (no persistent
+    // launch).
     void run(const Frame &F, long long tail, volatile int32_t *progress, lambda_render_stats *stats,
-             bool clear = true, const GuideNet *guide = nullptr, GuideQueue *Q = nullptr) {
+             bool clear = true, const GuideNet *guide = nullptr, GuideQueue *Q = nullptr, bool fused = false) {
         const long long units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
         if (stats) {
             for (auto &e : ev) {
@@ -1502,8 +1602,8 @@ struct Renderer {
         if (clear) k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
         k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
         k_fill_d<<<1, kBlock, 0, st>>>(reinterpret_cast<double *>(P.rays), 2);
-        if (guide) tail = 0;
-        k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns);
+        if (guide) tail = fused ? kAutoTailPaths : 0;
+        k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns, 0);
         // The work counter starts after the slots' first units.
         k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrWork, ns);
         HIP_OK(hipGetLastError());
@@ -1524,7 +1624,12 @@ struct Renderer {
                 if (stats) HIP_OK(hipEventRecord(ev[0].b, st));
                 k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done, nullptr);
                 if (stats) HIP_OK(hipEventRecord(ev[3].a, st));
-                k_persist<<<grid_for((long long)n_closest + n_done), kBlock, 0, st>>>(g.ds, F, P, n_closest, B.spill.p);
+                if (guide)
+                    k_persist_guided<<<grid_for((long long)n_closest + n_done), kBlock, 0, st>>>(
+                        g.ds, F, P, n_closest, B.spill.p, guide->dev());
+                else
+                    k_persist<<<grid_for((long long)n_closest + n_done), kBlock, 0, st>>>(g.ds, F, P, n_closest,
+                                                                                          B.spill.p);
                 if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
                 sync_counters();
                 if (stats) {
@@ -1575,6 +1680,97 @@ struct Renderer {
         }
     }
 
+    // Guided render whose network is used at the first F.guide_bounces
+    // vertices only: the work units go in batches of one per slot; a batch
+    // runs guided passes (k_trace, k_finalize, network, guided shading) while
+    // its live paths are within the guided bounces, then one persistent
+    // launch runs every remaining path of the batch to its end (no further
+    // units: the work counter stays at the end).
+    void run_guided_batches(const Frame &F, volatile int32_t *progress, lambda_render_stats *stats, bool clear,
+                            const GuideNet *guide, GuideQueue *Q) {
+        const long long units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
+        if (units > INT32_MAX / 2) throw Failure{LAMBDA_EINVAL, "too many work units for one guided render"};
+        if (stats) {
+            for (auto &e : ev) {
+                if (!e.a) HIP_OK(hipEventCreate(&e.a));
+                if (!e.b) HIP_OK(hipEventCreate(&e.b));
+            }
+        }
+        auto timed = [&](int k) {
+            float ms = 0;
+            HIP_OK(hipEventElapsedTime(&ms, ev[k].a, ev[k].b));
+            stats->kernel_ms[k] += ms;
+            stats->launches[k]++;
+        };
+        if (clear) k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
+        k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
+        k_fill_d<<<1, kBlock, 0, st>>>(reinterpret_cast<double *>(P.rays), 2);
+        // no unit is started after a batch's first ones
+        k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrWork, int(units));
+        HIP_OK(hipGetLastError());
+        for (long long base = 0; base < units; base += ns) {
+            const int nb = int(std::min<long long>(ns, units - base));
+            k_start<<<grid_for(nb), kBlock, 0, st>>>(g.ds, F, P, nb, base);
+            int n_closest = nb, n_shadow = 0, n_done = 0;
+            for (int pass = 0; pass < F.guide_bounces && n_closest > 0; pass++) {
+                if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
+                k_trace<<<grid_for((long long)n_closest + n_shadow), kBlock, 0, st>>>(g.ds, P, n_closest, n_shadow,
+                                                                                       B.spill.p, nullptr);
+                if (stats) {
+                    HIP_OK(hipEventRecord(ev[0].b, st));
+                    HIP_OK(hipEventRecord(ev[2].a, st));
+                }
+                k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done, nullptr);
+                if (stats) {
+                    HIP_OK(hipEventRecord(ev[2].b, st));
+                    HIP_OK(hipEventRecord(ev[1].a, st));
+                }
+                k_shade_guided_queue<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, n_closest, *Q);
+                guide->forward_dev(Q->feat, P.ctr + kCtrGuided, n_closest, Q->out, st);
+                k_shade_guided<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, *Q);
+                if (stats) HIP_OK(hipEventRecord(ev[1].b, st));
+                sync_counters();
+                if (stats) {
+                    stats->iterations++;
+                    stats->closest_rays += n_closest;
+                    stats->shadow_rays += n_shadow;
+                    for (int k = 0; k < 3; k++) timed(k);
+                }
+                n_closest = host_ctr[kCtrNext];
+                n_shadow = host_ctr[kCtrShadow];
+                n_done = host_ctr[kCtrDone];
+                std::swap(P.q_in, P.q_out);
+            }
+            // settle pending occlusion rays and ended samples, then run the
+            // batch's live paths to their ends
+            if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
+            k_trace<<<grid_for(n_shadow), kBlock, 0, st>>>(g.ds, P, 0, n_shadow, B.spill.p, nullptr);
+            if (stats) HIP_OK(hipEventRecord(ev[0].b, st));
+            k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done, nullptr);
+            if (stats) HIP_OK(hipEventRecord(ev[3].a, st));
+            if (n_closest + n_done > 0)
+                k_persist<<<grid_for((long long)n_closest + n_done), kBlock, 0, st>>>(g.ds, F, P, n_closest,
+                                                                                      B.spill.p);
+            if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
+            sync_counters();
+            if (stats) {
+                stats->shadow_rays += n_shadow;
+                timed(0);
+                timed(3);
+            }
+            if (progress) {
+                const long long done = host_ctr[kCtrPixels];
+                *progress = int32_t(std::min<long long>(done * F.npix / std::max<long long>(1, units), F.npix));
+            }
+        }
+        if (stats) {
+            unsigned long long r[2];
+            HIP_OK(hipMemcpy(r, P.rays, sizeof r, hipMemcpyDeviceToHost));
+            stats->closest_rays += (long long)r[0];
+            stats->shadow_rays += (long long)r[1];
+        }
+    }
+
     // The pass schedule with the bounce loop captured in a hipGraph: a graph
     // of `passes` (even) passes -- each k_pass_begin, k_trace, k_finalize,
     // k_shade with queue lengths read on the device and grids sized for the
@@ -1588,7 +1784,7 @@ struct Renderer {
         k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
         k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
         k_fill_d<<<1, kBlock, 0, st>>>(reinterpret_cast<double *>(P.rays), 2);
-        k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns);
+        k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns, 0);
         k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrWork, ns);
         k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrNext, ns);  // every slot starts in the closest-hit queue
         HIP_OK(hipGetLastError());
@@ -1636,12 +1832,12 @@ struct Renderer {
     // decreasing order of their summed count (longest first, so the last
     // pixels to start are short), pixels within a tile in row order.  The order changes when a pixel
     // is rendered, never what it is rendered with.
-    void order_by_cost(const Frame &F) {
+    void order_by_cost(const Frame &F, int root = 1) {
         Frame C = F;
         C.mode = LAMBDA_RNG_THROUGHPUT;
-        C.spp = 1;
-        C.sqrt_spp = 1;
-        C.total_work = F.npix;
+        C.spp = root * root;
+        C.sqrt_spp = root;
+        C.total_work = (long long)F.npix * C.spp;
         C.seed = F.seed ^ 0x9e3779b97f4a7c15ull;
         DevBuf<int> cost;
         cost.alloc(size_t(F.npix));
@@ -1697,7 +1893,11 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
         stats->path_slots = R.ns;
     }
     const bool by_cost = p.cost_order > 0 || (p.cost_order == 0 && F.mode == LAMBDA_RNG_REFERENCE && R.ns < F.npix);
-    if (by_cost) R.order_by_cost(F);
+    if (by_cost) {
+        // cost_order > 1: samples of the cost pre-pass (rounded down to a square)
+        const int root = std::max(1, int(std::sqrt(double(std::max(1, std::min(p.cost_order, 64))))));
+        R.order_by_cost(F, root);
+    }
     if (p.graph_passes > 0) R.run_graph(F, p.graph_passes, progress, stats);
     else R.run(F, tail, progress, stats);
     k_resolve<<<grid_for(F.npix), kBlock, 0, R.st>>>(F, R.P.fb, R.B.rgb.p);
@@ -1715,6 +1915,11 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
 // with fresh PCG32 streams to one FP64 framebuffer; with a guide, Lambertian
 // vertices sample the guiding mixture and each step can train the network on
 // the radiance its own paths gathered.
+// Guided vertices per sample when a session does not set them: the first
+// four bounces, after which the network's gain no longer pays for the
+// guided passes.
+constexpr int kDefaultGuidedBounces = 4;
+
 struct lambda_hip_progressive {
     lambda_hip_scene *scene = nullptr;
     lambda_render_params params{};
@@ -1729,6 +1934,8 @@ struct lambda_hip_progressive {
     int train_cap = 1 << 18;
     int batch = 1 << 14;
     int epochs = 1;
+    int guide_bounces = kDefaultGuidedBounces;
+    bool fused = false;          // guided: network inside the persistent kernel
     double seen_per_sample = 0;  // training candidates per sample per pixel, last step
     DevBuf<double> scratch_fb;   // framebuffer of steps that are not kept
 };
@@ -1748,6 +1955,7 @@ Frame progressive_frame(const lambda_hip_progressive &pr, int samples, int root)
         F.train = pr.train.p;
         F.train_cap = pr.train_cap;
         F.grec_stride = pr.params.depth + 1;
+        F.guide_bounces = pr.guide_bounces;
         // expected candidates from the last step (first step: 4 per path)
         const double expect = (pr.seen_per_sample > 0 ? pr.seen_per_sample : 4.0 * F.npix) * samples;
         F.train_keep = float(std::min(1.0, 1.1 * pr.train_cap / std::max(1.0, expect)));
@@ -2131,8 +2339,13 @@ int lambda_hip_progressive_create(lambda_hip_scene *scene, const lambda_render_p
         pr->params = p;
         pr->net = guide ? &guide->net : nullptr;
         const Frame F = make_frame(p);
+        // guided: the network inside the persistent kernel unless the
+        // caller asks for one launch per bounce (tail_paths -1)
+        pr->fused = guide && params->tail_paths != -1;
         long long slots = params->path_slots > 0 ? params->path_slots
-                          : guide ? (1LL << 20) : resident_persist_threads();
+                          : pr->fused ? resident_threads(k_persist_guided)
+                          : guide     ? (1LL << 20)
+                                      : resident_persist_threads();
         slots = std::max<long long>(1, std::min<long long>(slots, 1LL << 26));
         pr->R = std::make_unique<Renderer>(*scene, F, int(slots));
         if (guide) {
@@ -2170,8 +2383,13 @@ int lambda_hip_progressive_step(lambda_hip_progressive *pr, int32_t samples, int
             R.P.fb = pr->scratch_fb.p;
         }
         HIP_OK(hipEventRecord(ev.a, R.st));
-        R.run(F, pr->net ? 0 : (1LL << 40), nullptr, nullptr, !keep || pr->total == 0, pr->net,
-              pr->net ? &pr->Q : nullptr);
+        const bool clear = !keep || pr->total == 0;
+        if (pr->net && pr->fused)
+            R.run(F, kAutoTailPaths, nullptr, nullptr, clear, pr->net, &pr->Q, true);
+        else if (pr->net && pr->guide_bounces <= pr->params.depth)
+            R.run_guided_batches(F, nullptr, nullptr, clear, pr->net, &pr->Q);
+        else
+            R.run(F, pr->net ? 0 : kAutoTailPaths, nullptr, nullptr, clear, pr->net, pr->net ? &pr->Q : nullptr);
         HIP_OK(hipEventRecord(ev.b, R.st));
         R.P.fb = fb;
         HIP_OK(hipEventSynchronize(ev.b));
@@ -2219,12 +2437,14 @@ int lambda_hip_progressive_image(lambda_hip_progressive *pr, float *out_rgb, cha
 }
 
 int lambda_hip_progressive_set_training(lambda_hip_progressive *pr, int32_t epochs, int32_t batch, float lr,
-                                        char *err, size_t err_len) {
+                                        int32_t guided_bounces, char *err, size_t err_len) {
     return guarded(err, err_len, [&] {
         if (!pr || epochs < 1 || batch < kGuideRowsPerBlock || batch > pr->train_cap || !(lr > 0))
             throw Failure{LAMBDA_EINVAL, "invalid training settings"};
         pr->epochs = epochs;
         pr->batch = batch;
+        if (guided_bounces < 0) throw Failure{LAMBDA_EINVAL, "invalid training settings"};
+        pr->guide_bounces = guided_bounces > 0 ? guided_bounces : kDefaultGuidedBounces;
         if (pr->net) pr->net->lr = lr;
     });
 }

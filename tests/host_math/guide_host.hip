@@ -2,9 +2,12 @@
 // density, sampling, loss gradient) compiled for the host.
 //
 // argv: encode   <features.f32 [n][6]>            <out.f32 [n][96]>
+//       encode2  <features.f32 [n][6]>            <out.f32 [n][96]>  (by halves)
 //       pdf      <outputs.f32 [n][40]> <dirs.f32 [n][3]> <out.f32 [n]>
+//       pdfc     (as pdf, through the compact form)
 //       grad     <outputs.f32 [n][40]> <dirs.f32 [n][3]> <c.f32 [n]> <out.f32 [n][41] = loss, d[40]>
 //       sample   <outputs.f32 [40]> <count> <seed>   <out.f32 [count][3]>
+//       samplec  (as sample, through the compact form)
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -44,14 +47,35 @@ int main(int argc, char **argv) {
                 for (int k = 0; k < 16; k++) out[r * kGuideIn + s * 16 + k] = v[k];
             }
         write_f32(argv[3], out);
-    } else if (mode == "pdf" && argc >= 5) {
+    } else if (mode == "encode2") {
+        // the inference kernel's encoding: each half of a scalar's bands
+        // from its own sincos
+        std::vector<float> f = read_f32(argv[2]);
+        const size_t n = f.size() / kGuideScalars;
+        std::vector<float> out(n * kGuideIn);
+        for (size_t r = 0; r < n; r++)
+            for (int s = 0; s < kGuideScalars; s++)
+                for (int h = 0; h < 2; h++) {
+                    float v[8];
+                    guide_encode_half(f[r * kGuideScalars + s], h, v);
+                    for (int k = 0; k < 8; k++) out[r * kGuideIn + s * 16 + 8 * h + k] = v[k];
+                }
+        write_f32(argv[3], out);
+    } else if ((mode == "pdf" || mode == "pdfc") && argc >= 5) {
         std::vector<float> o = read_f32(argv[2]), w = read_f32(argv[3]);
         const size_t n = w.size() / 3;
         std::vector<float> out(n);
         for (size_t r = 0; r < n; r++) {
-            Lobe L[kLobes];
-            guide_lobes(&o[r * kLobeOut], L);
-            out[r] = guide_pdf(L, v3(w[3 * r], w[3 * r + 1], w[3 * r + 2]));
+            const V3 dir = v3(w[3 * r], w[3 * r + 1], w[3 * r + 2]);
+            if (mode == "pdf") {
+                Lobe L[kLobes];
+                guide_lobes(&o[r * kLobeOut], L);
+                out[r] = guide_pdf(L, dir);
+            } else {
+                float c[kLobeOut];
+                guide_compact(&o[r * kLobeOut], c);
+                out[r] = guide_pdf_c(c, dir);
+            }
         }
         write_f32(argv[4], out);
     } else if (mode == "grad" && argc >= 6) {
@@ -64,16 +88,18 @@ int main(int argc, char **argv) {
             std::memcpy(&out[r * (kLobeOut + 1) + 1], d, sizeof d);
         }
         write_f32(argv[5], out);
-    } else if (mode == "sample" && argc >= 6) {
+    } else if ((mode == "sample" || mode == "samplec") && argc >= 6) {
         std::vector<float> o = read_f32(argv[2]);
         const int count = std::atoi(argv[3]);
         Pcg rng = pcg_seed(uint64_t(std::atoll(argv[4])), 7);
         Lobe L[kLobes];
         guide_lobes(o.data(), L);
+        float c[kLobeOut];
+        guide_compact(o.data(), c);
         std::vector<float> out(size_t(count) * 3);
         for (int i = 0; i < count; i++) {
             float u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
-            V3 d = guide_sample(L, u0, u1, u2);
+            V3 d = mode == "sample" ? guide_sample(L, u0, u1, u2) : guide_sample_c(c, u0, u1, u2);
             out[3 * size_t(i)] = d.x;
             out[3 * size_t(i) + 1] = d.y;
             out[3 * size_t(i) + 2] = d.z;

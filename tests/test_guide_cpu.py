@@ -132,6 +132,10 @@ def test_device_encoding_matches_oracle(driver, tmp_path):
     # one sincos and 7 double-angle steps carry ~1e-5 of rounding; the
     # network stores its input in bfloat16 (8 significant bits) anyway
     np.testing.assert_allclose(got, go.encode(f), atol=1e-4)
+    # by halves (the inference kernel): bands 4..7 start from their own sincos
+    got2 = run(driver, tmp_path, ["encode2"], [f], (500, 96))
+    np.testing.assert_allclose(got2, go.encode(f), atol=1e-4)
+    assert np.array_equal(got2[:, :8], got[:, :8])
 
 
 def test_device_pdf_and_gradient_match_oracle(driver, tmp_path):
@@ -143,20 +147,46 @@ def test_device_pdf_and_gradient_match_oracle(driver, tmp_path):
     c = rng.uniform(0.1, 3, n).astype(np.float32)
     q = run(driver, tmp_path, ["pdf"], [o, w], (n,))
     np.testing.assert_allclose(q, go.pdf(o.astype(np.float64), w.astype(np.float64)), rtol=2e-4)
+    # the compact form the renderer shades with
+    qc = run(driver, tmp_path, ["pdfc"], [o, w], (n,))
+    np.testing.assert_allclose(qc, go.pdf(o.astype(np.float64), w.astype(np.float64)), rtol=2e-4)
     got = run(driver, tmp_path, ["grad"], [o, w, c], (n, 41))
     lref, dref = go.loss_grad(o.astype(np.float64), w.astype(np.float64), c.astype(np.float64))
     np.testing.assert_allclose(got[:, 0], lref, rtol=1e-4, atol=1e-4)
     np.testing.assert_allclose(got[:, 1:], dref, rtol=2e-3, atol=2e-3 * np.abs(dref).max())
 
 
-def test_device_sampling_follows_pdf(driver, tmp_path):
+def test_compact_pdf_at_high_concentration(driver, tmp_path):
+    """kappa up to the 2000 clamp: the compact form's exponent c + kappa mu.w
+    cancels as the lobe form's kappa (mu.w - 1) does; directions within a few
+    lobe widths of each mean."""
+    rng = np.random.default_rng(3)
+    n = 400
+    o = rng.normal(size=(n, go.N_OUT)).astype(np.float32)
+    o[:, 3::5] = rng.uniform(50, 2500, (n, 8))
+    k = rng.integers(0, 8, n)
+    mu = o[np.arange(n)[:, None], 5 * k[:, None] + np.arange(3)].astype(np.float64)
+    mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+    w = mu + rng.normal(scale=0.02, size=(n, 3))
+    w = (w / np.linalg.norm(w, axis=1, keepdims=True)).astype(np.float32)
+    ref = go.pdf(o.astype(np.float64), w.astype(np.float64))
+    ok = ref > 1e-3
+    assert ok.sum() > n // 2
+    for mode in ("pdf", "pdfc"):
+        q = run(driver, tmp_path, [mode], [o, w], (n,))
+        np.testing.assert_allclose(q[ok], ref[ok], rtol=2e-3)
+
+
+@pytest.mark.parametrize("mode", ["sample", "samplec"])
+def test_device_sampling_follows_pdf(driver, tmp_path, mode):
     """Histogram of 400k mixture samples over 12 x 24 sphere cells against
-    the cells' integrated density (oracle quadrature)."""
+    the cells' integrated density (oracle quadrature); the lobe form and the
+    compact form the renderer samples with."""
     rng = np.random.default_rng(6)
     o = rng.normal(size=(1, go.N_OUT)).astype(np.float32)
     o[0, 3::5] = rng.uniform(0, 4, 8)
     count = 400000
-    d = run(driver, tmp_path, ["sample", str(count), "11"], [o], (count, 3)).astype(np.float64)
+    d = run(driver, tmp_path, [mode, str(count), "11"], [o], (count, 3)).astype(np.float64)
     assert np.all(np.abs(np.linalg.norm(d, axis=1) - 1) < 1e-5)
     nt, nphi = 12, 24
     th = np.arccos(np.clip(d[:, 1], -1, 1))

@@ -102,7 +102,7 @@ def test_trained_guide_samples_toward_the_gap(room):
         pt = np.array(pt)
         f = np.zeros((1, 6))
         f[0, :3] = (pt - lo) / (hi - lo)
-        f[0, 3] = 0.0                       # outgoing direction along the normal (theta = 0)
+        f[0, 3] = 0.0                       # direction feature: the floor's normal (theta = 0)
         f[0, 4] = 0.5
         f[0, 5] = (550 - 360) / 470
         out = g.forward(f.astype(np.float32)).astype(np.float64)
