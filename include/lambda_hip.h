@@ -289,7 +289,7 @@ int lambda_hip_thinfilm_reflectance(lambda_hip_scene *scene, int32_t material, c
  * over the next direction, whose parameters come from an MLP of (position,
  * direction, wavelength): each of 6 features in [0, 1] is encoded
  * as sin/cos of 2^b pi x, b = 0..7 (96 inputs), 4 hidden layers of 64 ReLU
- * units, 40 outputs = 8 x {mean xyz (normalised), kappa (softplus),
+ * units, 40 outputs = 8 x {mean xyz (normalised), kappa (exp, clamped to [1e-3, 2000]),
  * weight (softmax logit)}.  BF16 weights and activations on the matrix
  * cores with FP32 accumulation; FP32 master weights; Adam.
  *

@@ -7,8 +7,10 @@
 // output 64 values of which the first 40 are 8 lobes x {m.xyz, kappa_raw,
 // weight_logit}; the other 24 are padding kept at zero.
 //
-// Lobe k: mean mu_k = m_k / |m_k|, concentration kappa_k = softplus(kappa_raw)
-// clamped to [kKappaMin, kKappaMax], weight pi_k = softmax(weight_logit)_k.
+// Lobe k: mean mu_k = m_k / |m_k|, concentration kappa_k = exp(kappa_raw)
+// clamped to [kKappaMin, kKappaMax] (log scale: the concentrations a narrow
+// opening needs, hundreds, are a few Adam steps from the initial 1), weight
+// pi_k = softmax(weight_logit)_k.
 // Density of a lobe: vMF(w) = kappa / (2 pi (1 - exp(-2 kappa))) *
 // exp(kappa (mu.w - 1)), which tends to 1 / (4 pi) as kappa -> 0.
 #pragma once
@@ -102,8 +104,6 @@ struct Lobe {
     float kappa, norm, weight;  // norm = kappa / (2 pi (1 - exp(-2 kappa)))
 };
 
-LAM_FN float softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
-
 LAM_FN float vmf_norm(float kappa) { return kappa / (2 * kPi * -expm1f(-2 * kappa)); }
 
 // Lobes from the 40 raw outputs.
@@ -116,7 +116,7 @@ LAM_FN void guide_lobes(const float *o, Lobe (&L)[kLobes]) {
         V3 m = v3(q[0], q[1], q[2]);
         float l = fmaxf(len(m), 1e-6f);
         L[k].mu = m * (1 / l);
-        L[k].kappa = fminf(fmaxf(softplus(q[3]), kKappaMin), kKappaMax);
+        L[k].kappa = fminf(fmaxf(expf(q[3]), kKappaMin), kKappaMax);
         L[k].norm = vmf_norm(L[k].kappa);
         L[k].weight = expf(q[4] - mx);
         sum += L[k].weight;
@@ -220,8 +220,8 @@ LAM_FN float guide_loss_grad(const float *o, V3 w, float c, float *d) {
         const float kap = L[k].kappa;
         const float e = expf(-2 * kap);
         const float dlog_dk = 1 / kap - 2 * e / (-expm1f(-2 * kap)) + (dot(L[k].mu, w) - 1);
-        const float sp = softplus(r[3]);
-        const float dk_draw = (sp > kKappaMin && sp < kKappaMax) ? 1 / (1 + expf(-r[3])) : 0.0f;
+        const float ek = expf(r[3]);
+        const float dk_draw = (ek > kKappaMin && ek < kKappaMax) ? ek : 0.0f;
         g[3] = -c * resp * dlog_dk * dk_draw;
         // d log vMF / d mu = kappa w; mu = m / |m|: d/dm = (I - mu mu^T) / |m|
         V3 m = v3(r[0], r[1], r[2]);

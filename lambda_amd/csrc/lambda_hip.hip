@@ -1942,7 +1942,9 @@ struct lambda_hip_progressive {
 
 namespace {
 
-Frame progressive_frame(const lambda_hip_progressive &pr, int samples, int root) {
+// Frame of one step; guided vertices are recorded for training only in
+// steps that train.
+Frame progressive_frame(const lambda_hip_progressive &pr, int samples, int root, bool train) {
     Frame F = make_frame(pr.params);
     F.mode = LAMBDA_RNG_THROUGHPUT;
     F.spp = samples;
@@ -1951,11 +1953,11 @@ Frame progressive_frame(const lambda_hip_progressive &pr, int samples, int root)
     F.seed = pr.params.seed + 0x9e3779b97f4a7c15ull * uint64_t(pr.steps + 1);
     F.box_lo = make_float3(pr.scene->box[0], pr.scene->box[1], pr.scene->box[2]);
     F.box_hi = make_float3(pr.scene->box[3], pr.scene->box[4], pr.scene->box[5]);
-    if (pr.net) {
+    if (pr.net) F.guide_bounces = pr.guide_bounces;
+    if (pr.net && train) {
         F.train = pr.train.p;
         F.train_cap = pr.train_cap;
         F.grec_stride = pr.params.depth + 1;
-        F.guide_bounces = pr.guide_bounces;
         // expected candidates from the last step (first step: 4 per path)
         const double expect = (pr.seen_per_sample > 0 ? pr.seen_per_sample : 4.0 * F.npix) * samples;
         F.train_keep = float(std::min(1.0, 1.1 * pr.train_cap / std::max(1.0, expect)));
@@ -2369,13 +2371,13 @@ int lambda_hip_progressive_step(lambda_hip_progressive *pr, int32_t samples, int
         int root = int(std::lround(std::sqrt(double(samples))));
         if (root * root != samples) throw Failure{LAMBDA_EINVAL, "samples per step must be a perfect square"};
         Renderer &R = *pr->R;
-        const Frame F = progressive_frame(*pr, samples, root);
+        const bool train = (flags & LAMBDA_STEP_TRAIN) != 0;
+        const Frame F = progressive_frame(*pr, samples, root, train);
         if ((long long)F.npix * samples >= (1LL << 31) - (1LL << 28))
             throw Failure{LAMBDA_EINVAL, "too many samples in one step"};
         EventPair ev;
         HIP_OK(hipEventCreate(&ev.a));
         HIP_OK(hipEventCreate(&ev.b));
-        const bool train = (flags & LAMBDA_STEP_TRAIN) != 0;
         const bool keep = (flags & LAMBDA_STEP_DISCARD) == 0;
         double *fb = R.P.fb;
         if (!keep) {
@@ -2395,8 +2397,8 @@ int lambda_hip_progressive_step(lambda_hip_progressive *pr, int32_t samples, int
         HIP_OK(hipEventSynchronize(ev.b));
         float render_ms = 0;
         HIP_OK(hipEventElapsedTime(&render_ms, ev.a, ev.b));
-        const long long collected = pr->net ? std::min<long long>(R.host_ctr[kCtrTrain], pr->train_cap) : 0;
-        if (pr->net) pr->seen_per_sample = double(R.host_ctr[kCtrTrainSeen]) / samples;
+        const long long collected = pr->net && train ? std::min<long long>(R.host_ctr[kCtrTrain], pr->train_cap) : 0;
+        if (pr->net && train) pr->seen_per_sample = double(R.host_ctr[kCtrTrainSeen]) / samples;
         float train_ms = 0;
         double loss = 0;
         if (pr->net && train && collected >= pr->batch) {

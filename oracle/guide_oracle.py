@@ -82,8 +82,9 @@ def forward(params: np.ndarray, features: np.ndarray, bf: bool = False):
     raise AssertionError
 
 
-def softplus(x):
-    return np.where(x > 20, x, np.log1p(np.exp(np.minimum(x, 20))))
+def kappa_of(raw):
+    """Concentration from its raw output: exp, clamped (guide.h guide_lobes)."""
+    return np.exp(np.minimum(raw, 80.0))
 
 
 def lobes(o: np.ndarray):
@@ -92,7 +93,7 @@ def lobes(o: np.ndarray):
     m = q[:, :, :3]
     l = np.maximum(np.linalg.norm(m, axis=2), 1e-6)
     mu = m / l[:, :, None]
-    kappa = np.clip(softplus(q[:, :, 3]), KAPPA_MIN, KAPPA_MAX)
+    kappa = np.clip(kappa_of(q[:, :, 3]), KAPPA_MIN, KAPPA_MAX)
     logits = q[:, :, 4]
     wgt = np.exp(logits - logits.max(axis=1, keepdims=True))
     wgt /= wgt.sum(axis=1, keepdims=True)
@@ -124,8 +125,8 @@ def loss_grad(o: np.ndarray, w: np.ndarray, c: np.ndarray):
     e = np.exp(-2 * kappa)
     dlog_dk = 1 / kappa - 2 * e / (-np.expm1(-2 * kappa)) + (cosang - 1)
     raw = o[:, :N_OUT].reshape(-1, LOBES, 5)
-    sp = softplus(raw[:, :, 3])
-    dk_draw = np.where((sp > KAPPA_MIN) & (sp < KAPPA_MAX), 1 / (1 + np.exp(-raw[:, :, 3])), 0.0)
+    ek = kappa_of(raw[:, :, 3])
+    dk_draw = np.where((ek > KAPPA_MIN) & (ek < KAPPA_MAX), ek, 0.0)
     d[:, :, 3] = -c * resp * dlog_dk * dk_draw
     m = raw[:, :, :3]
     l = np.maximum(np.linalg.norm(m, axis=2), 1e-6)

@@ -97,7 +97,7 @@ def test_oracle_mixture_integrates_to_one(kscale):
 
 def test_oracle_small_kappa_is_uniform():
     o = np.zeros((3, go.N_OUT))
-    o[:, 3::5] = -30  # softplus -> below the clamp: kappa = 1e-3
+    o[:, 3::5] = -30  # exp -> below the clamp: kappa = 1e-3
     o[:, 0::5] = 1
     q = go.pdf(o, random_dirs(np.random.default_rng(0), 3))
     np.testing.assert_allclose(q, 1 / (4 * np.pi), rtol=2e-3)
@@ -163,7 +163,7 @@ def test_compact_pdf_at_high_concentration(driver, tmp_path):
     rng = np.random.default_rng(3)
     n = 400
     o = rng.normal(size=(n, go.N_OUT)).astype(np.float32)
-    o[:, 3::5] = rng.uniform(50, 2500, (n, 8))
+    o[:, 3::5] = rng.uniform(4, 8, (n, 8))          # kappa 55 .. 2000 (clamped)
     k = rng.integers(0, 8, n)
     mu = o[np.arange(n)[:, None], 5 * k[:, None] + np.arange(3)].astype(np.float64)
     mu /= np.linalg.norm(mu, axis=1, keepdims=True)

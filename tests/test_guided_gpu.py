@@ -72,8 +72,9 @@ def test_guided_estimator_is_unbiased(room):
     plain, _ = step_means(room, None, 16, 64, False)
     g = lambda_amd.Guide(seed=5)
     untrained, st = step_means(room, g, 16, 64, False)
+    assert st["train_samples"] == 0          # steps that do not train record nothing
+    trained, st = step_means(room, g, 16, 64, True)
     assert st["train_samples"] > 10000
-    trained, _ = step_means(room, g, 16, 64, True)
     g.close()
     for name, m in (("untrained", untrained), ("trained", trained)):
         se = np.sqrt(m.var(ddof=1) / len(m) + plain.var(ddof=1) / len(plain))
