@@ -184,12 +184,13 @@ typedef struct lambda_render_params {
     int32_t tail_paths;    /* once fewer paths than this are live, one persistent
                               launch runs each remaining path to its end instead
                               of one launch per bounce; 0 = automatic, -1 = never */
-    int32_t cost_order;    /* start 8x8 tiles in decreasing order of a cost
-                              estimate from a short pre-pass (the image does
-                              not change): 0 = automatic (on for reference
-                              streams with fewer slots than pixels), -1 = off,
-                              n >= 1 = on with a pre-pass of n samples per
-                              pixel (rounded down to a square, at most 64) */
+    int32_t cost_order;    /* reference streams, persistent schedule: render
+                              the first n samples of every pixel, timing each
+                              pixel, then the rest with the slowest 8x8 tiles
+                              first (the image does not change): 0 =
+                              automatic (n = 2 when there are fewer slots
+                              than pixels), -1 = off, n >= 1 = on with n
+                              samples first (at most samples - 1) */
     int32_t graph_passes;  /* > 0: one launch per bounce throughout, the bounce
                               loop captured in a hipGraph of this many passes
                               (rounded up to even) and replayed until done;

@@ -156,7 +156,13 @@ struct Frame {
     float lmin, lmax, lrange;
     uint64_t seed;
     long long total_work;     // npix * spp (throughput mode)
-    int *cost;                // when set: per-pixel sum of path segments
+    // Reference mode: this launch runs samples [sample_begin, sample_end) of
+    // each pixel (sample_end 0 = spp); a pixel that stops before spp saves
+    // its PCG32 state in rng_save, where a later launch resumes it.
+    int sample_begin, sample_end;
+    uint4 *rng_save;
+    int *cost;                // when set (persistent kernel): per-pixel device-clock ticks
+    unsigned long long *unit_t0;  // with cost: start time of each slot's unit
     float *train;             // when set: training samples [train_cap][10]
     int train_cap;
     float train_keep;         // probability of keeping a training candidate
@@ -232,13 +238,21 @@ __device__ __forceinline__ bool trace_ray(const DeviceScene &S, V3 o, V3 d, floa
     return lam::traverse<kAny, kRing>(S, o, d, best, hit_inst, hit_obj, skip_inst, skip_obj, stk);
 }
 
-// Reference mode: the pixel of rank `rank` in the visiting order starts its
-// first sample in slot s with PCG32 stream (seed, pixel index)
-// (render/renderer.h:47-49); the slot keeps the pixel for all its samples.
+// Reference mode: the pixel of rank `rank` in the visiting order starts
+// sample F.sample_begin in slot s, with PCG32 stream (seed, pixel index)
+// (render/renderer.h:47-49) from its start or from the state a previous
+// launch saved; the slot keeps the pixel for the launch's samples.
 __device__ __forceinline__ void start_pixel(const DeviceScene &S, const Frame &F, const Paths &P, int s, int rank) {
     const int pixel = P.order[rank];
-    Pcg rng = pcg_seed(F.seed, uint64_t(pixel));
-    camera_sample(S, F, P, s, pixel, 0, rng);
+    Pcg rng;
+    if (F.sample_begin > 0) {
+        const uint4 v = F.rng_save[pixel];
+        rng.state = uint64_t(v.x) | (uint64_t(v.y) << 32);
+        rng.inc = uint64_t(v.z) | (uint64_t(v.w) << 32);
+    } else {
+        rng = pcg_seed(F.seed, uint64_t(pixel));
+    }
+    camera_sample(S, F, P, s, pixel, F.sample_begin, rng);
     store_rng(P, s, rng);
 }
 
@@ -743,9 +757,9 @@ __device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c) {
 // Adds the ended sample of slot s to its pixel's XYZ sum.  In reference
 // mode the pixel's next sample then starts in the same slot, continuing the
 // pixel's PCG32 stream (render/renderer.h:69-91); returns true when it did.
+// A pixel that reaches the launch's last sample before spp saves its stream.
 __device__ __forceinline__ bool retire_sample(const DeviceScene &S, const Frame &F, const Paths &P, int s) {
     int4 info = P.info[s];
-    if (F.cost) atomicAdd(F.cost + info.x, info.z);
     if (F.train) {
         // training samples from the sample's guided vertices: the radiance
         // that reached vertex b along its sampled direction is the radiance
@@ -779,11 +793,15 @@ __device__ __forceinline__ bool retire_sample(const DeviceScene &S, const Frame 
     unsafeAtomicAdd(px + 0, double(x));
     unsafeAtomicAdd(px + 1, double(y));
     unsafeAtomicAdd(px + 2, double(z));
-    if (F.mode == LAMBDA_RNG_REFERENCE && info.y + 1 < F.spp) {
+    if (F.mode == LAMBDA_RNG_REFERENCE && info.y + 1 < (F.sample_end ? F.sample_end : F.spp)) {
         Pcg rng = load_rng(P, s);
         camera_sample(S, F, P, s, info.x, info.y + 1, rng);
         store_rng(P, s, rng);
         return true;
+    }
+    if (F.mode == LAMBDA_RNG_REFERENCE && info.y + 1 < F.spp) {
+        F.rng_save[info.x] = P.rng[s];  // paused: a later launch resumes the stream here
+        return false;
     }
     atomicAdd(&P.ctr[kCtrPixels], 1);
     return false;
@@ -836,6 +854,7 @@ __global__ void __launch_bounds__(kBlock, kPersistWaves) k_persist(DeviceScene S
     const int s = t < n_in ? P.q_in[t] : P.q_out[t - n_in];
     Stack stk = make_stack(s_code, s_dist, spill);
     unsigned n_closest = 0, n_shadow = 0;
+    if (F.cost) F.unit_t0[s] = wall_clock64();
     while (true) {
         n_closest++;
         {
@@ -865,6 +884,13 @@ __global__ void __launch_bounds__(kBlock, kPersistWaves) k_persist(DeviceScene S
         }
         if (!ended) continue;
         if (retire_sample(S, F, P, s)) continue;
+        if (F.cost) {
+            // device-clock time of the unit (its pixel's samples in this launch)
+            const unsigned long long now = wall_clock64();
+            const unsigned long long dt = now - F.unit_t0[s];
+            atomicAdd(F.cost + P.info[s].x, int(dt < (1ull << 30) ? dt : (1ull << 30)));
+            F.unit_t0[s] = now;
+        }
         const int w = atomicAdd(&P.ctr[kCtrWork], 1);
         if (w >= work_units(F)) break;
         start_unit(S, F, P, s, w);
@@ -1538,6 +1564,31 @@ Frame make_frame(const lambda_render_params &p) {
 }
 
 // Device buffers, stream and counters of one render.
+// Visiting order from measured pixel costs: 8x8 tiles (the unit of the
+// default order, which keeps neighbouring paths in flight together) are
+// summed on the device, sorted on the host (longest first) and expanded back
+// to pixels, row order within a tile.  The order changes when a pixel runs,
+// never what it computes.
+__global__ void k_tile_cost(const int *cost, int w, int h, long long *tile_cost) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= w * h) return;
+    const int tx = (p % w) >> 3, ty = (p / w) >> 3;
+    atomicAdd(reinterpret_cast<unsigned long long *>(tile_cost + size_t(ty) * ((w + 7) >> 3) + tx),
+              (unsigned long long)cost[p]);
+}
+
+// Thread (i, k): pixel k of the tile at position i of the sorted list.
+__global__ void k_tile_expand(const int *tiles, const int *offsets, int ntiles, int w, int h, int *order) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = int(e >> 6), k = int(e & 63);
+    if (i >= ntiles) return;
+    const int t = tiles[i], tiles_x = (w + 7) >> 3;
+    const int tx = t % tiles_x, ty = t / tiles_x;
+    const int tw = min(8, w - 8 * tx), th = min(8, h - 8 * ty);
+    if (k >= tw * th) return;
+    order[offsets[i] + k] = (8 * ty + k / tw) * w + 8 * tx + k % tw;
+}
+
 struct Renderer {
     lambda_hip_scene &g;
     RenderBuffers B;
@@ -1827,49 +1878,64 @@ struct Renderer {
         }
     }
 
-    // Visiting order by cost: a one-sample render with throughput streams
-    // counts each pixel's path segments; 8x8 tiles are then started in
-    // decreasing order of their summed count (longest first, so the last
-    // pixels to start are short), pixels within a tile in row order.  The order changes when a pixel
-    // is rendered, never what it is rendered with.
-    void order_by_cost(const Frame &F, int root = 1) {
-        Frame C = F;
-        C.mode = LAMBDA_RNG_THROUGHPUT;
-        C.spp = root * root;
-        C.sqrt_spp = root;
-        C.total_work = (long long)F.npix * C.spp;
-        C.seed = F.seed ^ 0x9e3779b97f4a7c15ull;
-        DevBuf<int> cost;
+    // Reference streams in two launches: the first `first` samples of every
+    // pixel in tile order, timing each pixel on the device clock and saving
+    // its stream; then the remaining samples with the tiles in decreasing
+    // order of their time, so that the longest sample chains start first and
+    // the launch's drain is short.  The image is the same as one launch's:
+    // each pixel's samples still run in order on its own stream.
+    void run_cost_ordered(const Frame &F, int first, long long tail, volatile int32_t *progress,
+                          lambda_render_stats *stats) {
+        const int tiles_x = (F.width + 7) / 8, tiles_y = (F.height + 7) / 8, ntiles = tiles_x * tiles_y;
+        DevBuf<int> cost, order, tile_ids, offsets;
+        DevBuf<long long> tile_cost;
+        DevBuf<unsigned long long> t0;
+        DevBuf<uint4> save;
         cost.alloc(size_t(F.npix));
+        order.alloc(size_t(F.npix));
+        tile_cost.alloc(size_t(ntiles));
+        tile_ids.alloc(size_t(ntiles));
+        offsets.alloc(size_t(ntiles));
+        t0.alloc(size_t(ns));
+        save.alloc(size_t(F.npix));
         k_fill_i<<<grid_for(F.npix), kBlock, 0, st>>>(cost.p, F.npix);
-        C.cost = cost.p;
-        run(C, 1LL << 40, nullptr, nullptr);
-        std::vector<int> c(size_t(F.npix));
-        HIP_OK(hipMemcpyAsync(c.data(), cost.p, c.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+        k_fill_d<<<grid_for(ntiles), kBlock, 0, st>>>(reinterpret_cast<double *>(tile_cost.p), size_t(ntiles));
+        Frame F1 = F;
+        F1.sample_end = first;
+        F1.cost = cost.p;
+        F1.unit_t0 = t0.p;
+        F1.rng_save = save.p;
+        run(F1, tail, nullptr, stats, true);
+        k_tile_cost<<<grid_for(F.npix), kBlock, 0, st>>>(cost.p, F.width, F.height, tile_cost.p);
+        std::vector<long long> tc(static_cast<size_t>(ntiles));
+        HIP_OK(hipMemcpyAsync(tc.data(), tile_cost.p, tc.size() * sizeof(long long), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
-        // tiles of 8x8 pixels (the visiting order's unit), by summed cost
-        const std::vector<int> tiles = tile_order(F.width, F.height);
-        std::vector<std::pair<long long, int>> tc;  // (-cost, first index in tiles)
-        for (size_t i = 0; i < tiles.size();) {
-            const int tx = (tiles[i] % F.width) / 8, ty = (tiles[i] / F.width) / 8;
-            size_t j = i;
-            long long sum = 0;
-            while (j < tiles.size() && (tiles[j] % F.width) / 8 == tx && (tiles[j] / F.width) / 8 == ty)
-                sum += c[size_t(tiles[j++])];
-            tc.emplace_back(-sum, int(i));
-            i = j;
+        std::vector<int> ids(static_cast<size_t>(ntiles)), offs(static_cast<size_t>(ntiles));
+        for (int t = 0; t < ntiles; t++) ids[size_t(t)] = t;
+        std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return tc[size_t(a)] > tc[size_t(b)]; });
+        int pos = 0;
+        for (int i = 0; i < ntiles; i++) {
+            const int t = ids[size_t(i)];
+            offs[size_t(i)] = pos;
+            pos += std::min(8, F.width - 8 * (t % tiles_x)) * std::min(8, F.height - 8 * (t / tiles_x));
         }
-        std::stable_sort(tc.begin(), tc.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
-        std::vector<int> order;
-        order.reserve(tiles.size());
-        for (const auto &t : tc) {
-            const int tx = (tiles[size_t(t.second)] % F.width) / 8, ty = (tiles[size_t(t.second)] / F.width) / 8;
-            for (size_t j = size_t(t.second);
-                 j < tiles.size() && (tiles[j] % F.width) / 8 == tx && (tiles[j] / F.width) / 8 == ty; j++)
-                order.push_back(tiles[j]);
+        HIP_OK(hipMemcpyAsync(tile_ids.p, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(offsets.p, offs.data(), offs.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        k_tile_expand<<<grid_for((long long)ntiles * 64), kBlock, 0, st>>>(tile_ids.p, offsets.p, ntiles, F.width,
+                                                                           F.height, order.p);
+        HIP_OK(hipGetLastError());
+        Frame F2 = F;
+        F2.sample_begin = first;
+        F2.rng_save = save.p;
+        const int *tiles = P.order;
+        P.order = order.p;
+        try {
+            run(F2, tail, progress, stats, false);
+        } catch (...) {
+            P.order = tiles;
+            throw;
         }
-        HIP_OK(hipMemcpyAsync(B.order.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, st));
-        HIP_OK(hipStreamSynchronize(st));
+        P.order = tiles;
     }
 };
 
@@ -1892,13 +1958,13 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
         std::memset(stats, 0, sizeof(*stats));
         stats->path_slots = R.ns;
     }
-    const bool by_cost = p.cost_order > 0 || (p.cost_order == 0 && F.mode == LAMBDA_RNG_REFERENCE && R.ns < F.npix);
-    if (by_cost) {
-        // cost_order > 1: samples of the cost pre-pass (rounded down to a square)
-        const int root = std::max(1, int(std::sqrt(double(std::max(1, std::min(p.cost_order, 64))))));
-        R.order_by_cost(F, root);
-    }
-    if (p.graph_passes > 0) R.run_graph(F, p.graph_passes, progress, stats);
+    // reference streams, persistent schedule, more pixels than slots: the
+    // two-launch cost order (cost_order samples first; 0 = automatic: 2, the
+    // best measured on the 1080p instanced-mesh frame)
+    const bool by_cost = F.mode == LAMBDA_RNG_REFERENCE && F.spp > 1 && p.graph_passes <= 0 &&
+                         tail > R.ns && (p.cost_order > 0 || (p.cost_order == 0 && R.ns < F.npix));
+    if (by_cost) R.run_cost_ordered(F, std::min(p.cost_order > 0 ? p.cost_order : 2, F.spp - 1), tail, progress, stats);
+    else if (p.graph_passes > 0) R.run_graph(F, p.graph_passes, progress, stats);
     else R.run(F, tail, progress, stats);
     k_resolve<<<grid_for(F.npix), kBlock, 0, R.st>>>(F, R.P.fb, R.B.rgb.p);
     HIP_OK(hipGetLastError());
