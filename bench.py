@@ -91,7 +91,7 @@ def cpu_baseline(scene, seconds: float, threads: int) -> dict:
 
 
 def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, warmup: int = 0,
-                   train_steps: int = 1 << 30, epochs: int = 1, batch: int = 1 << 14):
+                   train_steps: int = 1 << 30, epochs: int = 1, batch: int = 1 << 14, guided_bounces: int = 0):
     """Progressive render until relMSE against ref <= target; returns the
     render-and-train wall time at which the target was crossed (log-linear
     interpolation between steps), the samples per pixel then, and the curve.
@@ -101,7 +101,7 @@ def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, 
     import lambda_amd
     p = lambda_amd.Progressive(dev, guide)
     if guide is not None:
-        p.set_training(epochs, batch, 1e-3)
+        p.set_training(epochs, batch, 1e-3, guided_bounces)
     elapsed, spp, curve = 0.0, 0, []
     prev = None
     hit = None
@@ -157,7 +157,8 @@ def guided_leg(args, scene_dir: str) -> dict:
     guide = lambda_amd.Guide(seed=args.guided_seed)
     guided, guided_curve = time_to_target(dev, guide, ref, 1e-3, args.guided_step, args.guided_max_spp,
                                           warmup=args.guided_warmup, train_steps=args.guided_train_steps,
-                                          epochs=args.guided_epochs, batch=args.guided_batch)
+                                          epochs=args.guided_epochs, batch=args.guided_batch,
+                                          guided_bounces=args.guided_bounces)
     guide.close()
     log("guided leg: guided reaches relMSE 1e-3 at %s" % (guided,))
     dev.close()
@@ -165,6 +166,8 @@ def guided_leg(args, scene_dir: str) -> dict:
            "reference_spp": args.guided_ref_spp, "target_rel_mse": 1e-3, "step_spp": args.guided_step,
            "guided_warmup_steps": args.guided_warmup, "guided_train_steps": min(args.guided_train_steps, 1 << 30),
            "guided_epochs": args.guided_epochs, "guided_batch": args.guided_batch, "adam_lr": 1e-3,
+           "guided_bounces": args.guided_bounces or 4,
+           "guided_schedule": "persistent kernel, network evaluated per wave on the matrix cores",
            "plain_s": plain[0] if plain else None, "plain_spp": plain[1] if plain else None,
            "guided_s": guided[0] if guided else None, "guided_spp": guided[1] if guided else None,
            "plain_curve": plain_curve[::max(1, len(plain_curve) // 12)],
@@ -280,10 +283,13 @@ def main() -> int:
     ap.add_argument("--guided-step", type=int, default=64)
     ap.add_argument("--guided-max-spp", type=int, default=65536)
     ap.add_argument("--guided-seed", type=int, default=1)
-    ap.add_argument("--guided-warmup", type=int, default=48, help="guided steps that train without adding samples")
-    ap.add_argument("--guided-train-steps", type=int, default=0, help="kept guided steps that also train")
+    ap.add_argument("--guided-warmup", type=int, default=0, help="guided steps that train without adding samples")
+    ap.add_argument("--guided-train-steps", type=int, default=48,
+                    help="first kept guided steps that also train (their samples are unbiased and count)")
     ap.add_argument("--guided-epochs", type=int, default=4, help="Adam epochs over each step's samples")
     ap.add_argument("--guided-batch", type=int, default=4096)
+    ap.add_argument("--guided-bounces", type=int, default=0,
+                    help="vertices guided per path (bounce index below this; 0 = the library default, 4)")
     ap.add_argument("--throughput-steps", type=int, default=2,
                     help="also time this many renders with throughput streams (0 to skip)")
     ap.add_argument("--stats-pixels", type=int, default=20000, help="pixels sampled to count traversal work")
