@@ -225,6 +225,15 @@ __device__ __forceinline__ void mlp_rows64(const GuideDev &G, const float *feat,
     }
 }
 
+// mlp_rows64 as a call of its own, for kernels whose other code is register
+// bound (the persistent guided kernel): the caller's live registers are
+// saved once around the call instead of the network's working set spilling
+// into the caller's loops.
+__device__ __attribute__((noinline)) void mlp_rows64_call(const GuideDev &G, const float *feat, int fstride,
+                                                          int nvalid, float *out, int ostride) {
+    mlp_rows64(G, feat, fstride, nvalid, out, ostride);
+}
+
 // Inference for n rows (n read from n_dev when given): 64 rows per wave,
 // registers only.  ostride must be a multiple of 4.
 __global__ void __launch_bounds__(256, 2) k_guide_forward(GuideDev G, const float *feat, int fstride, int n,

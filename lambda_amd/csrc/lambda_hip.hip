@@ -955,7 +955,7 @@ __global__ void __launch_bounds__(kBlock, kPersistGuidedWaves) k_persist_guided(
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             // reads the features in its first layer, writes [64][40] at the end
-            mlp_rows64(G, io, kGuideScalars, 64, io, kLobeOut);
+            mlp_rows64_call(G, io, kGuideScalars, 64, io, kLobeOut);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
