@@ -209,12 +209,14 @@ def serve_leg(args, mesh_dev, scene_dir: str) -> dict:
             "seed": args.serve_seed}
 
 
-# FLOP per row of the guiding network (2 per multiply-add): forward
-# 96x64 + 3 x 64x64 + 64x64 (output padded to 64), gradient = forward + the
-# backward data and weight products (2x the forward's, layer 0 without the
-# data product).
-GUIDE_FWD_FLOP = 2 * (96 * 64 + 4 * 64 * 64)
-GUIDE_GRAD_FLOP = GUIDE_FWD_FLOP + 2 * (2 * (96 * 64 + 4 * 64 * 64) - 96 * 64)
+# FLOP per row of the guiding network (2 per multiply-add).  Forward: 96x64
+# + 3 x 64x64 + 64x40 (the 40 outputs used; the inference kernel computes 48
+# output columns, 43,008 FLOP on the matrix cores).  Gradient, as computed
+# (output padded to 64): the padded forward + the backward data and weight
+# products (2x the forward's, layer 0 without the data product).
+GUIDE_FWD_FLOP = 2 * (96 * 64 + 3 * 64 * 64 + 64 * 40)
+GUIDE_PADDED_FWD_FLOP = 2 * (96 * 64 + 4 * 64 * 64)
+GUIDE_GRAD_FLOP = GUIDE_PADDED_FWD_FLOP + 2 * (2 * (96 * 64 + 4 * 64 * 64) - 96 * 64)
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
