@@ -322,8 +322,10 @@ int lambda_hip_guide_forward(lambda_guide *guide, const float *features, int32_t
  * the loss itself; the parameters do not change. */
 int lambda_hip_guide_gradient(lambda_guide *guide, const float *samples, int32_t n, float *grads, double *loss,
                               char *err, size_t err_len);
-/* Adam steps (rate lr) over consecutive batches of `batch` samples; *loss
- * receives the summed loss of the samples seen. */
+/* Adam steps (rate lr) over consecutive batches of `batch` samples, each
+ * sample's weight capped at 20 x the mean weight of the n samples (heavy-
+ * tailed radiance weights then cannot steer a whole step); *loss receives
+ * the summed loss of the samples seen. */
 int lambda_hip_guide_train(lambda_guide *guide, const float *samples, int32_t n, int32_t batch, float lr,
                            double *loss, char *err, size_t err_len);
 

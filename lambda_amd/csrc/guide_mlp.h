@@ -254,7 +254,7 @@ constexpr int kGuideSample = 10;
 // gradients dW_l = H_{l-1}^T dZ_l sum over the block's rows on the MFMA
 // (16 x 16 tiles shared among the block's waves).
 __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float *samples, int n, float *grads,
-                                                        double *loss) {
+                                                        double *loss, const float *c_cap) {
     __shared__ __attribute__((aligned(16))) __bf16 sX[kGuideRowsPerBlock * kLdX];
     __shared__ __attribute__((aligned(16))) __bf16 sH[4][kGuideRowsPerBlock * kLdH];   // H1..H4
     __shared__ __attribute__((aligned(16))) __bf16 sD[2][kGuideRowsPerBlock * kLdH];   // dZ ping-pong
@@ -287,7 +287,8 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
         for (int c = 0; c < kLobeOut; c++) d[c] = 0;
         if (row0 + r < n) {
             const float *sm = samples + size_t(row0 + r) * kGuideSample;
-            lsum = guide_loss_grad(sO + r * 64, v3(sm[6], sm[7], sm[8]), sm[9], d);
+            const float c = c_cap ? fminf(sm[9], *c_cap) : sm[9];
+            lsum = guide_loss_grad(sO + r * 64, v3(sm[6], sm[7], sm[8]), c, d);
         }
 #pragma unroll
         for (int c = 0; c < 64; c++) sD[0][r * kLdH + c] = __bf16(c < kLobeOut ? d[c] : 0.0f);
@@ -397,9 +398,30 @@ __global__ void k_guide_pack(const float *params, __bf16 *w_nk, __bf16 *w_kn) {
     }
 }
 
+// Training weights are single-path radiance estimates with heavy tails; a
+// training call caps each at kWeightCap times their mean over the samples it
+// trains on, so that one bright path does not steer a whole Adam step.  The
+// cap shapes the learned proposal only: the renderer's estimator weighs
+// every sample by its own density.
+constexpr float kWeightCap = 20.0f;
+
+// *cap = factor x the mean weight of samples [0, n) (one block).
+__global__ void __launch_bounds__(1024) k_weight_cap(const float *samples, int n, float factor, float *cap) {
+    __shared__ double part[1024];
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) s += double(samples[size_t(i) * kGuideSample + 9]);
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (int(threadIdx.x) < off) part[threadIdx.x] += part[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *cap = float(factor * part[0] / double(n > 0 ? n : 1));
+}
+
 // Host side of the network: parameters, Adam state and the BF16 copies.
 struct GuideNet {
-    DevBuf<float> params, grads, m, v;
+    DevBuf<float> params, grads, m, v, cap;
     DevBuf<__bf16> w_nk, w_kn;
     DevBuf<double> loss;
     int step = 0;
@@ -415,6 +437,7 @@ struct GuideNet {
         w_nk.alloc(kGuideWeightsBf);
         w_kn.alloc(kGuideWeightsBf);
         loss.alloc(1);
+        cap.alloc(1);
     }
 
     // He-uniform weights from a seeded PCG32, zero biases; the output layer
@@ -467,11 +490,12 @@ struct GuideNet {
         HIP_OK(hipGetLastError());
     }
 
-    // Accumulates the gradient of samples [0, n) into grads.
-    void gradient(const float *samples, int n, hipStream_t st) {
+    // Accumulates the gradient of samples [0, n) into grads (weights capped
+    // at *c_cap when given).
+    void gradient(const float *samples, int n, hipStream_t st, const float *c_cap = nullptr) {
         if (n <= 0) return;
         k_guide_gradient<<<(n + kGuideRowsPerBlock - 1) / kGuideRowsPerBlock, 256, 0, st>>>(dev(), samples, n,
-                                                                                            grads.p, loss.p);
+                                                                                            grads.p, loss.p, c_cap);
         HIP_OK(hipGetLastError());
     }
 
@@ -484,10 +508,14 @@ struct GuideNet {
         HIP_OK(hipGetLastError());
     }
 
-    // Minibatch Adam over samples [0, n): steps of `batch` samples.
+    // Minibatch Adam over samples [0, n): steps of `batch` samples, weights
+    // capped at kWeightCap x their mean over [0, n).
     void train(const float *samples, int n, int batch, hipStream_t st) {
+        if (n <= 0) return;
+        k_weight_cap<<<1, 1024, 0, st>>>(samples, n, kWeightCap, cap.p);
+        HIP_OK(hipGetLastError());
         for (int b0 = 0; b0 + batch <= n; b0 += batch) {
-            gradient(samples + size_t(b0) * kGuideSample, batch, st);
+            gradient(samples + size_t(b0) * kGuideSample, batch, st, cap.p);
             adam(1.0f / float(batch), st);
         }
     }

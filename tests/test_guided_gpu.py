@@ -51,10 +51,12 @@ def test_reference_is_mostly_finite(reference):
     assert fmean(reference) > 0
 
 
-def step_means(room, guide, steps, spp, train):
+def step_means(room, guide, steps, spp, train, bounces=0, **overrides):
     """Image mean of each step's own samples (from the running images) and
     the last step's statistics."""
-    p = lambda_amd.Progressive(room, guide, path_slots=1 << 18)
+    p = lambda_amd.Progressive(room, guide, path_slots=1 << 18, **overrides)
+    if guide is not None:
+        p.set_training(4, 4096, 1e-3, bounces)
     prev, means, st = None, [], None
     for k in range(1, steps + 1):
         st = p.step(spp, train=train)
@@ -81,6 +83,24 @@ def test_guided_estimator_is_unbiased(room):
         z = (m.mean() - plain.mean()) / se
         print("%s guide mean %.5g, plain %.5g, z = %.2f" % (name, m.mean(), plain.mean(), z), flush=True)
         assert abs(z) < 4
+
+
+@pytest.mark.parametrize("bounces", [4, 64])
+def test_per_bounce_guided_schedule_is_unbiased(room, bounces):
+    """The per-bounce guided schedule (tail_paths -1: the network as a kernel
+    of its own; with fewer guided bounces than the depth, batches of guided
+    passes each finished by a persistent launch) trains and stays unbiased
+    against the plain tracer."""
+    plain, _ = step_means(room, None, 16, 64, False)
+    g = lambda_amd.Guide(seed=9)
+    m, st = step_means(room, g, 16, 64, True, bounces, tail_paths=-1)
+    g.close()
+    assert st["train_samples"] > 10000
+    se = np.sqrt(m.var(ddof=1) / len(m) + plain.var(ddof=1) / len(plain))
+    z = (m.mean() - plain.mean()) / se
+    print("per-bounce guided (%d bounces) mean %.5g, plain %.5g, z = %.2f" % (bounces, m.mean(), plain.mean(), z),
+          flush=True)
+    assert abs(z) < 4
 
 
 def test_trained_guide_samples_toward_the_gap(room):
