@@ -142,11 +142,13 @@ __device__ __forceinline__ bf16x8 cat_bf16x4(bf16x4 a, bf16x4 b) {
 // H[row][32m + 16 + 4q + 0..3], and the weight fragment is read in the same
 // order.  Layer 0 encodes its inputs in place: lane (r16, q) needs, at k-step
 // m, bands 4 (q & 1) .. +3 of scalar 2m + (q >> 1) of each of its rows.
-// Rows >= nvalid read zero features and are not written; out[row][0..39].
+// Rows >= nvalid read zero features and are not written; out[row][0..39];
+// row tiles of 16 with no valid row are skipped (nvalid is wave-uniform).
 __device__ __forceinline__ void mlp_rows64(const GuideDev &G, const float *feat, int fstride, int nvalid, float *out,
                                            int ostride) {
     const int l = threadIdx.x & 63;
     const int r16 = l & 15, q = l >> 4;
+    const int nrt = min(4, (nvalid + 15) >> 4);  // row tiles with valid rows (the same in every lane)
     f32x4 acc[4][4];
     bf16x8 hb[2][4];
     // layer 0: K = 96 in natural order, k = 32m + 8q + j; the inputs of one
@@ -163,6 +165,7 @@ __device__ __forceinline__ void mlp_rows64(const GuideDev &G, const float *feat,
             bf16x8 xb[4];
 #pragma unroll
             for (int rt = 0; rt < 4; rt++) {
+                if (rt >= nrt) break;
                 const int row = rt * 16 + r16;
                 const float x = row < nvalid ? feat[size_t(row) * fstride + sc] : 0.0f;
                 float v[8];
@@ -175,7 +178,7 @@ __device__ __forceinline__ void mlp_rows64(const GuideDev &G, const float *feat,
                 const bf16x8 a = load8(Wt + (nt * 16 + r16) * kGuideIn + 32 * m + 8 * q);
 #pragma unroll
                 for (int rt = 0; rt < 4; rt++)
-                    acc[nt][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xb[rt], acc[nt][rt], 0, 0, 0);
+                    if (rt < nrt) acc[nt][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xb[rt], acc[nt][rt], 0, 0, 0);
             }
         }
     }
@@ -205,7 +208,7 @@ __device__ __forceinline__ void mlp_rows64(const GuideDev &G, const float *feat,
                                             *reinterpret_cast<const bf16x4 *>(w + 16));
 #pragma unroll
                 for (int rt = 0; rt < 4; rt++)
-                    acc[nt][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hb[m][rt], acc[nt][rt], 0, 0, 0);
+                    if (rt < nrt) acc[nt][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hb[m][rt], acc[nt][rt], 0, 0, 0);
             }
         }
     }

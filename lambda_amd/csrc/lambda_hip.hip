@@ -909,9 +909,10 @@ constexpr int kPersistGuidedWaves = 2;
 // kernel.  Every thread runs its paths as in k_persist; after each closest
 // hit, the lanes at a guided vertex (Lambertian, bounce index below
 // F.guide_bounces) put their features in the wave's LDS rows, and if any
-// lane has one, the whole wave runs the network for its 64 rows on the
-// matrix cores (mlp_rows64) and each lane shades with its own row of
-// outputs.  The loop ends for the wave together, when no lane has a path.
+// lane has one, the whole wave runs the network on the matrix cores for the
+// guided lanes only (packed into the first rows, empty 16-row tiles
+// skipped) and each guided lane shades with its own row of outputs.  The
+// loop ends for the wave together, when no lane has a path.
 __global__ void __launch_bounds__(kBlock, kPersistGuidedWaves) k_persist_guided(DeviceScene S, Frame F, Paths P,
                                                                                 int n_in, int2 *spill, GuideDev G) {
     __shared__ int s_code[kRing * kBlock];
@@ -949,25 +950,30 @@ __global__ void __launch_bounds__(kBlock, kPersistGuidedWaves) k_persist_guided(
                 }
             }
         }
-        if (__any(guided)) {
+        // the wave's guided lanes take rows 0 .. n-1 in lane order
+        const unsigned long long gmask = __ballot(guided);
+        const int grow = __popcll(gmask & ((1ull << lane) - 1ull));
+        if (gmask) {
             // every lane of the wave is here (the condition is uniform)
+            if (guided) {
 #pragma unroll
-            for (int k = 0; k < kGuideScalars; k++) io[lane * kGuideScalars + k] = guided ? gf.f[k] : 0.0f;
+                for (int k = 0; k < kGuideScalars; k++) io[grow * kGuideScalars + k] = gf.f[k];
+            }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // reads the features in its first layer, writes [64][40] at the end
-            mlp_rows64_call(G, io, kGuideScalars, 64, io, kLobeOut);
+            // reads the features in its first layer, writes [n][40] at the end
+            mlp_rows64_call(G, io, kGuideScalars, __popcll(gmask), io, kLobeOut);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             // each guided lane turns its own row into the compact mixture
-            if (guided) guide_compact(io + lane * kLobeOut, io + lane * kLobeOut);
+            if (guided) guide_compact(io + grow * kLobeOut, io + grow * kLobeOut);
         }
         if (alive) {
             bool push = false;
             ShadowRay sh;
-            const bool ended = shade_slot<true>(S, F, P, s, push, sh, guided ? io + lane * kLobeOut : nullptr, gf.f);
+            const bool ended = shade_slot<true>(S, F, P, s, push, sh, guided ? io + grow * kLobeOut : nullptr, gf.f);
             if (push) {
                 n_shadow++;
                 float best = sh.o.w;
