@@ -85,6 +85,10 @@ struct DevBuf {
         if (count == 0) return;
         HIP_OK(hipMalloc(&p, count * sizeof(T)));
     }
+    // alloc unless it already holds exactly count elements
+    void ensure(size_t count) {
+        if (!p || n != count) alloc(count);
+    }
     void upload(const T *src, size_t count) {
         alloc(count);
         if (count) HIP_OK(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
@@ -1325,6 +1329,11 @@ struct lambda_hip_scene {
     DeviceScene ds{};
     int num_materials = 0;
     std::vector<int> material_type;
+    // the last render's buffers (a Renderer), reused by the next render of
+    // the same size and slot count; renders of one scene serialize here
+    std::shared_ptr<void> renderer;
+    int r_width = 0, r_height = 0, r_slots = 0;
+    std::mutex render_mutex;
 };
 
 struct lambda_guide {
@@ -1600,6 +1609,12 @@ __global__ void k_tile_expand(const int *tiles, const int *offsets, int ntiles, 
 struct Renderer {
     lambda_hip_scene &g;
     RenderBuffers B;
+    struct CostOrderBuffers {
+        DevBuf<int> cost, order, tile_ids, offsets;
+        DevBuf<long long> tile_cost;
+        DevBuf<unsigned long long> t0;
+        DevBuf<uint4> save;
+    } C;                                  // two-launch cost order (allocated on first use)
     Paths P{};
     int ns = 0;
     int *host_ctr = nullptr;
@@ -1895,17 +1910,17 @@ struct Renderer {
     void run_cost_ordered(const Frame &F, int first, long long tail, volatile int32_t *progress,
                           lambda_render_stats *stats) {
         const int tiles_x = (F.width + 7) / 8, tiles_y = (F.height + 7) / 8, ntiles = tiles_x * tiles_y;
-        DevBuf<int> cost, order, tile_ids, offsets;
-        DevBuf<long long> tile_cost;
-        DevBuf<unsigned long long> t0;
-        DevBuf<uint4> save;
-        cost.alloc(size_t(F.npix));
-        order.alloc(size_t(F.npix));
-        tile_cost.alloc(size_t(ntiles));
-        tile_ids.alloc(size_t(ntiles));
-        offsets.alloc(size_t(ntiles));
-        t0.alloc(size_t(ns));
-        save.alloc(size_t(F.npix));
+        DevBuf<int> &cost = C.cost, &order = C.order, &tile_ids = C.tile_ids, &offsets = C.offsets;
+        DevBuf<long long> &tile_cost = C.tile_cost;
+        DevBuf<unsigned long long> &t0 = C.t0;
+        DevBuf<uint4> &save = C.save;
+        cost.ensure(size_t(F.npix));
+        order.ensure(size_t(F.npix));
+        tile_cost.ensure(size_t(ntiles));
+        tile_ids.ensure(size_t(ntiles));
+        offsets.ensure(size_t(ntiles));
+        t0.ensure(size_t(ns));
+        save.ensure(size_t(F.npix));
         k_fill_i<<<grid_for(F.npix), kBlock, 0, st>>>(cost.p, F.npix);
         k_fill_d<<<grid_for(ntiles), kBlock, 0, st>>>(reinterpret_cast<double *>(tile_cost.p), size_t(ntiles));
         Frame F1 = F;
@@ -1947,6 +1962,9 @@ struct Renderer {
     }
 };
 
+void render_with(Renderer &R, const Frame &F, const lambda_render_params &p, long long tail, float *out_rgb,
+                 volatile int32_t *progress, lambda_render_stats *stats);
+
 void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_rgb, volatile int32_t *progress,
                  lambda_render_stats *stats = nullptr) {
     auto wall0 = std::chrono::steady_clock::now();
@@ -1961,7 +1979,28 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
     // Below this many live paths the rest of the render runs in one
     // persistent launch (per-thread chains instead of one pass per bounce).
     const long long tail = p.tail_paths < 0 ? 0 : p.tail_paths > 0 ? p.tail_paths : kAutoTailPaths;
-    Renderer R(g, F, int(slots));
+    std::lock_guard<std::mutex> lock(g.render_mutex);
+    if (!g.renderer || g.r_width != F.width || g.r_height != F.height || g.r_slots != int(slots)) {
+        g.renderer.reset();  // the old buffers go first
+        g.renderer = std::make_shared<Renderer>(g, F, int(slots));
+        g.r_width = F.width;
+        g.r_height = F.height;
+        g.r_slots = int(slots);
+    }
+    Renderer &R = *static_cast<Renderer *>(g.renderer.get());
+    try {
+        render_with(R, F, p, tail, out_rgb, progress, stats);
+    } catch (...) {
+        g.renderer.reset();  // a failed render leaves no half-used buffers behind
+        throw;
+    }
+    if (stats)
+        stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+}
+
+// One render with the buffers of R.
+void render_with(Renderer &R, const Frame &F, const lambda_render_params &p, long long tail, float *out_rgb,
+                 volatile int32_t *progress, lambda_render_stats *stats) {
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->path_slots = R.ns;
@@ -1979,8 +2018,6 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
     HIP_OK(hipMemcpyAsync(out_rgb, R.B.rgb.p, size_t(F.npix) * 3 * sizeof(float), hipMemcpyDeviceToHost, R.st));
     HIP_OK(hipStreamSynchronize(R.st));
     if (progress) *progress = F.npix;
-    if (stats)
-        stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
 }
 
 }  // namespace
