@@ -188,7 +188,7 @@ typedef struct lambda_render_params {
                               the first n samples of every pixel, timing each
                               pixel, then the rest with the slowest 8x8 tiles
                               first (the image does not change): 0 =
-                              automatic (n = 2 when there are fewer slots
+                              automatic (n = 4 when there are fewer slots
                               than pixels), -1 = off, n >= 1 = on with n
                               samples first (at most samples - 1) */
     int32_t graph_passes;  /* > 0: one launch per bounce throughout, the bounce

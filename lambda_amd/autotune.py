@@ -27,7 +27,7 @@ CACHE = os.environ.get("LAMBDA_AUTOTUNE_CACHE", os.path.join(os.path.expanduser(
 def candidates(npix: int, rng_mode: int) -> list[dict]:
     """The search space: one dict of render-parameter overrides each."""
     c = [
-        {"tail_paths": 0, "path_slots": 0, "cost_order": 2},        # persistent, resident slots, slowest tiles first
+        {"tail_paths": 0, "path_slots": 0, "cost_order": 4},        # persistent, resident slots, slowest tiles first
         {"tail_paths": 0, "path_slots": 0, "cost_order": 8},        # ... timed over the first 8 samples
         {"tail_paths": 0, "path_slots": 0, "cost_order": -1},       # persistent, tile order
         {"tail_paths": 65536, "path_slots": npix, "cost_order": -1},  # passes, persistent tail

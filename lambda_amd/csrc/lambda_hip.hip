@@ -2006,11 +2006,11 @@ void render_with(Renderer &R, const Frame &F, const lambda_render_params &p, lon
         stats->path_slots = R.ns;
     }
     // reference streams, persistent schedule, more pixels than slots: the
-    // two-launch cost order (cost_order samples first; 0 = automatic: 2, the
-    // best measured on the 1080p instanced-mesh frame)
+    // two-launch cost order (cost_order samples first; 0 = automatic: 4, the
+    // best measured on the 1080p instanced-mesh frame against 1, 2 and 8)
     const bool by_cost = F.mode == LAMBDA_RNG_REFERENCE && F.spp > 1 && p.graph_passes <= 0 &&
                          tail > R.ns && (p.cost_order > 0 || (p.cost_order == 0 && R.ns < F.npix));
-    if (by_cost) R.run_cost_ordered(F, std::min(p.cost_order > 0 ? p.cost_order : 2, F.spp - 1), tail, progress, stats);
+    if (by_cost) R.run_cost_ordered(F, std::min(p.cost_order > 0 ? p.cost_order : 4, F.spp - 1), tail, progress, stats);
     else if (p.graph_passes > 0) R.run_graph(F, p.graph_passes, progress, stats);
     else R.run(F, tail, progress, stats);
     k_resolve<<<grid_for(F.npix), kBlock, 0, R.st>>>(F, R.P.fb, R.B.rgb.p);
