@@ -257,7 +257,7 @@ constexpr int kGuideSample = 10;
 // gradients dW_l = H_{l-1}^T dZ_l sum over the block's rows on the MFMA
 // (16 x 16 tiles shared among the block's waves).
 __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float *samples, int n, float *grads,
-                                                        double *loss, const float *c_cap) {
+                                                        float *loss, const float *c_cap) {
     __shared__ __attribute__((aligned(16))) __bf16 sX[kGuideRowsPerBlock * kLdX];
     __shared__ __attribute__((aligned(16))) __bf16 sH[4][kGuideRowsPerBlock * kLdH];   // H1..H4
     __shared__ __attribute__((aligned(16))) __bf16 sD[2][kGuideRowsPerBlock * kLdH];   // dZ ping-pong
@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
     // wave 0 holds all 64 rows' losses
     if (w == 0) {
         for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
-        if (l == 0) atomicAdd(loss, double(lsum));
+        if (l == 0) atomicAdd(loss, lsum);
     }
     __syncthreads();
 
@@ -410,23 +410,23 @@ constexpr float kWeightCap = 20.0f;
 
 // *cap = factor x the mean weight of samples [0, n) (one block).
 __global__ void __launch_bounds__(1024) k_weight_cap(const float *samples, int n, float factor, float *cap) {
-    __shared__ double part[1024];
-    double s = 0;
-    for (int i = threadIdx.x; i < n; i += 1024) s += double(samples[size_t(i) * kGuideSample + 9]);
+    __shared__ float part[1024];
+    float s = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) s += samples[size_t(i) * kGuideSample + 9];
     part[threadIdx.x] = s;
     __syncthreads();
     for (int off = 512; off > 0; off >>= 1) {
         if (int(threadIdx.x) < off) part[threadIdx.x] += part[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *cap = float(factor * part[0] / double(n > 0 ? n : 1));
+    if (threadIdx.x == 0) *cap = factor * part[0] / float(n > 0 ? n : 1);
 }
 
 // Host side of the network: parameters, Adam state and the BF16 copies.
 struct GuideNet {
     DevBuf<float> params, grads, m, v, cap;
     DevBuf<__bf16> w_nk, w_kn;
-    DevBuf<double> loss;
+    DevBuf<float> loss;       // summed loss of the last gradient or training call
     int step = 0;
     float lr = 1e-3f;
 

@@ -1117,6 +1117,11 @@ __global__ void k_fill_d(double *p, size_t n) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
 }
+
+__global__ void k_fill_u64(unsigned long long *p, size_t n) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
 __global__ void k_fill_i(int *p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -1675,7 +1680,7 @@ struct Renderer {
         };
         if (clear) k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
         k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
-        k_fill_d<<<1, kBlock, 0, st>>>(reinterpret_cast<double *>(P.rays), 2);
+        k_fill_u64<<<1, kBlock, 0, st>>>(P.rays, 2);
         if (guide) tail = fused ? kAutoTailPaths : 0;
         k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns, 0);
         // The work counter starts after the slots' first units.
@@ -1778,7 +1783,7 @@ struct Renderer {
         };
         if (clear) k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
         k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
-        k_fill_d<<<1, kBlock, 0, st>>>(reinterpret_cast<double *>(P.rays), 2);
+        k_fill_u64<<<1, kBlock, 0, st>>>(P.rays, 2);
         // no unit is started after a batch's first ones
         k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrWork, int(units));
         HIP_OK(hipGetLastError());
@@ -1857,7 +1862,7 @@ struct Renderer {
         counts.alloc(3);
         k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
         k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
-        k_fill_d<<<1, kBlock, 0, st>>>(reinterpret_cast<double *>(P.rays), 2);
+        k_fill_u64<<<1, kBlock, 0, st>>>(P.rays, 2);
         k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns, 0);
         k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrWork, ns);
         k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrNext, ns);  // every slot starts in the closest-hit queue
@@ -1922,7 +1927,8 @@ struct Renderer {
         t0.ensure(size_t(ns));
         save.ensure(size_t(F.npix));
         k_fill_i<<<grid_for(F.npix), kBlock, 0, st>>>(cost.p, F.npix);
-        k_fill_d<<<grid_for(ntiles), kBlock, 0, st>>>(reinterpret_cast<double *>(tile_cost.p), size_t(ntiles));
+        k_fill_u64<<<grid_for(ntiles), kBlock, 0, st>>>(reinterpret_cast<unsigned long long *>(tile_cost.p),
+                                                         size_t(ntiles));
         Frame F1 = F;
         F1.sample_end = first;
         F1.cost = cost.p;
@@ -2258,10 +2264,14 @@ int lambda_hip_guide_gradient(lambda_guide *guide, const float *samples, int32_t
         DevBuf<float> sm;
         if (n > 0) sm.upload(samples, size_t(n) * kGuideSample);
         HIP_OK(hipMemset(net.grads.p, 0, kGuideParams * sizeof(float)));
-        HIP_OK(hipMemset(net.loss.p, 0, sizeof(double)));
+        HIP_OK(hipMemset(net.loss.p, 0, sizeof(float)));
         net.gradient(sm.p, n, nullptr);
         HIP_OK(hipMemcpy(grads, net.grads.p, kGuideParams * sizeof(float), hipMemcpyDeviceToHost));
-        if (loss) HIP_OK(hipMemcpy(loss, net.loss.p, sizeof(double), hipMemcpyDeviceToHost));
+        if (loss) {
+            float l = 0;
+            HIP_OK(hipMemcpy(&l, net.loss.p, sizeof(float), hipMemcpyDeviceToHost));
+            *loss = l;
+        }
         HIP_OK(hipMemset(net.grads.p, 0, kGuideParams * sizeof(float)));
     });
 }
@@ -2274,11 +2284,15 @@ int lambda_hip_guide_train(lambda_guide *guide, const float *samples, int32_t n,
         GuideNet &net = guide->net;
         DevBuf<float> sm;
         if (n > 0) sm.upload(samples, size_t(n) * kGuideSample);
-        HIP_OK(hipMemset(net.loss.p, 0, sizeof(double)));
+        HIP_OK(hipMemset(net.loss.p, 0, sizeof(float)));
         net.lr = lr;
         net.train(sm.p, n, batch, nullptr);
         HIP_OK(hipDeviceSynchronize());
-        if (loss) HIP_OK(hipMemcpy(loss, net.loss.p, sizeof(double), hipMemcpyDeviceToHost));
+        if (loss) {
+            float l = 0;
+            HIP_OK(hipMemcpy(&l, net.loss.p, sizeof(float), hipMemcpyDeviceToHost));
+            *loss = l;
+        }
     });
 }
 
@@ -2337,7 +2351,7 @@ int lambda_hip_serve(lambda_hip_scene *const *scenes, int32_t num_scenes, const 
             ~StreamFree() { if (s) (void)hipStreamDestroy(s); }
         } sfree{st};
         k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
-        k_fill_d<<<1, kBlock, 0, st>>>(reinterpret_cast<double *>(P.rays), 2);
+        k_fill_u64<<<1, kBlock, 0, st>>>(P.rays, 2);
         int batches = 0;
         double first_start_ms = -1;
         size_t next = 0;
@@ -2514,13 +2528,15 @@ int lambda_hip_progressive_step(lambda_hip_progressive *pr, int32_t samples, int
         double loss = 0;
         if (pr->net && train && collected >= pr->batch) {
             GuideNet &net = *pr->net;
-            HIP_OK(hipMemsetAsync(net.loss.p, 0, sizeof(double), R.st));
+            HIP_OK(hipMemsetAsync(net.loss.p, 0, sizeof(float), R.st));
             HIP_OK(hipEventRecord(ev.a, R.st));
             for (int e = 0; e < pr->epochs; e++) net.train(pr->train.p, int(collected), pr->batch, R.st);
             HIP_OK(hipEventRecord(ev.b, R.st));
             HIP_OK(hipEventSynchronize(ev.b));
             HIP_OK(hipEventElapsedTime(&train_ms, ev.a, ev.b));
-            HIP_OK(hipMemcpy(&loss, net.loss.p, sizeof(double), hipMemcpyDeviceToHost));
+            float lsum = 0;
+            HIP_OK(hipMemcpy(&lsum, net.loss.p, sizeof(float), hipMemcpyDeviceToHost));
+            loss = lsum;
             loss /= double(collected / pr->batch * pr->batch) * pr->epochs;
         }
         if (keep) pr->total += samples;
