@@ -266,7 +266,9 @@ def main() -> int:
     ap.add_argument("--workload", default="instanced_mesh", choices=["instanced_mesh", "closed_box", "thinfilm_grid"])
     ap.add_argument("--rng", default="reference", choices=["reference", "throughput"])
     ap.add_argument("--path-slots", type=int, default=0)
-    ap.add_argument("--cost-order", type=int, default=0, help="start pixels longest first (0 automatic, 1 on, -1 off)")
+    ap.add_argument("--cost-order", type=int, default=0,
+                    help="reference streams: time the first N samples of every pixel, then run the slowest tiles "
+                         "first (0 automatic: N = 4; -1 off)")
     ap.add_argument("--graph-passes", type=int, default=0, help="capture the bounce loop in a hipGraph of this many passes")
     ap.add_argument("--autotune", action="store_true", help="pick the schedule by timing short renders first")
     ap.add_argument("--tail-paths", type=int, default=0,
