@@ -27,6 +27,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lambda_hip.h"
@@ -1647,7 +1648,35 @@ struct Renderer {
     }
     ~Renderer() {
         if (host_ctr) (void)hipHostFree(host_ctr);
+        if (host_prog) (void)hipHostFree(host_prog);
         if (st) (void)hipStreamDestroy(st);
+        if (pst) (void)hipStreamDestroy(pst);
+    }
+
+    int *host_prog = nullptr;   // finished-pixel count read while a launch runs
+    hipStream_t pst = nullptr;  // stream of those reads
+
+    // Waits for the render stream; while it runs, copies the finished-pixel
+    // counter (a copy on a stream of its own, every 20 ms) to *progress,
+    // which only grows -- main.cpp's 100 ms progress poll sees a persistent
+    // launch advance instead of jumping at its end.
+    void wait_with_progress(const Frame &F, volatile int32_t *progress) {
+        if (!progress) return;
+        if (!pst) HIP_OK(hipStreamCreateWithFlags(&pst, hipStreamNonBlocking));
+        if (!host_prog) HIP_OK(hipHostMalloc((void **)&host_prog, sizeof(int), hipHostMallocDefault));
+        const long long units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
+        while (true) {
+            const hipError_t q = hipStreamQuery(st);
+            if (q == hipSuccess) return;
+            if (q != hipErrorNotReady) HIP_OK(q);
+            HIP_OK(hipMemcpyAsync(host_prog, P.ctr + kCtrPixels, sizeof(int), hipMemcpyDeviceToHost, pst));
+            HIP_OK(hipStreamSynchronize(pst));
+            long long done = host_prog[0];
+            if (F.mode != LAMBDA_RNG_REFERENCE) done = done * F.npix / std::max<long long>(1, units);
+            const int32_t v = int32_t(std::min<long long>(done, F.npix));
+            if (v > *progress) *progress = v;
+            std::this_thread::sleep_for(std::chrono::milliseconds(20));
+        }
     }
 
     void sync_counters() {
@@ -1710,6 +1739,8 @@ struct Renderer {
                     k_persist<<<grid_for((long long)n_closest + n_done), kBlock, 0, st>>>(g.ds, F, P, n_closest,
                                                                                           B.spill.p);
                 if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
+                HIP_OK(hipGetLastError());
+                wait_with_progress(F, progress);
                 sync_counters();
                 if (stats) {
                     unsigned long long r[2];

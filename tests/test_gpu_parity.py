@@ -8,6 +8,7 @@ vectors, through the C ABI.
     image; lower-spp renders of every golden scene against the oracle.
 """
 import os
+import time
 
 import numpy as np
 import pytest
@@ -178,6 +179,37 @@ def test_progress_reaches_all_pixels(devices):
     devices["box"].render(progress=prog)
     p = devices["box"].scene.params
     assert prog[0] == p.width * p.height
+
+
+def test_progress_advances_during_a_persistent_launch(tmp_path):
+    """The finished-pixel count grows while the persistent kernel runs (as
+    main.cpp's 100 ms poll expects), never decreases, and ends at W*H."""
+    import threading
+    from lambda_amd import scenes as gen
+    dev = lambda_amd.DeviceScene(lambda_amd.Scene.load(
+        gen.write_closed_box(str(tmp_path), width=512, height=512, samples=1024)))
+    prog = np.zeros(1, np.int32)
+    seen, stop = [], threading.Event()
+
+    def poll():
+        while not stop.is_set():
+            seen.append(int(prog[0]))
+            time.sleep(0.002)
+
+    th = threading.Thread(target=poll)
+    th.start()
+    t0 = time.perf_counter()
+    dev.render(progress=prog)
+    dt = time.perf_counter() - t0
+    stop.set()
+    th.join()
+    dev.close()
+    npix = 512 * 512
+    mid = [v for v in seen if 0 < v < npix]
+    print("render %.0f ms, %d polls, %d intermediate values" % (dt * 1e3, len(seen), len(mid)), flush=True)
+    assert prog[0] == npix
+    assert all(b >= a for a, b in zip(seen, seen[1:]))
+    assert dt < 0.1 or len(set(mid)) >= 2
 
 
 def test_invalid_scene_rejected_without_launch(scenes):
