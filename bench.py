@@ -348,14 +348,18 @@ def main() -> int:
         args.graph_passes = best.get("graph_passes", 0)
     paths_per_step = p.width * p.height * p.samples
 
+    # the output buffer is allocated once, as main.cpp allocates h_output
+    img = np.zeros((p.height, p.width, 3), np.float32)
     for i in range(args.warmup):
         t0 = time.perf_counter()
-        dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order, graph_passes=args.graph_passes)
+        dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order,
+                   graph_passes=args.graph_passes, out=img)
         log("warmup %d: %.3f s" % (i, time.perf_counter() - t0))
 
     t0 = time.perf_counter()
     for i in range(args.steps):
-        img = dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order, graph_passes=args.graph_passes)
+        dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order,
+                   graph_passes=args.graph_passes, out=img)
     elapsed = time.perf_counter() - t0
     value = paths_per_step * args.steps / elapsed
     log("timed %d steps: %.3f s -> %.4g paths/s" % (args.steps, elapsed, value))
@@ -369,10 +373,11 @@ def main() -> int:
     # estimator, different random numbers, no per-pixel sample chains.
     thr = None
     if mode == lambda_amd.LAMBDA_RNG_REFERENCE and args.throughput_steps > 0:
-        dev.render(rng_mode=lambda_amd.LAMBDA_RNG_THROUGHPUT)
+        thr_img = np.zeros_like(img)
+        dev.render(rng_mode=lambda_amd.LAMBDA_RNG_THROUGHPUT, out=thr_img)
         t0 = time.perf_counter()
         for _ in range(args.throughput_steps):
-            dev.render(rng_mode=lambda_amd.LAMBDA_RNG_THROUGHPUT)
+            dev.render(rng_mode=lambda_amd.LAMBDA_RNG_THROUGHPUT, out=thr_img)
         t_thr = (time.perf_counter() - t0) / args.throughput_steps
         thr = {"value": paths_per_step / t_thr, "unit": "paths/s", "ms_per_step": t_thr * 1e3,
                "steps": args.throughput_steps}

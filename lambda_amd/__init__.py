@@ -157,9 +157,14 @@ class DeviceScene:
             pass
 
     def render(self, rng_mode: int = LAMBDA_RNG_REFERENCE, seed: int = 42, path_slots: int = 0,
-               progress: np.ndarray | None = None, **overrides) -> np.ndarray:
+               progress: np.ndarray | None = None, out: np.ndarray | None = None, **overrides) -> np.ndarray:
+        """Renders into out (float32 [H, W, 3], C order) when given, as main.cpp
+        renders into the h_output it allocates once, else into a new array."""
         p = self.scene.render_params(rng_mode=rng_mode, seed=seed, path_slots=path_slots, **overrides)
-        out = np.zeros((p.height, p.width, 3), np.float32)
+        if out is None:
+            out = np.zeros((p.height, p.width, 3), np.float32)
+        elif out.shape != (p.height, p.width, 3) or out.dtype != np.float32 or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-ordered float32 array of shape (%d, %d, 3)" % (p.height, p.width))
         err = C.create_string_buffer(4096)
         prog = None if progress is None else _abi.ptr(progress, C.c_int32)
         _check(self._lib.lambda_hip_scene_render(self._h, C.byref(p), _abi.ptr(out, C.c_float), prog, err, 4096), err)
