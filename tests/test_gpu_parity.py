@@ -137,7 +137,7 @@ def test_reference_stream_render_matches_oracle(name, devices, scenes):
 
 
 def test_check_c_4096spp_against_reference(tmp_path):
-    """North-star check (c): 128x128 closed box at 4096 spp within 1e-3
+    """Check (c): 128x128 closed box at 4096 spp within 1e-3
     relRMSE of the reference CPU render of the same scene."""
     g = golden("box_4096spp.npz")
     lrd = tmp_path / "box.lrd"
