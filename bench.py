@@ -136,7 +136,8 @@ def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, 
 def guided_leg(args, scene_dir: str) -> dict:
     """Wall time to relMSE 1e-3 against a plain reference of 65536 spp on the
     room lit through a door gap, plain tracer vs guided (the guide trains on
-    the paths of every step)."""
+    the paths of the first steps); the guided figure is the median of
+    args.guided_runs runs."""
     import lambda_amd
     from lambda_amd import scenes
     w, h = (int(x) for x in args.guided_res.split("x"))
@@ -154,21 +155,32 @@ def guided_leg(args, scene_dir: str) -> dict:
     log("guided leg: %dx%d reference at %d spp in %.1f s" % (w, h, args.guided_ref_spp, t_ref))
     plain, plain_curve = time_to_target(dev, None, ref, 1e-3, args.guided_step, args.guided_max_spp)
     log("guided leg: plain reaches relMSE 1e-3 at %s" % (plain,))
-    guide = lambda_amd.Guide(seed=args.guided_seed)
-    guided, guided_curve = time_to_target(dev, guide, ref, 1e-3, args.guided_step, args.guided_max_spp,
-                                          warmup=args.guided_warmup, train_steps=args.guided_train_steps,
-                                          epochs=args.guided_epochs, batch=args.guided_batch,
-                                          guided_bounces=args.guided_bounces)
-    guide.close()
-    log("guided leg: guided reaches relMSE 1e-3 at %s" % (guided,))
+    # Training is not deterministic (the order of the gradient atomics and of
+    # the training-sample buffer), so the guided time varies from run to run:
+    # several runs with their own guide seeds, the median reported.
+    runs = []
+    for k in range(max(1, args.guided_runs)):
+        guide = lambda_amd.Guide(seed=args.guided_seed + k)
+        hit, curve = time_to_target(dev, guide, ref, 1e-3, args.guided_step, args.guided_max_spp,
+                                    warmup=args.guided_warmup, train_steps=args.guided_train_steps,
+                                    epochs=args.guided_epochs, batch=args.guided_batch,
+                                    guided_bounces=args.guided_bounces)
+        guide.close()
+        log("guided leg: guided run %d reaches relMSE 1e-3 at %s" % (k, hit))
+        runs.append((hit, curve))
     dev.close()
+    done = sorted((r for r in runs if r[0]), key=lambda r: r[0][0])
+    # the median run; none when a run did not reach the target
+    guided, guided_curve = done[(len(done) - 1) // 2] if len(done) == len(runs) else (None, runs[0][1])
     out = {"workload": "guiding_room", "width": w, "height": h, "depth": 16,
            "reference_spp": args.guided_ref_spp, "target_rel_mse": 1e-3, "step_spp": args.guided_step,
            "guided_warmup_steps": args.guided_warmup, "guided_train_steps": min(args.guided_train_steps, 1 << 30),
            "guided_epochs": args.guided_epochs, "guided_batch": args.guided_batch, "adam_lr": 1e-3,
-           "guided_bounces": args.guided_bounces or 4,
+           "guided_bounces": args.guided_bounces or 8,
            "guided_schedule": "persistent kernel, network evaluated per wave on the matrix cores",
            "plain_s": plain[0] if plain else None, "plain_spp": plain[1] if plain else None,
+           "guided_runs": len(runs), "guided_s_runs": [r[0][0] if r[0] else None for r in runs],
+           "guided_spp_runs": [r[0][1] if r[0] else None for r in runs],
            "guided_s": guided[0] if guided else None, "guided_spp": guided[1] if guided else None,
            "plain_curve": plain_curve[::max(1, len(plain_curve) // 12)],
            "guided_curve": guided_curve[::max(1, len(guided_curve) // 12)]}
@@ -287,13 +299,15 @@ def main() -> int:
     ap.add_argument("--guided-step", type=int, default=64)
     ap.add_argument("--guided-max-spp", type=int, default=65536)
     ap.add_argument("--guided-seed", type=int, default=1)
+    ap.add_argument("--guided-runs", type=int, default=3,
+                    help="guided runs (guide seeds guided-seed, +1, ...); the median time is reported")
     ap.add_argument("--guided-warmup", type=int, default=0, help="guided steps that train without adding samples")
     ap.add_argument("--guided-train-steps", type=int, default=48,
                     help="first kept guided steps that also train (their samples are unbiased and count)")
     ap.add_argument("--guided-epochs", type=int, default=4, help="Adam epochs over each step's samples")
     ap.add_argument("--guided-batch", type=int, default=4096)
     ap.add_argument("--guided-bounces", type=int, default=0,
-                    help="vertices guided per path (bounce index below this; 0 = the library default, 4)")
+                    help="vertices guided per path (bounce index below this; 0 = the library default, 8)")
     ap.add_argument("--throughput-steps", type=int, default=2,
                     help="also time this many renders with throughput streams (0 to skip)")
     ap.add_argument("--stats-pixels", type=int, default=20000, help="pixels sampled to count traversal work")

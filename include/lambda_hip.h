@@ -411,7 +411,7 @@ int lambda_hip_progressive_step(lambda_hip_progressive *session, int32_t samples
 int lambda_hip_progressive_image(lambda_hip_progressive *session, float *out_rgb, char *err, size_t err_len);
 /* Training per step: `epochs` passes of Adam (rate lr) over the step's
  * samples in batches of `batch` (defaults 1, 16384, 1e-3).  Vertices with
- * bounce index below guided_bounces are guided (0 = the default of 4; a
+ * bounce index below guided_bounces are guided (0 = the default of 8; a
  * value above the path depth guides every vertex); with fewer guided
  * bounces than the depth, each batch of paths runs its guided bounces as
  * passes and the rest in one persistent launch. */

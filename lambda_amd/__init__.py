@@ -323,7 +323,7 @@ class Progressive:
 
     def set_training(self, epochs: int = 1, batch: int = 1 << 14, lr: float = 1e-3, guided_bounces: int = 0) -> None:
         """Adam epochs per step, batch, rate, and the vertices guided: those
-        with a bounce index below guided_bounces (0 = the default of 4; a
+        with a bounce index below guided_bounces (0 = the default of 8; a
         value above the depth guides every vertex)."""
         err = C.create_string_buffer(4096)
         _check(self._lib.lambda_hip_progressive_set_training(self._h, epochs, batch, lr, guided_bounces, err, 4096),

@@ -2066,7 +2066,7 @@ void render_with(Renderer &R, const Frame &F, const lambda_render_params &p, lon
 // Guided vertices per sample when a session does not set them: the first
 // four bounces (on the guiding room, 2 and 8 or every bounce gave less
 // variance reduction per unit time).
-constexpr int kDefaultGuidedBounces = 4;
+constexpr int kDefaultGuidedBounces = 8;
 
 struct lambda_hip_progressive {
     lambda_hip_scene *scene = nullptr;
