@@ -136,6 +136,30 @@ def test_reference_stream_render_matches_oracle(name, devices, scenes):
     assert err < 2e-2
 
 
+@pytest.mark.parametrize("shape", [(1, 1, 0, 1), (1, 1, 8, 4), (13, 7, 8, 4), (9, 17, 0, 9), (16, 8, 1, 1),
+                                   (31, 5, 3, 16)])
+def test_edge_frame_shapes_match_oracle(shape, devices, scenes):
+    """One-pixel frames, sizes that are not multiples of the 8x8 tiles, path
+    depth 0 and 1, one sample per pixel: the default schedule against the
+    oracle, and the two-launch cost order (first sample, then the rest from
+    the saved streams; fewer slots than pixels) bit for bit against it."""
+    w, h, depth, spp = shape
+    root = int(round(spp ** 0.5))
+    kw = dict(width=w, height=h, depth=depth, samples=spp, sqrt_samples=root)
+    print("shape %dx%d depth %d spp %d" % shape, flush=True)
+    want = oracle.render(scenes["box"], scenes["box"].render_params(**kw))
+    got = devices["box"].render(**kw)
+    assert got.shape == (h, w, 3)
+    m = np.isfinite(got) & np.isfinite(want)
+    assert np.array_equal(m, np.isfinite(want))
+    err = rel_rmse(got[m], want[m]) if np.any(want[m]) else float(np.abs(got[m]).max())
+    print("relRMSE vs oracle %.3g" % err, flush=True)
+    assert err < 1e-4
+    if spp > 1:
+        two = devices["box"].render(cost_order=1, path_slots=max(1, (w * h) // 3), **kw)
+        assert np.array_equal(two.view(np.uint32), got.view(np.uint32))
+
+
 def test_check_c_4096spp_against_reference(tmp_path):
     """Check (c): 128x128 closed box at 4096 spp within 1e-3
     relRMSE of the reference CPU render of the same scene."""
