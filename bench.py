@@ -20,6 +20,13 @@ Printed JSON line (rank 0, one GPU):
   cpu_baseline the CPU oracle (bit-identical restatement of the reference
                CPU path) on a band of rows, all host threads
   throughput_streams  the same frame with independent streams per sample
+  guided_vs_plain     wall time to relMSE 1e-3 against a 65536-spp plain
+               reference on the room lit through a door gap (320x180, depth
+               16): the plain tracer, and the median of 3 guided runs
+  guide_mlp_roofline  the guiding network's forward and gradient kernels,
+               algorithmic FLOP per second against 2.5 PFLOP/s dense BF16
+  batched_serving     32 seeded jobs with Poisson arrivals in persistent
+               launches: aggregate paths/s and latency p50 / p99
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
