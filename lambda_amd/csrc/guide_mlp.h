@@ -116,7 +116,6 @@ __device__ __forceinline__ void layer_rows16(const GuideDev &G, int layer, const
 
 // Inference works on 64 rows per wave (4 waves per block).
 constexpr int kFwdRowsPerWave = 64;
-constexpr int kFwdRowsPerBlock = kFwdRowsPerWave * kGuideWaves;  // 256
 
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
@@ -237,14 +236,33 @@ __device__ __attribute__((noinline)) void mlp_rows64_call(const GuideDev &G, con
     mlp_rows64(G, feat, fstride, nvalid, out, ostride);
 }
 
-// Inference for n rows (n read from n_dev when given): 64 rows per wave,
-// registers only.  ostride must be a multiple of 4.
-__global__ void __launch_bounds__(256, 2) k_guide_forward(GuideDev G, const float *feat, int fstride, int n,
-                                                          const int *n_dev, float *out, int ostride) {
+// Inference for n rows (n read from n_dev when given).  The block stages
+// every layer's weights (W^T, 45 KB of bfloat16) in LDS once; its waves then
+// take 64-row chunks, every (waves in the grid)-th one, with the activations
+// in registers.  Without the staging each wave reloads all the weights
+// through the vector L1 for every 64 rows, and that cache's access rate
+// bounds the kernel.  ostride must be a multiple of 4.
+constexpr int kFwdThreads = 256;
+constexpr int kFwdWaves = kFwdThreads / 64;
+constexpr int kFwdMaxBlocks = 768;  // three blocks per compute unit (3 waves/SIMD)
+
+__global__ void __launch_bounds__(kFwdThreads, 3) k_guide_forward(GuideDev G, const float *feat, int fstride, int n,
+                                                               const int *n_dev, float *out, int ostride) {
+    __shared__ __attribute__((aligned(16))) __bf16 sW[kGuideWeightsBf];
+    for (int i = threadIdx.x * 8; i < kGuideWeightsBf; i += kFwdThreads * 8)
+        *reinterpret_cast<bf16x8 *>(sW + i) = load8(G.w_nk + i);
+    __syncthreads();
     if (n_dev) n = *n_dev;
-    const int row0 = blockIdx.x * kFwdRowsPerBlock + (threadIdx.x >> 6) * kFwdRowsPerWave;
-    if (row0 >= n) return;
-    mlp_rows64(G, feat + size_t(row0) * fstride, fstride, n - row0, out + size_t(row0) * ostride, ostride);
+    GuideDev L = G;
+    L.w_nk = sW;
+    const long long step = (long long)gridDim.x * kFwdWaves * kFwdRowsPerWave;
+    for (long long row0 = ((long long)blockIdx.x * kFwdWaves + (threadIdx.x >> 6)) * kFwdRowsPerWave; row0 < n;
+         row0 += step) {
+        // the weights are read from LDS in every chunk, not held in
+        // registers across chunks
+        asm volatile("" ::: "memory");
+        mlp_rows64(L, feat + size_t(row0) * fstride, fstride, int(n - row0), out + size_t(row0) * ostride, ostride);
+    }
 }
 
 // Training sample: features[6], direction[3], weight c.
@@ -480,16 +498,17 @@ struct GuideNet {
 
     void forward(const float *feat, int fstride, int n, float *out, int ostride, hipStream_t st) const {
         if (n <= 0) return;
-        const int blocks = (n + kFwdRowsPerBlock - 1) / kFwdRowsPerBlock;
-        k_guide_forward<<<blocks, 256, 0, st>>>(dev(), feat, fstride, n, nullptr, out, ostride);
+        const int blocks = std::min((n + kFwdWaves * kFwdRowsPerWave - 1) / (kFwdWaves * kFwdRowsPerWave), kFwdMaxBlocks);
+        k_guide_forward<<<blocks, kFwdThreads, 0, st>>>(dev(), feat, fstride, n, nullptr, out, ostride);
         HIP_OK(hipGetLastError());
     }
 
     // Inference for at most `bound` rows, the count read on the device.
     void forward_dev(const float *feat, const int *n_dev, int bound, float *out, hipStream_t st) const {
         if (bound <= 0) return;
-        const int blocks = (bound + kFwdRowsPerBlock - 1) / kFwdRowsPerBlock;
-        k_guide_forward<<<blocks, 256, 0, st>>>(dev(), feat, kGuideScalars, 0, n_dev, out, kLobeOut);
+        const int blocks =
+            std::min((bound + kFwdWaves * kFwdRowsPerWave - 1) / (kFwdWaves * kFwdRowsPerWave), kFwdMaxBlocks);
+        k_guide_forward<<<blocks, kFwdThreads, 0, st>>>(dev(), feat, kGuideScalars, 0, n_dev, out, kLobeOut);
         HIP_OK(hipGetLastError());
     }
 
