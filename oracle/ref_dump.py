@@ -177,3 +177,15 @@ def pcg(seed: int, stream: int, n: int) -> np.ndarray:
 def unit_vectors(seed: int, stream: int, n: int) -> np.ndarray:
     rows = [[float.fromhex(t) for t in line.split()] for line in run("unitvec", seed, stream, n).splitlines()]
     return np.array(rows, np.float32)
+
+
+def ppm(image: np.ndarray, workdir: str) -> bytes:
+    """The bytes of the PPM the reference's Writer makes of a float [H, W, 3]
+    image (core/writer.h:17-41)."""
+    img = np.ascontiguousarray(image, np.float32)
+    src = os.path.join(workdir, "ppm_in.f32")
+    dst = os.path.join(workdir, "ppm_out.ppm")
+    img.tofile(src)
+    run("ppm", src, img.shape[1], img.shape[0], dst)
+    with open(dst, "rb") as fh:
+        return fh.read()

@@ -1,8 +1,8 @@
 // Test infrastructure only: a driver around the reference's own headers
 // (compiled from /root/reference/src by oracle/Makefile into oracle/_ref/).
 // It emits golden vectors: the reference's parsed scene arrays, rendered
-// float images, Scene::hit results, thin-film reflectances, PCG32 draws and
-// randomUnitVector outputs.  Nothing in the product links or runs it.
+// float images, Scene::hit results, thin-film reflectances, PCG32 draws,
+// randomUnitVector outputs and PPM files from its Writer.  Nothing in the product links or runs it.
 //
 // The reference keeps some of what is tested here (thinFilmReflectance,
 // Renderer fields) private; the driver opens access with a macro after every
@@ -34,6 +34,7 @@
 #include "core/constants.h"
 #include "core/parser.h"
 #include "core/utils.h"
+#include "core/writer.h"
 #include "render/renderer.h"
 #include "scene/scene.h"
 #undef private
@@ -219,6 +220,17 @@ int cmd_unitvec(uint64_t seed, uint64_t stream, int n) {
     return 0;
 }
 
+// Writes a float RGB buffer (raw float[H][W][3]) as a PPM with the
+// reference's Writer (core/writer.h:17-41: tone map, Random(0, 0) dither).
+int cmd_ppm(const std::string &in, int width, int height, const std::string &out) {
+    std::vector<char> b = slurp(in);
+    if (b.size() != size_t(width) * size_t(height) * 3 * sizeof(Float)) throw std::runtime_error("ppm: size mismatch");
+    std::vector<Float> buffer(size_t(width) * size_t(height) * 3);
+    std::memcpy(buffer.data(), b.data(), b.size());
+    Writer::write(out, width, height, buffer.data());
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -230,7 +242,8 @@ int main(int argc, char **argv) {
         if (c == "thinfilm" && argc == 6) return cmd_thinfilm(argv[2], std::atoi(argv[3]), argv[4], argv[5]);
         if (c == "pcg" && argc == 5) return cmd_pcg(std::strtoull(argv[2], nullptr, 10), std::strtoull(argv[3], nullptr, 10), std::atoi(argv[4]));
         if (c == "unitvec" && argc == 5) return cmd_unitvec(std::strtoull(argv[2], nullptr, 10), std::strtoull(argv[3], nullptr, 10), std::atoi(argv[4]));
-        std::fprintf(stderr, "usage: ref_harness scene|render|hits|thinfilm|pcg|unitvec ...\n");
+        if (c == "ppm" && argc == 6) return cmd_ppm(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), argv[5]);
+        std::fprintf(stderr, "usage: ref_harness scene|render|hits|thinfilm|pcg|unitvec|ppm ...\n");
         return 2;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "%s\n", e.what());

@@ -11,8 +11,12 @@ stores only inputs and outputs:
   thinfilm.npz                 thin-film reflectance queries and results
   rng.npz                      PCG32 draws and randomUnitVector outputs
   box_4096spp.npz              128x128 closed-box render at 4096 spp
+  box_render.ppm               the reference Writer's PPM of box_render.npy
+  ppm_cases.npy, ppm_cases.ppm a float image of edge values (negative, NaN,
+                               infinities, the sRGB knee, above 1) and the
+                               reference Writer's PPM of it
 
-Usage: python tests/golden/make_golden.py [--skip-4096]
+Usage: python tests/golden/make_golden.py [--skip-4096] [--ppm-only]
 """
 from __future__ import annotations
 
@@ -114,9 +118,38 @@ def shadow_set(arrs: dict, hits: np.ndarray, rays: np.ndarray, rng: np.random.Ge
     return np.concatenate([pts, dirs], 1).astype(np.float32), li, lo
 
 
+def ppm_cases() -> np.ndarray:
+    """16 x 8 float image: every PPM quantisation edge the writer meets."""
+    knee = np.float32(0.0031308)
+    special = np.array([0.0, -0.0, -1.0, 1.0, 2.0, 1e30, np.nan, np.inf, -np.inf, 1e-38, 1e-45,
+                        knee, np.nextafter(knee, np.float32(0)), np.nextafter(knee, np.float32(1)), 0.5, 0.999999],
+                       np.float32)
+    rng = np.random.Generator(np.random.PCG64(77))
+    ramp = np.linspace(0, 1, 16 * 8 * 3 - len(special), dtype=np.float32)
+    vals = np.concatenate([special, ramp + rng.normal(0, 1e-3, ramp.shape).astype(np.float32)])
+    return vals.astype(np.float32).reshape(8, 16, 3)
+
+
+def make_ppm(tmp: str) -> None:
+    box = np.load(os.path.join(HERE, "box_render.npy"))
+    with open(os.path.join(HERE, "box_render.ppm"), "wb") as fh:
+        fh.write(ref_dump.ppm(box, tmp))
+    cases = ppm_cases()
+    np.save(os.path.join(HERE, "ppm_cases.npy"), cases)
+    with open(os.path.join(HERE, "ppm_cases.ppm"), "wb") as fh:
+        fh.write(ref_dump.ppm(cases, tmp))
+
+
 def main(argv) -> None:
     if not ref_dump.available():
         raise SystemExit("reference harness missing: run `make -C oracle ref` first")
+    if "--ppm-only" in argv:
+        tmp = tempfile.mkdtemp()
+        try:
+            make_ppm(tmp)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        return
     paths = make_scenes()
     tmp = tempfile.mkdtemp()
     try:
@@ -164,6 +197,8 @@ def main(argv) -> None:
         draws = {"pcg_42_54": ref_dump.pcg(42, 54, 6), "pcg_7_123456789": ref_dump.pcg(7, 123456789, 64),
                  "unitvec_3_9": ref_dump.unit_vectors(3, 9, 16)}
         np.savez_compressed(os.path.join(HERE, "rng.npz"), **draws)
+
+        make_ppm(tmp)
 
         if "--skip-4096" not in argv:
             big = scenes.write_closed_box(tmp, 128, 128, 8, 4096, name="box_4096spp")

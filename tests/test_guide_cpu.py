@@ -64,6 +64,173 @@ def test_oracle_gradient_matches_finite_differences():
         assert abs(fd - g[i]) <= 1e-5 * max(1.0, abs(fd)), (i, fd, g[i])
 
 
+def test_oracle_encoding_matches_per_element_definition():
+    """Input 16 s + 2 b is sin(2^b pi x_s), input 16 s + 2 b + 1 its cosine
+    (the encoding the renderer's network reads), element by element with
+    the math module."""
+    import math
+    f = np.random.default_rng(21).random((7, 6))
+    got = go.encode(f)
+    for r in range(7):
+        for sc in range(6):
+            for b in range(8):
+                x = f[r, sc] * 2.0 ** b * math.pi
+                assert got[r, 16 * sc + 2 * b] == pytest.approx(math.sin(x), abs=1e-14)
+                assert got[r, 16 * sc + 2 * b + 1] == pytest.approx(math.cos(x), abs=1e-14)
+
+
+def test_oracle_bf16_rounding_matches_torch():
+    """Round to nearest even at bfloat16, ties, subnormals, infinities and
+    values that round up into the next binade, against torch's conversion."""
+    import torch
+    rng = np.random.default_rng(22)
+    x = np.concatenate([rng.normal(0, 10, 4000), rng.normal(0, 1e-30, 200), [0.0, -0.0, 1.0, -1.0, 3e38, -3e38,
+                                                                             np.inf, -np.inf, 1e-40, 1.00390625,
+                                                                             1.01171875, 255.5, 256.5]])
+    x32 = x.astype(np.float32)
+    # exact ties between two bfloat16 values: low 16 bits = 0x8000
+    ties = (np.arange(1, 200, dtype=np.uint32) << 16 | 0x8000).view(np.float32)
+    x32 = np.concatenate([x32, ties, -ties])
+    want = torch.from_numpy(x32).to(torch.bfloat16).to(torch.float64).numpy()
+    got = go.bf16(x32)
+    assert np.array_equal(got, want)
+
+
+def torch_net(p):
+    """The network as a torch float64 Linear/ReLU stack with the oracle's
+    parameter layout (W_l [K, 64] row-major then b_l)."""
+    import torch
+    layers = []
+    for l in range(go.LAYERS):
+        k = go.layer_in(l)
+        lin = torch.nn.Linear(k, 64, dtype=torch.float64)
+        with torch.no_grad():
+            lin.weight.copy_(torch.from_numpy(p[go.w_offset(l):go.w_offset(l) + k * 64].reshape(k, 64).T.copy()))
+            lin.bias.copy_(torch.from_numpy(p[go.b_offset(l):go.b_offset(l) + 64].copy()))
+        layers.append(lin)
+        if l < go.LAYERS - 1:
+            layers.append(torch.nn.ReLU())
+    return torch.nn.Sequential(*layers)
+
+
+def torch_encode(f):
+    import torch
+    f = torch.as_tensor(f, dtype=torch.float64)
+    cols = []
+    for sc in range(6):
+        for b in range(8):
+            x = f[:, sc] * (2.0 ** b) * np.pi
+            cols += [torch.sin(x), torch.cos(x)]
+    return torch.stack(cols, 1)
+
+
+def test_oracle_forward_matches_torch_linear_stack():
+    """go.forward against an independent torch float64 evaluation with the
+    same weights: 1e-12 relative."""
+    import torch
+    rng = np.random.default_rng(23)
+    p = random_params(rng)
+    f = rng.random((300, 6))
+    got, acts = go.forward(p, f)
+    with torch.no_grad():
+        want = torch_net(p)(torch_encode(f)).numpy()
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.abs(want).max()
+    assert len(acts) == go.LAYERS and acts[0].shape == (300, 96)
+
+
+def test_oracle_bf16_forward_matches_torch_bfloat16_storage():
+    """With bf16=True the oracle rounds the input, the weights and the hidden
+    activations to bfloat16 and accumulates in float64: the same with torch
+    tensors cast through torch.bfloat16 at those points."""
+    import torch
+    rng = np.random.default_rng(24)
+    p = random_params(rng)
+    f = rng.random((200, 6))
+
+    def rnd(t):
+        return t.to(torch.float32).to(torch.bfloat16).to(torch.float64)
+
+    with torch.no_grad():
+        h = rnd(torch_encode(f))
+        for l in range(go.LAYERS):
+            k = go.layer_in(l)
+            w = rnd(torch.from_numpy(p[go.w_offset(l):go.w_offset(l) + k * 64].reshape(k, 64).copy()))
+            z = h @ w + torch.from_numpy(p[go.b_offset(l):go.b_offset(l) + 64].copy())
+            h = rnd(torch.relu(z)) if l < go.LAYERS - 1 else z
+    got, _ = go.forward(p, f, bf=True)
+    assert np.max(np.abs(got - h.numpy())) <= 1e-12 * np.abs(h.numpy()).max()
+
+
+def torch_loss(p_t, samples):
+    """Summed -c log q(w) with q the vMF mixture written in the textbook form
+    C(kappa) = kappa / (4 pi sinh kappa), through torch autograd."""
+    import torch
+    s = torch.as_tensor(samples, dtype=torch.float64)
+    h = torch_encode(s[:, :6].numpy())
+    for l in range(go.LAYERS):
+        k = go.layer_in(l)
+        w = p_t[go.w_offset(l):go.w_offset(l) + k * 64].reshape(k, 64)
+        h = h @ w + p_t[go.b_offset(l):go.b_offset(l) + 64]
+        if l < go.LAYERS - 1:
+            h = torch.relu(h)
+    q = h[:, :go.N_OUT].reshape(-1, go.LOBES, 5)
+    m = q[:, :, :3]
+    mu = m / torch.clamp(torch.linalg.norm(m, dim=2, keepdim=True), min=1e-6)
+    kappa = torch.clamp(torch.exp(torch.clamp(q[:, :, 3], max=80.0)), go.KAPPA_MIN, go.KAPPA_MAX)
+    wgt = torch.softmax(q[:, :, 4], dim=1)
+    d = s[:, 6:9]
+    comp = kappa / (4 * np.pi * torch.sinh(kappa)) * torch.exp(kappa * (mu * d[:, None, :]).sum(2))
+    return -(s[:, 9] * torch.log((wgt * comp).sum(1))).sum()
+
+
+def test_oracle_gradient_matches_torch_autograd():
+    """go.gradient (hand-derived loss gradient and backpropagation) against
+    torch autograd of an independently written loss, 1e-9 relative per
+    layer; and the summed loss."""
+    import torch
+    rng = np.random.default_rng(25)
+    p = random_params(rng)
+    s = random_samples(rng, 64)
+    g, lval = go.gradient(p, s)
+    p_t = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    lt = torch_loss(p_t, s)
+    lt.backward()
+    gt = p_t.grad.numpy()
+    assert lval == pytest.approx(lt.item(), rel=1e-10)
+    for l in range(go.LAYERS):
+        for off, size in ((go.w_offset(l), go.layer_in(l) * 64), (go.b_offset(l), 64)):
+            a, b = g[off:off + size], gt[off:off + size]
+            assert np.max(np.abs(a - b)) <= 1e-9 * max(np.abs(b).max(), 1e-12), (l, off)
+
+
+def test_oracle_adam_first_step_closed_form():
+    """At step 1 the bias-corrected moments are g and g^2, so the update is
+    exactly -lr g / (|g| + eps) for every element (0 for g = 0, -lr/11 for
+    g = 1e-9 and eps = 1e-8)."""
+    g = np.array([2.0, -3.0, 0.0, 1e-9, -1e-9, 1e3])
+    p0 = np.array([0.5, 0.25, -1.0, 0.0, 0.0, 7.0])
+    lr = 1e-3
+    p1, m1, v1 = go.adam(p0, np.zeros(6), np.zeros(6), g, 1, lr=lr)
+    np.testing.assert_allclose(p1 - p0, -lr * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=1e-18)
+    assert p1[3] - p0[3] == pytest.approx(-lr / 11, rel=1e-12)
+    np.testing.assert_allclose(m1, 0.1 * g, rtol=1e-15)
+    np.testing.assert_allclose(v1, 0.001 * g * g, rtol=1e-15)
+
+
+def test_oracle_adam_second_step_by_hand():
+    """g1 = 2 then g2 = -1 (beta1 0.9, beta2 0.999): m2 = 0.09*2 - 0.1 =
+    0.08, v2 = 0.000999*4 + 0.001 = 0.004996; bias corrections 1 - 0.81 =
+    0.19 and 1 - 0.998001 = 0.001999, so the second update is
+    -lr (8/19) / (sqrt(4996/1999) + eps)."""
+    lr = 1e-3
+    p1, m1, v1 = go.adam(np.array([1.0]), np.zeros(1), np.zeros(1), np.array([2.0]), 1, lr=lr)
+    p2, m2, v2 = go.adam(p1, m1, v1, np.array([-1.0]), 2, lr=lr)
+    assert m2[0] == pytest.approx(0.08, rel=1e-13)
+    assert v2[0] == pytest.approx(0.004996, rel=1e-13)
+    assert p2[0] - p1[0] == pytest.approx(-lr * (8 / 19) / (np.sqrt(4996 / 1999) + 1e-8), rel=1e-12)
+    assert p1[0] == pytest.approx(1.0 - lr * 2 / (2 + 1e-8), rel=1e-15)
+
+
 def test_padding_outputs_have_zero_gradient():
     rng = np.random.default_rng(4)
     p = random_params(rng)

@@ -5,7 +5,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN_SCENES, golden, load_golden_scene
+from conftest import GOLDEN, GOLDEN_SCENES, golden, load_golden_scene
 
 import lambda_amd
 
@@ -191,3 +191,18 @@ def test_write_ppm(tmp_path):
     assert lines[:3] == ["P3", "3 2", "255"]
     vals = np.array([int(x) for ln in lines[3:] if ln for x in ln.split()])
     assert vals.min() >= 0 and vals.max() <= 255 and vals[-1] == 255
+
+
+@pytest.mark.parametrize("case", ["box_render", "ppm_cases"])
+def test_write_ppm_byte_identical_to_reference_writer(tmp_path, case):
+    """lambda_write_ppm against the reference's Writer (core/writer.h:23-41):
+    the golden box render and an image of edge values (negative, NaN,
+    infinities, the sRGB knee and its neighbours, values above 1) give the
+    same bytes as the PPM files the reference harness wrote."""
+    img = np.load(os.path.join(GOLDEN, case + ".npy"))
+    out = tmp_path / "o.ppm"
+    lambda_amd.write_ppm(str(out), img)
+    with open(os.path.join(GOLDEN, case + ".ppm"), "rb") as fh:
+        want = fh.read()
+    assert out.read_bytes() == want
+

@@ -179,6 +179,100 @@ def test_path_stats_counts_work(scenes):
     assert abs(oracle.ray_bytes(sub, "closest") / oracle.ray_bytes(st, "closest") - 1) < 0.05
 
 
+def _camera_rays(scene, hit=None, draws_after_hit=0):
+    """Every camera ray of a render, in float64, from the reference's draw
+    order (u jitter, v jitter, wavelength; render/renderer.h:69-82) on the
+    PCG32 streams (seed 42, pixel).  A sample whose camera ray satisfies
+    hit(o, d) draws draws_after_hit more numbers before the next sample
+    (Appendix A of the reference's per-sample draw order); others none."""
+    p = scene.params
+    cam = np.array(scene.view.camera[:], np.float64).reshape(4, 3)
+    n = p.samples
+    out = []
+    for py in range(p.height):
+        for px in range(p.width):
+            r = oracle.pcg32(42, py * p.width + px, (3 + draws_after_hit) * n).astype(np.float64) / 2.0 ** 32
+            pos = 0
+            for k in range(n):
+                i, j = k // p.sqrt_samples, k % p.sqrt_samples
+                u = (px + (i + r[pos]) / p.sqrt_samples) / p.width
+                v = (py + (j + r[pos + 1]) / p.sqrt_samples) / p.height
+                pos += 3
+                d = cam[1] + cam[2] * u + cam[3] * v - cam[0]
+                d /= np.linalg.norm(d)
+                out.append(d)
+                if hit is not None and hit(cam[0][None], d[None])[0]:
+                    pos += draws_after_hit
+    return cam[0], np.array(out)
+
+
+def _hits_box(o, d, lo, hi):
+    with np.errstate(divide="ignore"):
+        t0, t1 = (lo - o) / d, (hi - o) / d
+    tn = np.minimum(t0, t1).max(1)
+    tf = np.maximum(t0, t1).min(1)
+    return (tn <= tf) & (tf >= 0)
+
+
+def _hits_sphere(o, d, c, r):
+    oc = np.broadcast_to(o - c, d.shape)
+    b = (d * oc).sum(1)
+    disc = b * b - ((oc * oc).sum(1) - r * r)
+    t = -b - np.sqrt(np.maximum(disc, 0))
+    return (disc > 0) & (t > 1e-4)
+
+
+SPHERE_SCENE = """Render 24 16 %d 4 380 780
+Material m %s
+Object o sphere m (0 0 0) 1
+Instance o default (0 0 0) (0 0 0) (1 1 1)
+Background equirectangular %s
+Camera (0.3 0.2 -3) (-1.2 1.1 -2) (2.4 0 0) (0 -2.2 0)
+"""
+
+
+def test_path_stats_brute_force_emitter_sphere():
+    """One emissive sphere, one instance: every path is one camera ray that
+    tests the instance-level root; the rays that pass its box (the sphere's
+    box, found here by a slab test in float64) enter the instance, test its
+    object-level root and the sphere; nothing is traced after an emitter or
+    a miss, and there are no occlusion rays."""
+    s = lambda_amd.Scene.parse(SPHERE_SCENE % (3, "emissive 2", "0"), "e.lrd")
+    st = oracle.path_stats(s)
+    o, d = _camera_rays(s)
+    a = int(_hits_box(o, d, np.full(3, -1.0), np.full(3, 1.0)).sum())
+    paths = 24 * 16 * 4
+    assert 0 < a < paths
+    assert st["paths"] == paths and st["closest_rays"] == paths and st["shadow_rays"] == 0
+    assert st["closest_nodes"] == paths + a
+    assert st["closest_instances"] == a and st["closest_spheres"] == a
+    assert st["closest_quads"] == 0 and st["closest_triangles"] == 0
+    assert oracle.ray_bytes(st, "closest") == pytest.approx((32 * (paths + a) + 64 * a + 16 * a) / paths + 128)
+
+
+def test_path_stats_brute_force_lambertian_sphere_under_sky():
+    """One Lambertian sphere under a constant sky, depth 1: camera rays as
+    above (replaying the draws a hit consumes); each camera ray that hits the sphere (float64 quadratic) scatters
+    once more from its surface, and that ray, starting inside the box,
+    tests both roots and the sphere and escapes; each occlusion ray toward
+    the sky does the same (2 nodes, 1 instance, 1 sphere)."""
+    s = lambda_amd.Scene.parse(SPHERE_SCENE % (1, "lambertian 0.5", "1"), "l.lrd")
+    st = oracle.path_stats(s)
+    # a hit draws 1 (light choice) + 3 (sky direction) + 2 (cosine scatter);
+    # the escaping second ray draws nothing
+    o, d = _camera_rays(s, lambda o, d: _hits_sphere(o, d, np.zeros(3), 1.0), 6)
+    a = int(_hits_box(o, d, np.full(3, -1.0), np.full(3, 1.0)).sum())
+    h = int(_hits_sphere(o, d, np.zeros(3), 1.0).sum())
+    paths = 24 * 16 * 4
+    assert 0 < h < a < paths
+    assert st["closest_rays"] == paths + h
+    assert st["closest_nodes"] == paths + a + 2 * h
+    assert st["closest_instances"] == a + h and st["closest_spheres"] == a + h
+    sr = st["shadow_rays"]
+    assert 0.35 * h < sr < 0.65 * h       # a uniform direction faces the normal half the time
+    assert st["shadow_nodes"] == 2 * sr and st["shadow_instances"] == sr and st["shadow_spheres"] == sr
+
+
 @pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "ref_harness")),
                     reason="reference harness not built (reference sources absent)")
 def test_parser_messages_match_reference_binary(tmp_path):
