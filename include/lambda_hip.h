@@ -243,6 +243,20 @@ typedef struct lambda_render_stats {
 int lambda_hip_scene_render_stats(lambda_hip_scene *scene, const lambda_render_params *params,
                                   float *out_rgb, lambda_render_stats *stats, char *err, size_t err_len);
 
+/* What the library made of an uploaded scene (no reference counterpart). */
+typedef struct lambda_scene_info {
+    int32_t stack_need;     /* deepest traversal stack of the scene (entries) */
+    int32_t stack_lds;      /* entries per lane kept in LDS; deeper entries go to
+                               a global spill area */
+    int32_t wide_nodes;     /* 4-wide nodes built from the scene's BVH nodes */
+    int32_t num_nodes;      /* the scene's BVH nodes (node_bounds rows) */
+    int32_t num_objects;
+    int32_t num_instances;
+    float bounds[6];        /* instance-level root box: min xyz, max xyz */
+} lambda_scene_info;
+
+int lambda_hip_scene_info(const lambda_hip_scene *scene, lambda_scene_info *info, char *err, size_t err_len);
+
 /* Closest-hit record for one ray (math/intersection.h:8-17). */
 typedef struct lambda_hit {
     int32_t hit;        /* 1 if the ray hit an object */
@@ -306,6 +320,8 @@ int lambda_hip_thinfilm_reflectance(lambda_hip_scene *scene, int32_t material, c
 #define LAMBDA_GUIDE_FEATURES 6
 #define LAMBDA_GUIDE_OUTPUTS 40
 #define LAMBDA_GUIDE_SAMPLE 10
+/* bfloat16 weights per copy: 96*64 + 4*64*64 */
+#define LAMBDA_GUIDE_WEIGHTS_BF16 22528
 
 typedef struct lambda_guide lambda_guide;
 
@@ -328,6 +344,20 @@ int lambda_hip_guide_gradient(lambda_guide *guide, const float *samples, int32_t
  * the summed loss of the samples seen. */
 int lambda_hip_guide_train(lambda_guide *guide, const float *samples, int32_t n, int32_t batch, float lr,
                            double *loss, char *err, size_t err_len);
+
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected by the
+ * step count since the last set_params) with the gradient grads[PARAMS] x
+ * scale and rate lr: the step a training call takes after each batch.  It
+ * updates the FP32 master weights and moments and rewrites both bfloat16
+ * weight copies. */
+int lambda_hip_guide_adam_step(lambda_guide *guide, const float *grads, float scale, float lr, char *err,
+                               size_t err_len);
+/* Adam state: moments m and v [PARAMS], the step count, and the two bfloat16
+ * weight copies as raw 16-bit words, [LAMBDA_GUIDE_WEIGHTS_BF16] each:
+ * w_nk holds W_l^T (n-major, [64][K_l]) and w_kn holds W_l ([K_l][64]),
+ * layer after layer.  Any pointer may be NULL. */
+int lambda_hip_guide_get_state(lambda_guide *guide, float *m, float *v, int32_t *step, uint16_t *w_nk,
+                               uint16_t *w_kn, char *err, size_t err_len);
 
 /* Times the network's kernels on `rows` device-resident samples: mean
  * milliseconds per forward pass and per gradient pass over `iters` runs

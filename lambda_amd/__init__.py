@@ -186,6 +186,16 @@ class DeviceScene:
             stats[name + "_launches"] = st.launches[k]
         return out, stats
 
+    def info(self) -> dict:
+        """What the library built from the scene: traversal stack depth (and
+        the part of it kept in LDS), 4-wide node count, counts and bounds."""
+        st = _abi.SceneInfo()
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_scene_info(self._h, C.byref(st), err, 4096), err)
+        return {"stack_need": st.stack_need, "stack_lds": st.stack_lds, "wide_nodes": st.wide_nodes,
+                "num_nodes": st.num_nodes, "num_objects": st.num_objects, "num_instances": st.num_instances,
+                "bounds": list(st.bounds)}
+
     def closest_hit(self, rays: np.ndarray) -> np.ndarray:
         rays = np.ascontiguousarray(rays, np.float32)
         out = np.zeros(len(rays), _abi.HIT_DTYPE)
@@ -276,6 +286,31 @@ class Guide:
         _check(self._lib.lambda_hip_guide_gradient(self._h, _abi.ptr(s, C.c_float), len(s), _abi.ptr(g, C.c_float),
                                                    C.byref(loss), err, 4096), err)
         return g, loss.value
+
+    def adam_step(self, grads: np.ndarray, scale: float = 1.0, lr: float = 1e-3) -> None:
+        """One Adam step with gradient grads * scale (lambda_hip_guide_adam_step)."""
+        g = np.ascontiguousarray(grads, np.float32)
+        assert g.shape == (GUIDE_PARAMS,)
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_guide_adam_step(self._h, _abi.ptr(g, C.c_float), scale, lr, err, 4096), err)
+
+    def state(self) -> dict:
+        """Adam moments m, v, the step count and the two bfloat16 weight
+        copies (as float32): w_nk = W_l^T per layer, w_kn = W_l per layer."""
+        m = np.zeros(GUIDE_PARAMS, np.float32)
+        v = np.zeros(GUIDE_PARAMS, np.float32)
+        step = C.c_int32()
+        nk = np.zeros(_abi.GUIDE_WEIGHTS_BF16, np.uint16)
+        kn = np.zeros(_abi.GUIDE_WEIGHTS_BF16, np.uint16)
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_guide_get_state(self._h, _abi.ptr(m, C.c_float), _abi.ptr(v, C.c_float),
+                                                    C.byref(step), _abi.ptr(nk, C.c_uint16),
+                                                    _abi.ptr(kn, C.c_uint16), err, 4096), err)
+
+        def widen(h):
+            return (h.astype(np.uint32) << 16).view(np.float32)
+
+        return {"m": m, "v": v, "step": step.value, "w_nk": widen(nk), "w_kn": widen(kn)}
 
     def benchmark(self, rows: int = 1 << 20, iters: int = 10):
         """(forward ms, gradient ms) per pass over `rows` device-resident rows."""

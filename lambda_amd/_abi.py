@@ -67,6 +67,15 @@ class RenderStats(C.Structure):
     ]
 
 
+class SceneInfo(C.Structure):
+    _fields_ = [("stack_need", C.c_int32), ("stack_lds", C.c_int32), ("wide_nodes", C.c_int32),
+                ("num_nodes", C.c_int32), ("num_objects", C.c_int32), ("num_instances", C.c_int32),
+                ("bounds", C.c_float * 6)]
+
+
+GUIDE_WEIGHTS_BF16 = 22528
+
+
 class Job(C.Structure):
     _fields_ = [("scene", C.c_int32), ("params", RenderParams), ("arrival_ms", C.c_double),
                 ("out_rgb", C.POINTER(C.c_float))]
@@ -158,6 +167,10 @@ def hip():
         lib.lambda_hip_guide_gradient.argtypes = [vp, _f32p, C.c_int32, _f32p, f64p, C.c_char_p, C.c_size_t]
         lib.lambda_hip_guide_train.argtypes = [vp, _f32p, C.c_int32, C.c_int32, C.c_float, f64p, C.c_char_p,
                                                C.c_size_t]
+        lib.lambda_hip_guide_adam_step.argtypes = [vp, _f32p, C.c_float, C.c_float, C.c_char_p, C.c_size_t]
+        u16p = C.POINTER(C.c_uint16)
+        lib.lambda_hip_guide_get_state.argtypes = [vp, _f32p, _f32p, _i32p, u16p, u16p, C.c_char_p, C.c_size_t]
+        lib.lambda_hip_scene_info.argtypes = [vp, C.POINTER(SceneInfo), C.c_char_p, C.c_size_t]
         lib.lambda_hip_guide_benchmark.argtypes = [vp, C.c_int32, C.c_int32, f64p, f64p, C.c_char_p, C.c_size_t]
         lib.lambda_hip_serve.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Job), C.c_int32, C.c_int64,
                                          C.POINTER(JobResult), C.POINTER(ServeStats), C.c_char_p, C.c_size_t]

@@ -26,6 +26,7 @@ constexpr int kLdX = kGuideIn + 8;
 constexpr int kLdH = kGuideHidden + 8;
 // bf16 weight elements per layout: sum over layers of K_l * 64.
 constexpr int kGuideWeightsBf = kGuideIn * 64 + 4 * 64 * 64;
+static_assert(kGuideWeightsBf == LAMBDA_GUIDE_WEIGHTS_BF16, "bfloat16 weight count of the C ABI");
 
 __host__ __device__ inline int guide_wbf_offset(int l) { return l == 0 ? 0 : kGuideIn * 64 + (l - 1) * 64 * 64; }
 
@@ -236,7 +237,7 @@ __device__ __attribute__((noinline)) void mlp_rows64_call(const GuideDev &G, con
     mlp_rows64(G, feat, fstride, nvalid, out, ostride);
 }
 
-// Inference for n rows (n read from n_dev when given).  The block stages
+// Inference for n rows (min(*n_dev, n) when n_dev is given).  The block stages
 // every layer's weights (W^T, 45 KB of bfloat16) in LDS once; its waves then
 // take 64-row chunks, every (waves in the grid)-th one, with the activations
 // in registers.  Without the staging each wave reloads all the weights
@@ -252,7 +253,8 @@ __global__ void __launch_bounds__(kFwdThreads, 3) k_guide_forward(GuideDev G, co
     for (int i = threadIdx.x * 8; i < kGuideWeightsBf; i += kFwdThreads * 8)
         *reinterpret_cast<bf16x8 *>(sW + i) = load8(G.w_nk + i);
     __syncthreads();
-    if (n_dev) n = *n_dev;
+    // a count read on the device is capped at the rows the caller's buffers hold
+    if (n_dev) n = min(*n_dev, n);
     GuideDev L = G;
     L.w_nk = sW;
     const long long step = (long long)gridDim.x * kFwdWaves * kFwdRowsPerWave;
@@ -508,7 +510,7 @@ struct GuideNet {
         if (bound <= 0) return;
         const int blocks =
             std::min((bound + kFwdWaves * kFwdRowsPerWave - 1) / (kFwdWaves * kFwdRowsPerWave), kFwdMaxBlocks);
-        k_guide_forward<<<blocks, kFwdThreads, 0, st>>>(dev(), feat, kGuideScalars, 0, n_dev, out, kLobeOut);
+        k_guide_forward<<<blocks, kFwdThreads, 0, st>>>(dev(), feat, kGuideScalars, bound, n_dev, out, kLobeOut);
         HIP_OK(hipGetLastError());
     }
 

@@ -2064,8 +2064,8 @@ void render_with(Renderer &R, const Frame &F, const lambda_render_params &p, lon
 // vertices sample the guiding mixture and each step can train the network on
 // the radiance its own paths gathered.
 // Guided vertices per sample when a session does not set them: the first
-// four bounces (on the guiding room, 2 and 8 or every bounce gave less
-// variance reduction per unit time).
+// eight bounces (on the guiding room, 2, 4, 6, 10, 12, 16 or every bounce
+// gave less variance reduction per unit time).
 constexpr int kDefaultGuidedBounces = 8;
 
 struct lambda_hip_progressive {
@@ -2282,7 +2282,10 @@ int lambda_hip_guide_forward(lambda_guide *guide, const float *features, int32_t
         DevBuf<float> f, o;
         f.upload(features, size_t(n) * kGuideScalars);
         o.alloc(size_t(n) * kLobeOut);
-        guide->net.forward(f.p, kGuideScalars, n, o.p, kLobeOut, nullptr);
+        // the row count is read on the device, as in the guided passes
+        DevBuf<int> cnt;
+        cnt.upload(&n, 1);
+        guide->net.forward_dev(f.p, cnt.p, n, o.p, nullptr);
         HIP_OK(hipMemcpy(out, o.p, size_t(n) * kLobeOut * sizeof(float), hipMemcpyDeviceToHost));
     });
 }
@@ -2324,6 +2327,46 @@ int lambda_hip_guide_train(lambda_guide *guide, const float *samples, int32_t n,
             HIP_OK(hipMemcpy(&l, net.loss.p, sizeof(float), hipMemcpyDeviceToHost));
             *loss = l;
         }
+    });
+}
+
+int lambda_hip_guide_adam_step(lambda_guide *guide, const float *grads, float scale, float lr, char *err,
+                               size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!guide || !grads || !(lr > 0) || !std::isfinite(scale)) throw Failure{LAMBDA_EINVAL, "invalid arguments"};
+        GuideNet &net = guide->net;
+        HIP_OK(hipMemcpy(net.grads.p, grads, kGuideParams * sizeof(float), hipMemcpyHostToDevice));
+        net.lr = lr;
+        net.adam(scale, nullptr);
+        HIP_OK(hipDeviceSynchronize());
+    });
+}
+
+int lambda_hip_guide_get_state(lambda_guide *guide, float *m, float *v, int32_t *step, uint16_t *w_nk,
+                               uint16_t *w_kn, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!guide) throw Failure{LAMBDA_EINVAL, "null argument"};
+        GuideNet &net = guide->net;
+        HIP_OK(hipDeviceSynchronize());
+        if (m) HIP_OK(hipMemcpy(m, net.m.p, kGuideParams * sizeof(float), hipMemcpyDeviceToHost));
+        if (v) HIP_OK(hipMemcpy(v, net.v.p, kGuideParams * sizeof(float), hipMemcpyDeviceToHost));
+        if (step) *step = net.step;
+        if (w_nk) HIP_OK(hipMemcpy(w_nk, net.w_nk.p, kGuideWeightsBf * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        if (w_kn) HIP_OK(hipMemcpy(w_kn, net.w_kn.p, kGuideWeightsBf * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    });
+}
+
+int lambda_hip_scene_info(const lambda_hip_scene *scene, lambda_scene_info *info, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scene || !info) throw Failure{LAMBDA_EINVAL, "null argument"};
+        std::memset(info, 0, sizeof(*info));
+        info->stack_need = scene->stack_need;
+        info->stack_lds = kRing;
+        info->wide_nodes = int32_t(scene->wide.n);
+        info->num_objects = scene->ds.num_objects;
+        info->num_instances = scene->ds.num_instances;
+        info->num_nodes = scene->ds.num_nodes;
+        for (int k = 0; k < 6; k++) info->bounds[k] = scene->box[k];
     });
 }
 

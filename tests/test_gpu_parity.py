@@ -133,7 +133,8 @@ def test_reference_stream_render_matches_oracle(name, devices, scenes):
     err = rel_rmse(got, want)
     print("%s: relRMSE vs oracle %.3g" % (name, err), flush=True)
     assert np.all(np.isfinite(got))
-    assert err < 2e-2
+    # measured 5.5e-8 .. 1.1e-6 (libm rounding only)
+    assert err < 1e-5
 
 
 @pytest.mark.parametrize("shape", [(1, 1, 0, 1), (1, 1, 8, 4), (13, 7, 8, 4), (9, 17, 0, 9), (16, 8, 1, 1),

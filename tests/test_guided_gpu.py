@@ -136,8 +136,8 @@ def test_trained_guide_samples_toward_the_gap(room):
 
 def test_trained_guide_against_plain_at_equal_samples(tmp_path_factory):
     """320x180, 1024 spp, 3 independent sessions each against a 16384-spp
-    plain reference: the frozen guide is not worse than the plain tracer
-    (beyond noise); the measured ratio is printed."""
+    plain reference: the guide, trained for 48 steps and then frozen, lowers
+    the relMSE at equal samples by at least 1.2x (measured 1.5-1.8x)."""
     d = tmp_path_factory.mktemp("room320")
     dev = lambda_amd.DeviceScene(lambda_amd.Scene.load(
         gen.write_guiding_room(str(d), width=320, height=180, depth=16, samples=1)))
@@ -169,4 +169,4 @@ def test_trained_guide_against_plain_at_equal_samples(tmp_path_factory):
     print("1024 spp relMSE: plain %s (mean %.4g), guided %s (mean %.4g), plain/guided %.2f" % (
         " ".join("%.3g" % e for e in plain), np.mean(plain), " ".join("%.3g" % e for e in guided),
         np.mean(guided), np.mean(plain) / np.mean(guided)), flush=True)
-    assert np.mean(guided) < 1.25 * np.mean(plain)
+    assert np.mean(plain) / np.mean(guided) >= 1.2
