@@ -30,7 +30,8 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
 HOST_SOURCES = ["scene_model.cpp", "parser.cpp", "host_api.cpp"]
-HIP_SOURCES = ["lambda_hip.hip"]
+HIP_SOURCES = ["lambda_hip.hip", "k_passes.hip", "k_shade.hip", "k_persist.hip", "k_persist_guided.hip",
+               "k_serve.hip", "k_query.hip", "guide_net.hip"]
 
 # No -march: the host front end must round exactly like the reference's
 # FMA-free x86-64 build.
@@ -62,11 +63,22 @@ def build_host():
     return out
 
 
-def build_hip():
+def build_hip(jobs: int = 0):
+    """Each translation unit to an object in parallel (the kernels are
+    spread over several files so that their device code compiles
+    concurrently), then one shared library."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIB, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    objdir = os.path.join(ROOT, "_build", "hip")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in HIP_SOURCES]
+    cmds = [[HIPCC, *HIP_FLAGS, "-I" + INC, "-I" + CSRC, "-c", os.path.join(CSRC, s), "-o", o]
+            for s, o in zip(HIP_SOURCES, objs)]
+    with ThreadPoolExecutor(max_workers=jobs or min(len(cmds), os.cpu_count() or 1)) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
     out = os.path.join(LIB, "liblambda_hip.so")
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I" + INC, "-I" + CSRC, *srcs, "-o", out + ".tmp"])
+    _run([HIPCC, *HIP_FLAGS, "-shared", *objs, "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -31,8 +31,9 @@
 #include <vector>
 
 #include "lambda_hip.h"
-#include "dscene.h"
-#include "guide.h"
+#include "guide_net.h"
+#include "integrator.h"
+#include "launch.h"
 #include "scene_prep.h"
 
 namespace lam {
@@ -42,1168 +43,6 @@ namespace lam {
 using namespace lam;
 
 namespace {
-
-constexpr int kBlock = 256;
-
-// ------------------------------------------------------------------ errors
-
-struct Failure {
-    int code;
-    std::string msg;
-};
-
-void set_err(char *err, size_t len, const std::string &msg) {
-    if (!err || len == 0) return;
-    size_t n = std::min(msg.size(), len - 1);
-    std::memcpy(err, msg.data(), n);
-    err[n] = 0;
-}
-
-#define HIP_OK(call)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (call);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            throw Failure{e_ == hipErrorOutOfMemory ? LAMBDA_ENOMEM : LAMBDA_EHIP,            \
-                          std::string(#call) + ": " + hipGetErrorString(e_)};                 \
-    } while (0)
-
-template <typename T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t count) {
-        release();
-        n = count;
-        if (count == 0) return;
-        HIP_OK(hipMalloc(&p, count * sizeof(T)));
-    }
-    // alloc unless it already holds exactly count elements
-    void ensure(size_t count) {
-        if (!p || n != count) alloc(count);
-    }
-    void upload(const T *src, size_t count) {
-        alloc(count);
-        if (count) HIP_OK(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
-    }
-    void upload(const std::vector<T> &v) { upload(v.data(), v.size()); }
-};
-
-#include "guide_mlp.h"
-
-// --------------------------------------------------------- wave utilities
-
-// Wave-aggregated slot reservation: one atomicAdd per wave; every lane of
-// the wave must call it (pred false for lanes with nothing to push).
-__device__ __forceinline__ int wave_push(int *counter, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0) return -1;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader);
-    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-    return pred ? base + rank : -1;
-}
-
-// --------------------------------------------------------------- state
-
-enum : int {
-    kFlagSpecular = 1,   // previous vertex was specular (or the camera)
-    kFlagFinished = 2,   // the sample's path has ended
-    kFlagGuidePending = 4,  // a guided vertex awaits the radiance base of its continuation
-};
-
-enum : int {
-    kCtrNext = 0,        // next closest-hit queue length
-    kCtrShadow = 1,      // occlusion queue length
-    kCtrWork = 2,        // next work item (throughput mode)
-    kCtrPixels = 3,      // finished pixels (reference mode) / samples (throughput)
-    kCtrError = 4,       // error bits
-    kCtrDone = 5,        // samples ended by the last shade pass
-    kCtrGuided = 6,      // paths queued for the guiding network this pass
-    kCtrTrain = 7,       // training samples written since the last training
-    kCtrTrainSeen = 8,   // training candidates met (kept or not)
-    kCtrCount = 16
-};
-
-struct Paths {
-    float4 *o;        // origin, previous scatter pdf
-    float4 *d;        // direction
-    float4 *lam;      // hero wavelengths
-    float4 *beta;     // throughput
-    float4 *L;        // radiance
-    uint4 *rng;       // PCG32 state and increment (lo, hi, lo, hi)
-    int4 *info;       // pixel, sample index, bounce, flags
-    int2 *hit;        // instance, object (-1 on a miss)
-    int *q_in, *q_out;
-    int *done;                // slots whose sample ended in the last shade pass
-    float4 *s_o, *s_d, *s_c;  // shadow: origin+tmax, direction, contribution
-    int4 *s_i;                // shadow: slot, light instance, light object
-    int *ctr;
-    unsigned long long *rays; // rays traced by the persistent kernel: closest, occlusion
-    double *fb;               // XYZ sums, [pixel][3]
-    const int *order;         // pixel visiting order (8x8 tiles)
-    float4 *grec;             // guided vertices, [slot][bounce][3] (guided training steps only)
-};
-
-struct Frame {
-    int width, height, depth, spp, sqrt_spp, mode, npix;
-    float lmin, lmax, lrange;
-    uint64_t seed;
-    long long total_work;     // npix * spp (throughput mode)
-    // Reference mode: this launch runs samples [sample_begin, sample_end) of
-    // each pixel (sample_end 0 = spp); a pixel that stops before spp saves
-    // its PCG32 state in rng_save, where a later launch resumes it.
-    int sample_begin, sample_end;
-    uint4 *rng_save;
-    int *cost;                // when set (persistent kernel): per-pixel device-clock ticks
-    unsigned long long *unit_t0;  // with cost: start time of each slot's unit
-    float *train;             // when set: training samples [train_cap][10]
-    int train_cap;
-    float train_keep;         // probability of keeping a training candidate
-    int grec_stride;          // guided vertices per slot (depth + 1)
-    int guide_bounces;        // vertices with bounce index below this are guided
-    float3 box_lo, box_hi;    // scene box for the guiding features
-};
-
-// New camera sample `sample` of `pixel` into slot s (render/renderer.h:71-82,
-// render/camera.h:13-16).  Consumes three draws.
-__device__ __forceinline__ void camera_sample(const DeviceScene &S, const Frame &F, const Paths &P, int s, int pixel,
-                                              int sample, Pcg &rng) {
-    const int px = pixel % F.width, py = pixel / F.width;
-    const int i = sample / F.sqrt_spp, j = sample % F.sqrt_spp;
-    float u = (float(px) + (float(i) + rng.next()) / float(F.sqrt_spp)) / float(F.width);
-    float v = (float(py) + (float(j) + rng.next()) / float(F.sqrt_spp)) / float(F.height);
-    float lambda = F.lmin + rng.next() * (F.lmax - F.lmin);
-    float4 lam;
-    lam.x = F.lmin + fmodf(lambda - F.lmin + float(0) * F.lrange / kHero, F.lrange);
-    lam.y = F.lmin + fmodf(lambda - F.lmin + float(1) * F.lrange / kHero, F.lrange);
-    lam.z = F.lmin + fmodf(lambda - F.lmin + float(2) * F.lrange / kHero, F.lrange);
-    lam.w = F.lmin + fmodf(lambda - F.lmin + float(3) * F.lrange / kHero, F.lrange);
-    V3 pos = v3(S.cam_pos), corner = v3(S.cam_corner), ch = v3(S.cam_h), cv = v3(S.cam_v);
-    V3 dir = normalize(corner + ch * u + cv * v - pos);
-    P.o[s] = make_float4(pos.x, pos.y, pos.z, 1.0f);
-    P.d[s] = make_float4(dir.x, dir.y, dir.z, 0.0f);
-    P.lam[s] = lam;
-    P.beta[s] = make_float4(1, 1, 1, 1);
-    P.L[s] = make_float4(0, 0, 0, 0);
-    P.info[s] = make_int4(pixel, sample, 0, kFlagSpecular);
-}
-
-__device__ __forceinline__ void store_rng(const Paths &P, int s, const Pcg &r) {
-    P.rng[s] = make_uint4(uint32_t(r.state), uint32_t(r.state >> 32), uint32_t(r.inc), uint32_t(r.inc >> 32));
-}
-__device__ __forceinline__ Pcg load_rng(const Paths &P, int s) {
-    uint4 v = P.rng[s];
-    Pcg r;
-    r.state = uint64_t(v.x) | (uint64_t(v.y) << 32);
-    r.inc = uint64_t(v.z) | (uint64_t(v.w) << 32);
-    return r;
-}
-
-// ---------------------------------------------------------------- kernels
-
-// Traversal stacks: kRing entries per lane in LDS, deeper entries in a
-// global area of the launch, [entry][thread] (allocated by the host when the
-// scene's worst-case depth exceeds kRing).
-constexpr int kRing = 16;
-// Waves per SIMD the persistent kernel is compiled for (its registers are
-// capped so that this many fit; measured best on gfx950).
-constexpr int kPersistWaves = 3;
-// Automatic schedule: live paths below which a render switches to the
-// persistent schedule (above any slot count: persistent from the first pass
-// on).  Slots default to the number of persistent threads resident at once.
-constexpr long long kAutoTailPaths = 1LL << 40;
-using Stack = TraversalStack<kRing>;
-
-__device__ __forceinline__ Stack make_stack(int *s_code, float *s_dist, int2 *spill) {
-    Stack s;
-    s.lcode = s_code + threadIdx.x;
-    s.ldist = s_dist + threadIdx.x;
-    s.stride = kBlock;
-    const size_t threads = size_t(gridDim.x) * blockDim.x;
-    s.spill = spill ? spill + (size_t(blockIdx.x) * blockDim.x + threadIdx.x) : nullptr;
-    s.spill_stride = threads;
-    return s;
-}
-
-template <bool kAny>
-__device__ __forceinline__ bool trace_ray(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_inst, int &hit_obj,
-                                         int skip_inst, int skip_obj, Stack &stk) {
-    return lam::traverse<kAny, kRing>(S, o, d, best, hit_inst, hit_obj, skip_inst, skip_obj, stk);
-}
-
-// Reference mode: the pixel of rank `rank` in the visiting order starts
-// sample F.sample_begin in slot s, with PCG32 stream (seed, pixel index)
-// (render/renderer.h:47-49) from its start or from the state a previous
-// launch saved; the slot keeps the pixel for the launch's samples.
-__device__ __forceinline__ void start_pixel(const DeviceScene &S, const Frame &F, const Paths &P, int s, int rank) {
-    const int pixel = P.order[rank];
-    Pcg rng;
-    if (F.sample_begin > 0) {
-        const uint4 v = F.rng_save[pixel];
-        rng.state = uint64_t(v.x) | (uint64_t(v.y) << 32);
-        rng.inc = uint64_t(v.z) | (uint64_t(v.w) << 32);
-    } else {
-        rng = pcg_seed(F.seed, uint64_t(pixel));
-    }
-    camera_sample(S, F, P, s, pixel, F.sample_begin, rng);
-    store_rng(P, s, rng);
-}
-
-// Throughput mode: work item w = sample * npix + rank starts in slot s with
-// its own PCG32 stream (seed, w).
-__device__ __forceinline__ void start_work(const DeviceScene &S, const Frame &F, const Paths &P, int s, long long w) {
-    int rank = int(w % F.npix);
-    int sample = int(w / F.npix);
-    int pixel = P.order[rank];
-    Pcg rng = pcg_seed(F.seed, uint64_t(w));
-    camera_sample(S, F, P, s, pixel, sample, rng);
-    store_rng(P, s, rng);
-}
-
-// Work units: pixels (reference mode) or (pixel, sample) items.
-__device__ __forceinline__ long long work_units(const Frame &F) {
-    return F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
-}
-
-__device__ __forceinline__ void start_unit(const DeviceScene &S, const Frame &F, const Paths &P, int s, long long w) {
-    if (F.mode == LAMBDA_RNG_REFERENCE) start_pixel(S, F, P, s, int(w));
-    else start_work(S, F, P, s, w);
-}
-
-// First work unit of every slot: slot s takes unit base + s.
-__global__ void __launch_bounds__(kBlock) k_start(DeviceScene S, Frame F, Paths P, int nslots, long long base) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    start_unit(S, F, P, s, base + s);
-    P.q_in[s] = s;
-}
-
-// Threads [0, n_closest) find the closest hit of queued path q_in[t];
-// threads [n_closest, n_closest + n_shadow) test queued occlusion ray
-// t - n_closest and, if the light is visible, add its contribution to the
-// path's radiance.
-// Queue lengths of a pass taken on the device (graph-captured passes):
-// {closest, shadow, done}; a kernel given counts reads its lengths there.
-__global__ void k_pass_begin(const int *ctr, int *counts) {
-    counts[0] = ctr[kCtrNext];
-    counts[1] = ctr[kCtrShadow];
-    counts[2] = ctr[kCtrDone];
-}
-
-__global__ void __launch_bounds__(kBlock) k_trace(DeviceScene S, Paths P, int n_closest, int n_shadow,
-                                                  int2 *spill, const int *counts) {
-    __shared__ int s_code[kRing * kBlock];
-    __shared__ float s_dist[kRing * kBlock];
-    if (counts) {
-        n_closest = counts[0];
-        n_shadow = counts[1];
-    }
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) P.ctr[kCtrNext] = 0;
-    Stack stk = make_stack(s_code, s_dist, spill);
-    if (t < n_closest) {
-        const int s = P.q_in[t];
-        float4 o = P.o[s], d = P.d[s];
-        float best = kMax;
-        int hi = -1, ho = -1;
-        trace_ray<false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk);
-        P.hit[s] = make_int2(hi, ho);
-    } else if (t < n_closest + n_shadow) {
-        const int q = t - n_closest;
-        float4 so = P.s_o[q], sd = P.s_d[q];
-        int4 si = P.s_i[q];
-        float best = so.w;
-        int hi = -1, ho = -1;
-        bool blocked = trace_ray<true>(S, v3(so), v3(sd), best, hi, ho, si.y, si.z, stk);
-        if (!blocked) {
-            float4 c = P.s_c[q];
-            float4 L = P.L[si.x];
-            L.x += c.x;
-            L.y += c.y;
-            L.z += c.z;
-            L.w += c.w;
-            P.L[si.x] = L;
-        }
-    }
-}
-
-__device__ __forceinline__ S4 lambertian_albedo(const DeviceScene &S, int4 mt, const Hit &h, const S4 &lam) {
-    return spectrum_texture4(S, mt.y, h, lam);
-}
-
-// Occlusion ray queued by a shading step: origin and light distance,
-// direction, contribution, and {slot, light instance, light object}.
-struct ShadowRay {
-    float4 o, d, c;
-    int4 i;
-};
-
-// Guided vertex record: {features 0-3}, {features 4-5, w.xy}, {w.z,
-// cos / pdf, hero throughput after the vertex, radiance base}; radiance base
-// -2 = no record, -1 = awaiting the next vertex.
-constexpr float kRecNone = -2.0f, kRecPending = -1.0f;
-
-// One iteration body of Renderer::trace (render/renderer.h:102-192) for the
-// path in slot s after its closest-hit query (P.hit[s]).  Updates the slot's
-// state; returns true when the sample's path has ended.  *push_shadow tells
-// whether next-event estimation produced an occlusion ray.
-//
-// kGuided with gout set: a Lambertian vertex samples the next direction from
-// the one-sample mixture 0.5 cos/pi + 0.5 q (q: the guiding distribution in
-// the compact form of guide_compact, features gfeat); the mixture density is the
-// scatter pdf of both MIS weights (next-event estimation and emitter hits),
-// so the estimator stays unbiased.  The vertex is recorded for training in
-// P.grec.  Without gout the vertex is shaded as in the plain tracer.
-template <bool kGuided = false>
-__device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F, const Paths &P, int s,
-                                           bool &push_shadow, ShadowRay &sh, const float *gout = nullptr,
-                                           const float *gfeat = nullptr) {
-    push_shadow = false;
-    {
-        float4 o4 = P.o[s], d4 = P.d[s];
-        int4 info = P.info[s];
-        if (info.w & kFlagGuidePending) {
-            // the radiance gathered so far is the base of the previous guided
-            // vertex's continuation (never set in unguided renders)
-            if (info.z >= 1 && info.z - 1 < F.grec_stride)
-                P.grec[(size_t(s) * F.grec_stride + info.z - 1) * 3 + 2].w = P.L[s].x;
-            info.w &= ~kFlagGuidePending;
-        }
-        int2 hit = P.hit[s];
-        S4 lam = s4(P.lam[s]);
-        S4 beta = s4(P.beta[s]);
-        S4 L = s4(P.L[s]);
-        V3 o = v3(o4), d = v3(d4);
-        float prev_pdf = o4.w;
-        bool specular = (info.w & kFlagSpecular) != 0;
-        const int bounce = info.z;
-        const float total = S.total_power;
-        bool finished = false;
-        Pcg rng = load_rng(P, s);
-
-        if (hit.x < 0) {
-            // Miss: environment with MIS against its NEE (render/renderer.h:105-113).
-            float w = 1;
-            if (!specular && total > 0)
-                w = power_heuristic(prev_pdf, 1 / (4 * kPi) * S.light_pow[S.num_lights - 1] / total);
-            S4 bg = background(S, d, lam);
-            S4 c = beta * bg * w;
-#pragma unroll
-            for (int k = 0; k < kHero; k++) L.v[k] += c.v[k];
-            finished = true;
-        } else {
-            Hit h;
-            if (!hit_record(S, hit.x, hit.y, o, d, h)) {
-                atomicOr(&P.ctr[kCtrError], 1);
-                finished = true;
-            } else {
-                const int mat = material_of(S, hit.x, hit.y);
-                const int4 mt = S.mat[mat];
-                if (mt.x == LAMBDA_MATERIAL_EMISSIVE) {
-                    // Emitter hit (render/renderer.h:119-131).
-                    float w = 1;
-                    if (!specular && total > 0) {
-                        float lp = instance_area(S, hit.x, hit.y) * spectrum_texture_average(S, mt.y);
-                        w = power_heuristic(prev_pdf, instance_pdf(S, hit.x, hit.y, o, d) * lp / total);
-                    }
-                    S4 e = spectrum_texture4(S, mt.y, h, lam);
-                    S4 c = beta * e * w;
-#pragma unroll
-                    for (int k = 0; k < kHero; k++) L.v[k] += c.v[k];
-                    finished = true;
-                } else {
-                    const bool lambertian = mt.x == LAMBDA_MATERIAL_LAMBERTIAN;
-                    S4 albedo = s4(0);
-                    if (lambertian) albedo = lambertian_albedo(S, mt, h, lam);
-                    const bool guided = kGuided && lambertian && gout != nullptr;
-                    if (lambertian && total > 0) {
-                        // Next-event estimation (render/renderer.h:133-175).
-                        float target = total * rng.next();
-                        float cum = 0;
-                        int2 lio = S.light_io[0];
-                        float lp = S.light_pow[0];
-                        for (int j = 0; j < S.num_lights; j++) {
-                            cum += S.light_pow[j];
-                            if (cum >= target) {
-                                lio = S.light_io[j];
-                                lp = S.light_pow[j];
-                                break;
-                            }
-                        }
-                        V3 ldir = lio.x != -1 ? instance_sample(S, lio.x, lio.y, h.p, rng) : random_unit_vector(rng);
-                        float lprob = lio.x != -1 ? instance_pdf(S, lio.x, lio.y, h.p, ldir) : 1 / (4 * kPi);
-                        float ct = dot(h.n, ldir);
-                        if (lprob > 0 && ct > 0) {
-                            lprob *= lp / total;
-                            float tmax = kMax;
-                            S4 em;
-                            bool visible_light = true;
-                            if (lio.x != -1) {
-                                // The occlusion query first intersects the
-                                // light itself; a miss counts as occluded
-                                // (scene/scene.h:61-70).
-                                Hit lh;
-                                if (hit_record(S, lio.x, lio.y, h.p, ldir, lh)) {
-                                    tmax = lh.t;
-                                    em = spectrum_texture4(S, S.mat[material_of(S, lio.x, lio.y)].y, lh, lam);
-                                } else {
-                                    visible_light = false;
-                                }
-                            } else {
-                                em = background(S, ldir, lam);
-                            }
-                            if (visible_light) {
-                                S4 att = s4(0);
-                                if (dot(h.n, ldir) > 0) att = albedo / kPi;
-                                float sp = ct > 0 ? ct / kPi : 0;
-                                if constexpr (kGuided) {
-                                    if (guided) sp = 0.5f * sp + 0.5f * guide_pdf_c(gout, ldir);
-                                }
-                                float w = power_heuristic(lprob, sp);
-                                S4 c = beta * att * ct * em * w / lprob;
-                                push_shadow = true;
-                                sh.o = make_float4(h.p.x, h.p.y, h.p.z, tmax);
-                                sh.d = make_float4(ldir.x, ldir.y, ldir.z, 0);
-                                sh.c = f4(c);
-                                sh.i = make_int4(s, lio.x, lio.y, 0);
-                            }
-                        }
-                    }
-                    // BSDF sampling (scene/material.h:117-261).
-                    S4 att;
-                    V3 nd;
-                    if (lambertian) {
-                        bool from_guide = false;
-                        if constexpr (kGuided) {
-                            if (guided && rng.next() >= 0.5f) {
-                                float u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
-                                nd = guide_sample_c(gout, u0, u1, u2);
-                                from_guide = true;
-                            }
-                        }
-                        if (!from_guide) nd = random_in_hemisphere(h.n, rng);
-                        att = dot(h.n, nd) <= 0 ? s4(0) : albedo / kPi;
-                    } else if (mt.x == LAMBDA_MATERIAL_MIRROR) {
-                        nd = reflected(d, h.n);
-                        att = spectrum_texture4(S, mt.y, h, lam);
-                    } else if (mt.x == LAMBDA_MATERIAL_DIELECTRIC) {
-                        float factor = powf(1 + dot(d, h.n), 5.0f);
-                        const float *s0 = S.spectra + size_t(mt.y) * kBins;
-                        const float *s1 = S.spectra + size_t(mt.z) * kBins;
-                        float n0h = dense(s0, lam.v[0]);
-                        float n1h = dense(s1, lam.v[0]);
-                        float ratio = h.front ? n0h / n1h : n1h / n0h;
-                        float r0 = (n0h - n1h) / (n0h + n1h);
-                        r0 *= r0;
-                        float rh = r0 + (1 - r0) * factor;
-                        bool refl = rng.next() < rh;
-                        nd = refl ? reflected(d, h.n) : refracted(d, h.n, ratio);
-                        att.v[0] = 1;
-#pragma unroll
-                        for (int j = 1; j < kHero; j++) {
-                            float n0 = dense(s0, lam.v[j]);
-                            float n1 = dense(s1, lam.v[j]);
-                            float rr = (n0 - n1) / (n0 + n1);
-                            rr *= rr;
-                            float rj = rr + (1 - rr) * factor;
-                            att.v[j] = refl ? rj / rh
-                                            : (fabsf(n0 - n0h) < kEps && fabsf(n1 - n1h) < kEps) ? (1 - rj) / (1 - rh) : 0;
-                        }
-                    } else {
-                        // Thin film: hero wavelength picks reflect/refract.
-                        const int layers = mt.y;
-                        float thick[kMaxLayers];
-                        for (int j = 0; j < layers && j < kMaxLayers; j++)
-                            thick[j] = scalar_texture(S, S.props[mt.w + j], h.p);
-                        const float2 *c_out = S.cspectra + size_t(S.props[mt.z]) * kBins;
-                        const float2 *c_in = S.cspectra + size_t(S.props[mt.z + layers + 1]) * kBins;
-                        float n0h = dense_c(c_out, lam.v[0]).x;
-                        float n1h = dense_c(c_in, lam.v[0]).x;
-                        float ratio = h.front ? n0h / n1h : n1h / n0h;
-                        float rh = thinfilm_reflectance(S, mt, h.front, h.n, d, lam.v[0], thick);
-                        bool refl = rng.next() < rh;
-                        nd = refl ? reflected(d, h.n) : refracted(d, h.n, ratio);
-                        att.v[0] = 1;
-                        for (int j = 1; j < kHero; j++) {
-                            float n0 = dense_c(c_out, lam.v[j]).x;
-                            float n1 = dense_c(c_in, lam.v[j]).x;
-                            float rj = thinfilm_reflectance(S, mt, h.front, h.n, d, lam.v[j], thick);
-                            att.v[j] = refl ? rj / rh
-                                            : (fabsf(n0 - n0h) < kEps && fabsf(n1 - n1h) < kEps) ? (1 - rj) / (1 - rh) : 0;
-                        }
-                    }
-                    o = h.p;
-                    d = nd;
-                    float ct = dot(h.n, d);
-                    prev_pdf = lambertian ? (ct > 0 ? ct / kPi : 0) : 0;
-                    if constexpr (kGuided) {
-                        if (guided) {
-                            prev_pdf = 0.5f * prev_pdf + 0.5f * guide_pdf_c(gout, d);
-                            // below the surface the contribution is zero
-                            if (!(ct > 0) || !(prev_pdf > 0)) finished = true;
-                        }
-                    }
-                    specular = !lambertian;
-                    if (specular) {
-                        beta = beta * att;
-                    } else {
-                        S4 f = att * ct / prev_pdf;
-                        beta = beta * f;
-                    }
-                    if (bounce >= kRRStart) {
-                        float qq = 1 - clampf(s4max(beta), float(0.05), float(0.95));
-                        if (rng.next() < qq) finished = true;
-                        else {
-#pragma unroll
-                            for (int k = 0; k < kHero; k++) beta.v[k] /= 1 - qq;
-                        }
-                    }
-                    if (bounce >= F.depth) finished = true;
-                    if constexpr (kGuided) {
-                        if (guided && !finished && bounce < F.grec_stride) {
-                            float4 *r = P.grec + (size_t(s) * F.grec_stride + bounce) * 3;
-                            r[0] = make_float4(gfeat[0], gfeat[1], gfeat[2], gfeat[3]);
-                            r[1] = make_float4(gfeat[4], gfeat[5], d.x, d.y);
-                            r[2] = make_float4(d.z, ct / prev_pdf, beta.v[0], kRecPending);
-                            info.w |= kFlagGuidePending;
-                        }
-                    }
-                }
-            }
-        }
-        P.o[s] = make_float4(o.x, o.y, o.z, prev_pdf);
-        P.d[s] = make_float4(d.x, d.y, d.z, 0);
-        P.beta[s] = f4(beta);
-        P.L[s] = f4(L);
-        store_rng(P, s, rng);
-        P.info[s] = make_int4(info.x, info.y, bounce + 1,
-                              (specular ? kFlagSpecular : 0) | (finished ? kFlagFinished : 0) |
-                                  (info.w & kFlagGuidePending));
-        return finished;
-    }
-}
-
-// Shading of every queued path.
-__global__ void __launch_bounds__(kBlock) k_shade(DeviceScene S, Frame F, Paths P, int count, const int *counts) {
-    if (counts) count = counts[0];
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = q < count;
-    bool push_shadow = false;
-    bool path_done = false;
-    int s = 0;
-    ShadowRay sh;
-    if (active) {
-        s = P.q_in[q];
-        path_done = shade_slot(S, F, P, s, push_shadow, sh);
-    }
-    int pos = wave_push(&P.ctr[kCtrShadow], push_shadow);
-    if (push_shadow) {
-        P.s_o[pos] = sh.o;
-        P.s_d[pos] = sh.d;
-        P.s_c[pos] = sh.c;
-        P.s_i[pos] = sh.i;
-    }
-    // Continuing paths trace again next pass; ended ones wait for their
-    // occlusion ray and are then retired by finalize.
-    const bool cont = active && !path_done;
-    const int pn = wave_push(&P.ctr[kCtrNext], cont);
-    if (cont) P.q_out[pn] = s;
-    const bool ended = active && path_done;
-    const int pd = wave_push(&P.ctr[kCtrDone], ended);
-    if (ended) P.done[pd] = s;
-}
-
-// XYZ of one sample's radiance: the first half of spectrumToRGB
-// (core/utils.h:212-228), in float, normalised by 106.856895.
-__device__ __forceinline__ void sample_xyz(const DeviceScene &S, const Frame &F, float4 Lr, float4 lamr, float &x,
-                                           float &y, float &z) {
-    const float L[4] = {Lr.x, Lr.y, Lr.z, Lr.w};
-    const float lam[4] = {lamr.x, lamr.y, lamr.z, lamr.w};
-    x = 0;
-    y = 0;
-    z = 0;
-    float weight = F.lrange / kHero;
-#pragma unroll
-    for (int i = 0; i < kHero; i++) {
-        float l = lam[i];
-        int lo = int(l);
-        float c[3];
-        if (lo == 830) {
-            c[0] = S.cie[(lo - 360) * 3 + 0];
-            c[1] = S.cie[(lo - 360) * 3 + 1];
-            c[2] = S.cie[(lo - 360) * 3 + 2];
-        } else {
-            int hi = lo + 1;
-#pragma unroll
-            for (int k = 0; k < 3; k++)
-                c[k] = (float(hi) - l) * S.cie[(lo - 360) * 3 + k] + (l - float(lo)) * S.cie[(hi - 360) * 3 + k];
-        }
-        x += c[0] * L[i] * weight;
-        y += c[1] * L[i] * weight;
-        z += c[2] * L[i] * weight;
-    }
-    x /= float(106.856895);
-    y /= float(106.856895);
-    z /= float(106.856895);
-}
-
-// Retires the samples that ended in the previous shade pass (their
-// occlusion rays have been traced since): radiance to the framebuffer, then
-// the slot's next sample, which joins the closest-hit queue.
-// Paths queued for the guiding network in a guided pass.
-struct GuideQueue {
-    int *slot;       // [n] slot of each queued path
-    float *feat;     // [n][6] features
-    float *out;      // [n][40] network outputs
-};
-
-// Guided pass, step 1: paths on a Lambertian surface within the guided
-// bounces queue their features for the network, all others are shaded at
-// once (shading closes the record of a path's previous guided vertex).
-__global__ void __launch_bounds__(kBlock) k_shade_guided_queue(DeviceScene S, Frame F, Paths P, int count,
-                                                               GuideQueue Q) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = q < count;
-    bool guided = false, push_shadow = false, path_done = false;
-    int s = 0;
-    ShadowRay sh;
-    GuideFeatures gf;
-    if (active) {
-        s = P.q_in[q];
-        const int4 info = P.info[s];
-        if (info.z < F.grec_stride) P.grec[(size_t(s) * F.grec_stride + info.z) * 3 + 2].w = kRecNone;
-        const int2 hit = P.hit[s];
-        if (hit.x >= 0 && info.z < F.guide_bounces &&
-            S.mat[material_of(S, hit.x, hit.y)].x == LAMBDA_MATERIAL_LAMBERTIAN) {
-            const float4 o4 = P.o[s], d4 = P.d[s];
-            Hit h;
-            if (hit_record(S, hit.x, hit.y, v3(o4), v3(d4), h)) {
-                gf = guide_features(h.p, v3(F.box_lo.x, F.box_lo.y, F.box_lo.z), v3(F.box_hi.x, F.box_hi.y, F.box_hi.z),
-                                    h.n, P.lam[s].x);
-                guided = true;
-            }
-        }
-        if (!guided) path_done = shade_slot<false>(S, F, P, s, push_shadow, sh);
-    }
-    const int gp = wave_push(&P.ctr[kCtrGuided], guided);
-    if (guided) {
-        Q.slot[gp] = s;
-#pragma unroll
-        for (int k = 0; k < kGuideScalars; k++) Q.feat[size_t(gp) * kGuideScalars + k] = gf.f[k];
-    }
-    const bool plain = active && !guided;
-    int pos = wave_push(&P.ctr[kCtrShadow], push_shadow);
-    if (push_shadow) {
-        P.s_o[pos] = sh.o;
-        P.s_d[pos] = sh.d;
-        P.s_c[pos] = sh.c;
-        P.s_i[pos] = sh.i;
-    }
-    const bool cont = plain && !path_done;
-    const int pn = wave_push(&P.ctr[kCtrNext], cont);
-    if (cont) P.q_out[pn] = s;
-    const bool ended = plain && path_done;
-    const int pd = wave_push(&P.ctr[kCtrDone], ended);
-    if (ended) P.done[pd] = s;
-}
-
-// Guided pass, step 2: the queued Lambertian paths shade with the network's
-// mixture (P.ctr[kCtrGuided] of them).
-__global__ void __launch_bounds__(kBlock) k_shade_guided(DeviceScene S, Frame F, Paths P, GuideQueue Q) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = q < P.ctr[kCtrGuided];
-    bool push_shadow = false, path_done = false;
-    int s = 0;
-    ShadowRay sh;
-    if (active) {
-        s = Q.slot[q];
-        float *lobes = Q.out + size_t(q) * kLobeOut;
-        guide_compact(lobes, lobes);
-        path_done = shade_slot<true>(S, F, P, s, push_shadow, sh, lobes, Q.feat + size_t(q) * kGuideScalars);
-    }
-    int pos = wave_push(&P.ctr[kCtrShadow], push_shadow);
-    if (push_shadow) {
-        P.s_o[pos] = sh.o;
-        P.s_d[pos] = sh.d;
-        P.s_c[pos] = sh.c;
-        P.s_i[pos] = sh.i;
-    }
-    const bool cont = active && !path_done;
-    const int pn = wave_push(&P.ctr[kCtrNext], cont);
-    if (cont) P.q_out[pn] = s;
-    const bool ended = active && path_done;
-    const int pd = wave_push(&P.ctr[kCtrDone], ended);
-    if (ended) P.done[pd] = s;
-}
-
-// 32-bit integer hash of three words (lowbias32 finaliser by C. Wellons).
-__device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c) {
-    uint32_t x = a * 0x9e3779b1u ^ (b + 0x7f4a7c15u) * 0x85ebca6bu ^ (c + 0x165667b1u) * 0xc2b2ae35u;
-    x ^= x >> 16;
-    x *= 0x7feb352du;
-    x ^= x >> 15;
-    x *= 0x846ca68bu;
-    x ^= x >> 16;
-    return x;
-}
-
-// Adds the ended sample of slot s to its pixel's XYZ sum.  In reference
-// mode the pixel's next sample then starts in the same slot, continuing the
-// pixel's PCG32 stream (render/renderer.h:69-91); returns true when it did.
-// A pixel that reaches the launch's last sample before spp saves its stream.
-__device__ __forceinline__ bool retire_sample(const DeviceScene &S, const Frame &F, const Paths &P, int s) {
-    int4 info = P.info[s];
-    if (F.train) {
-        // training samples from the sample's guided vertices: the radiance
-        // that reached vertex b along its sampled direction is the radiance
-        // gathered after it over the throughput after it (hero wavelength)
-        const float Lend = P.L[s].x;
-        const int nb = min(min(info.z, F.grec_stride), F.guide_bounces);
-        for (int b = 0; b < nb; b++) {
-            float4 *r = P.grec + (size_t(s) * F.grec_stride + b) * 3;
-            const float4 r2 = r[2];
-            if (!(r2.w >= 0)) continue;
-            r[2].w = kRecNone;
-            const float c = (Lend - r2.w) / r2.z * r2.y;
-            if (!(c > 0) || !isfinite(c)) continue;
-            // a uniform subsample of the candidates (by a hash of slot,
-            // sample and vertex), so that the buffer does not fill with the
-            // paths that end first
-            atomicAdd(&P.ctr[kCtrTrainSeen], 1);
-            const uint32_t hsh = hash3(uint32_t(s), uint32_t(info.y) * 64u + uint32_t(b), uint32_t(F.seed));
-            if (float(hsh >> 8) * (1.0f / 16777216.0f) >= F.train_keep) continue;
-            const int k = atomicAdd(&P.ctr[kCtrTrain], 1);
-            if (k >= F.train_cap) continue;
-            const float4 r0 = r[0], r1 = r[1];
-            float *o = F.train + size_t(k) * kGuideSample;
-            o[0] = r0.x; o[1] = r0.y; o[2] = r0.z; o[3] = r0.w; o[4] = r1.x; o[5] = r1.y;
-            o[6] = r1.z; o[7] = r1.w; o[8] = r2.x; o[9] = c;
-        }
-    }
-    float x, y, z;
-    sample_xyz(S, F, P.L[s], P.lam[s], x, y, z);
-    double *px = P.fb + size_t(info.x) * 3;
-    unsafeAtomicAdd(px + 0, double(x));
-    unsafeAtomicAdd(px + 1, double(y));
-    unsafeAtomicAdd(px + 2, double(z));
-    if (F.mode == LAMBDA_RNG_REFERENCE && info.y + 1 < (F.sample_end ? F.sample_end : F.spp)) {
-        Pcg rng = load_rng(P, s);
-        camera_sample(S, F, P, s, info.x, info.y + 1, rng);
-        store_rng(P, s, rng);
-        return true;
-    }
-    if (F.mode == LAMBDA_RNG_REFERENCE && info.y + 1 < F.spp) {
-        F.rng_save[info.x] = P.rng[s];  // paused: a later launch resumes the stream here
-        return false;
-    }
-    atomicAdd(&P.ctr[kCtrPixels], 1);
-    return false;
-}
-
-// Retires the samples that ended in the previous shade pass (their
-// occlusion rays have been traced since): radiance to the framebuffer, then
-// the slot's next sample, which joins the closest-hit queue.
-__global__ void __launch_bounds__(kBlock) k_finalize(DeviceScene S, Frame F, Paths P, int n_done,
-                                                     const int *counts) {
-    if (counts) n_done = counts[2];
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q == 0) {
-        P.ctr[kCtrShadow] = 0;
-        P.ctr[kCtrDone] = 0;
-        P.ctr[kCtrGuided] = 0;
-    }
-    bool keep = false;
-    bool need_work = false;
-    int s = 0;
-    if (q < n_done) {
-        s = P.done[q];
-        keep = retire_sample(S, F, P, s);
-        need_work = !keep;
-    }
-    // a slot whose unit is finished takes the next one
-    int w = wave_push(&P.ctr[kCtrWork], need_work);
-    if (need_work && w < work_units(F)) {
-        start_unit(S, F, P, s, w);
-        keep = true;
-    }
-    int pos = wave_push(&P.ctr[kCtrNext], keep);
-    if (keep) P.q_out[pos] = s;
-}
-
-// Persistent per-thread schedule: each thread takes one live slot and runs
-// it -- closest hit, shading, the occlusion ray at once, retirement, the
-// slot's next sample, then further work units -- without waiting for other
-// paths.  Threads [0, n_in) take q_in (paths that need their closest hit);
-// the next P.ctr[kCtrNext] threads take q_out (samples just started by
-// k_finalize).  No occlusion ray or ended sample may be pending when it
-// starts.  Threads exit when no work unit is left.
-__global__ void __launch_bounds__(kBlock, kPersistWaves) k_persist(DeviceScene S, Frame F, Paths P, int n_in,
-                                                                 int2 *spill) {
-    __shared__ int s_code[kRing * kBlock];
-    __shared__ float s_dist[kRing * kBlock];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n_new = P.ctr[kCtrNext];
-    if (t >= n_in + n_new) return;
-    const int s = t < n_in ? P.q_in[t] : P.q_out[t - n_in];
-    Stack stk = make_stack(s_code, s_dist, spill);
-    unsigned n_closest = 0, n_shadow = 0;
-    if (F.cost) F.unit_t0[s] = wall_clock64();
-    while (true) {
-        n_closest++;
-        {
-            float4 o = P.o[s], d = P.d[s];
-            float best = kMax;
-            int hi = -1, ho = -1;
-            stk.sp = stk.lo = 0;
-            trace_ray<false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk);
-            P.hit[s] = make_int2(hi, ho);
-        }
-        bool push = false;
-        ShadowRay sh;
-        const bool ended = shade_slot(S, F, P, s, push, sh);
-        if (push) {
-            n_shadow++;
-            float best = sh.o.w;
-            int hi = -1, ho = -1;
-            stk.sp = stk.lo = 0;
-            if (!trace_ray<true>(S, v3(sh.o), v3(sh.d), best, hi, ho, sh.i.y, sh.i.z, stk)) {
-                float4 L = P.L[s];
-                L.x += sh.c.x;
-                L.y += sh.c.y;
-                L.z += sh.c.z;
-                L.w += sh.c.w;
-                P.L[s] = L;
-            }
-        }
-        if (!ended) continue;
-        if (retire_sample(S, F, P, s)) continue;
-        if (F.cost) {
-            // device-clock time of the unit (its pixel's samples in this launch)
-            const unsigned long long now = wall_clock64();
-            const unsigned long long dt = now - F.unit_t0[s];
-            atomicAdd(F.cost + P.info[s].x, int(dt < (1ull << 30) ? dt : (1ull << 30)));
-            F.unit_t0[s] = now;
-        }
-        const int w = atomicAdd(&P.ctr[kCtrWork], 1);
-        if (w >= work_units(F)) break;
-        start_unit(S, F, P, s, w);
-    }
-    atomicAdd(&P.rays[0], (unsigned long long)n_closest);
-    atomicAdd(&P.rays[1], (unsigned long long)n_shadow);
-}
-
-// Waves per SIMD of the guided persistent kernel (its LDS, stacks plus the
-// network's staging, allows two blocks per CU).  Staging in the traversal
-// stacks' LDS instead, at 3 waves (168 VGPRs), spilled 158 registers and
-// ran 4 % slower on the guiding room.
-constexpr int kPersistGuidedWaves = 2;
-
-// The persistent schedule with the guiding network evaluated inside the
-// kernel.  Every thread runs its paths as in k_persist; after each closest
-// hit, the lanes at a guided vertex (Lambertian, bounce index below
-// F.guide_bounces) put their features in the wave's LDS rows, and if any
-// lane has one, the whole wave runs the network on the matrix cores for the
-// guided lanes only (packed into the first rows, empty 16-row tiles
-// skipped) and each guided lane shades with its own row of outputs.  The
-// loop ends for the wave together, when no lane has a path.
-__global__ void __launch_bounds__(kBlock, kPersistGuidedWaves) k_persist_guided(DeviceScene S, Frame F, Paths P,
-                                                                                int n_in, int2 *spill, GuideDev G) {
-    __shared__ int s_code[kRing * kBlock];
-    __shared__ float s_dist[kRing * kBlock];
-    // per wave: features [64][6] in, network outputs [64][40] out
-    __shared__ __attribute__((aligned(16))) float s_io[kBlock / 64][64 * kLobeOut];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    float *io = s_io[threadIdx.x >> 6];
-    const int n_new = P.ctr[kCtrNext];
-    bool alive = t < n_in + n_new;
-    const int s = alive ? (t < n_in ? P.q_in[t] : P.q_out[t - n_in]) : 0;
-    Stack stk = make_stack(s_code, s_dist, spill);
-    unsigned n_closest = 0, n_shadow = 0;
-    while (__any(alive)) {
-        bool guided = false;
-        GuideFeatures gf{};
-        if (alive) {
-            n_closest++;
-            float4 o = P.o[s], d = P.d[s];
-            float best = kMax;
-            int hi = -1, ho = -1;
-            stk.sp = stk.lo = 0;
-            trace_ray<false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk);
-            P.hit[s] = make_int2(hi, ho);
-            const int bounce = P.info[s].z;
-            if (bounce < F.grec_stride && bounce < F.guide_bounces)
-                P.grec[(size_t(s) * F.grec_stride + bounce) * 3 + 2].w = kRecNone;
-            if (hi >= 0 && bounce < F.guide_bounces && S.mat[material_of(S, hi, ho)].x == LAMBDA_MATERIAL_LAMBERTIAN) {
-                Hit h;
-                if (hit_record(S, hi, ho, v3(o), v3(d), h)) {
-                    gf = guide_features(h.p, v3(F.box_lo.x, F.box_lo.y, F.box_lo.z),
-                                        v3(F.box_hi.x, F.box_hi.y, F.box_hi.z), h.n, P.lam[s].x);
-                    guided = true;
-                }
-            }
-        }
-        // the wave's guided lanes take rows 0 .. n-1 in lane order
-        const unsigned long long gmask = __ballot(guided);
-        const int grow = __popcll(gmask & ((1ull << lane) - 1ull));
-        if (gmask) {
-            // every lane of the wave is here (the condition is uniform)
-            if (guided) {
-#pragma unroll
-                for (int k = 0; k < kGuideScalars; k++) io[grow * kGuideScalars + k] = gf.f[k];
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // reads the features in its first layer, writes [n][40] at the end
-            mlp_rows64_call(G, io, kGuideScalars, __popcll(gmask), io, kLobeOut);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // each guided lane turns its own row into the compact mixture
-            if (guided) guide_compact(io + grow * kLobeOut, io + grow * kLobeOut);
-        }
-        if (alive) {
-            bool push = false;
-            ShadowRay sh;
-            const bool ended = shade_slot<true>(S, F, P, s, push, sh, guided ? io + grow * kLobeOut : nullptr, gf.f);
-            if (push) {
-                n_shadow++;
-                float best = sh.o.w;
-                int hi = -1, ho = -1;
-                stk.sp = stk.lo = 0;
-                if (!trace_ray<true>(S, v3(sh.o), v3(sh.d), best, hi, ho, sh.i.y, sh.i.z, stk)) {
-                    float4 L = P.L[s];
-                    L.x += sh.c.x;
-                    L.y += sh.c.y;
-                    L.z += sh.c.z;
-                    L.w += sh.c.w;
-                    P.L[s] = L;
-                }
-            }
-            if (ended && !retire_sample(S, F, P, s)) {
-                const int w = atomicAdd(&P.ctr[kCtrWork], 1);
-                if (w < work_units(F)) start_unit(S, F, P, s, w);
-                else alive = false;
-            }
-        }
-    }
-    atomicAdd(&P.rays[0], (unsigned long long)n_closest);
-    atomicAdd(&P.rays[1], (unsigned long long)n_shadow);
-}
-
-// ------------------------------------------------------ batched serving
-//
-// One persistent launch renders a batch of jobs.  A job is a scene and a
-// frame rendered with reference streams; its units are its pixels, numbered
-// consecutively across the batch in arrival order.  Threads claim units
-// from one counter; a thread that claims a unit of a job that has not
-// arrived yet sleeps until the job's arrival time on the device wall clock.
-// Each unit runs all samples of its pixel, so every job's image equals the
-// image of a single render of the same frame bit for bit.  The thread that
-// finishes a job's last pixel records the finishing time.
-struct ServeJob {
-    Frame F;                        // reference streams; fb below
-    double *fb;                     // XYZ sums of the job, [pixel][3]
-    long long first_unit;           // first unit of the job in the batch
-    unsigned long long arrival;     // wall-clock ticks after the batch start
-    int scene;                      // index into the scene table
-    int units;                      // pixels
-};
-
-struct ServeCtl {
-    unsigned long long t0;          // wall clock at the batch start
-    int next_unit;                  // unit counter
-    int pad;
-};
-
-__global__ void k_serve_clock(ServeCtl *ctl) {
-    ctl->t0 = wall_clock64();
-    ctl->next_unit = 0;
-}
-
-__global__ void __launch_bounds__(kBlock, kPersistWaves) k_serve(const DeviceScene *scenes, const ServeJob *jobs,
-                                                                  int njobs, long long total_units, Paths P,
-                                                                  ServeCtl *ctl, int *done, unsigned long long *finish,
-                                                                  int2 *spill) {
-    __shared__ int s_code[kRing * kBlock];
-    __shared__ float s_dist[kRing * kBlock];
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack stk = make_stack(s_code, s_dist, spill);
-    const unsigned long long t0 = ctl->t0;
-    unsigned n_closest = 0, n_shadow = 0;
-    int j = 0;
-    while (true) {
-        const int w = atomicAdd(&ctl->next_unit, 1);
-        if (w >= total_units) break;
-        // claims only grow, so the job index only moves forward
-        while (j + 1 < njobs && w >= jobs[j + 1].first_unit) j++;
-        const ServeJob &J = jobs[j];
-        const unsigned long long due = t0 + J.arrival;
-        while (wall_clock64() < due) __builtin_amdgcn_s_sleep(127);
-        const DeviceScene &S = scenes[J.scene];
-        const Frame &F = J.F;
-        Paths Pj = P;
-        Pj.fb = J.fb;
-        const int pixel = int(w - J.first_unit);
-        {
-            Pcg rng = pcg_seed(F.seed, uint64_t(pixel));
-            camera_sample(S, F, Pj, s, pixel, 0, rng);
-            store_rng(Pj, s, rng);
-        }
-        while (true) {
-            n_closest++;
-            {
-                float4 o = Pj.o[s], d = Pj.d[s];
-                float best = kMax;
-                int hi = -1, ho = -1;
-                stk.sp = stk.lo = 0;
-                trace_ray<false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk);
-                Pj.hit[s] = make_int2(hi, ho);
-            }
-            bool push = false;
-            ShadowRay sh;
-            const bool ended = shade_slot(S, F, Pj, s, push, sh);
-            if (push) {
-                n_shadow++;
-                float best = sh.o.w;
-                int hi = -1, ho = -1;
-                stk.sp = stk.lo = 0;
-                if (!trace_ray<true>(S, v3(sh.o), v3(sh.d), best, hi, ho, sh.i.y, sh.i.z, stk)) {
-                    float4 L = Pj.L[s];
-                    L.x += sh.c.x;
-                    L.y += sh.c.y;
-                    L.z += sh.c.z;
-                    L.w += sh.c.w;
-                    Pj.L[s] = L;
-                }
-            }
-            if (!ended) continue;
-            if (retire_sample(S, F, Pj, s)) continue;
-            break;
-        }
-        if (atomicAdd(&done[j], 1) == J.units - 1) finish[j] = wall_clock64();
-    }
-    atomicAdd(&P.rays[0], (unsigned long long)n_closest);
-    atomicAdd(&P.rays[1], (unsigned long long)n_shadow);
-}
-
-// Mean XYZ over samples to linear sRGB (core/utils.h:230), in double.
-__global__ void k_resolve(Frame F, const double *fb, float *rgb) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= F.npix) return;
-    double x = fb[3 * size_t(p) + 0] / F.spp;
-    double y = fb[3 * size_t(p) + 1] / F.spp;
-    double z = fb[3 * size_t(p) + 2] / F.spp;
-    rgb[3 * size_t(p) + 0] = float(x * double(float(3.2406255)) - y * double(float(1.5372080)) - z * double(float(0.4986286)));
-    rgb[3 * size_t(p) + 1] = float(-x * double(float(0.9689307)) + y * double(float(1.8757561)) + z * double(float(0.0415175)));
-    rgb[3 * size_t(p) + 2] = float(x * double(float(0.0557101)) - y * double(float(0.2040211)) + z * double(float(1.0569959)));
-}
-
-__global__ void k_set_i(int *p, int v) { *p = v; }
-
-__global__ void k_fill_d(double *p, size_t n) {
-    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0;
-}
-
-__global__ void k_fill_u64(unsigned long long *p, size_t n) {
-    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-__global__ void k_fill_i(int *p, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
-// ------------------------------------------------------------ query kernels
-
-__global__ void __launch_bounds__(kBlock) k_query_closest(DeviceScene S, const float *rays, int n, lambda_hit *out,
-                                                          int2 *spill) {
-    __shared__ int s_code[kRing * kBlock];
-    __shared__ float s_dist[kRing * kBlock];
-    Stack stk = make_stack(s_code, s_dist, spill);
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    V3 o = v3(rays[6 * i + 0], rays[6 * i + 1], rays[6 * i + 2]);
-    V3 d = v3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
-    float best = kMax;
-    int hi = -1, ho = -1;
-    bool ok = trace_ray<false>(S, o, d, best, hi, ho, -1, -2, stk);
-    lambda_hit r;
-    r.hit = 0;
-    r.instance = -1;
-    r.object = -1;
-    r.t = kMax;
-    r.point[0] = r.point[1] = r.point[2] = 0;
-    r.normal[0] = r.normal[1] = r.normal[2] = 0;
-    r.u = r.v = 0;
-    r.front_facing = 0;
-    if (ok) {
-        Hit h;
-        if (hit_record(S, hi, ho, o, d, h)) {
-            r.hit = 1;
-            r.instance = hi;
-            r.object = ho;
-            r.t = h.t;
-            r.point[0] = h.p.x; r.point[1] = h.p.y; r.point[2] = h.p.z;
-            r.normal[0] = h.n.x; r.normal[1] = h.n.y; r.normal[2] = h.n.z;
-            r.u = h.u;
-            r.v = h.v;
-            r.front_facing = h.front ? 1 : 0;
-        }
-    }
-    out[i] = r;
-}
-
-__global__ void __launch_bounds__(kBlock) k_query_occluded(DeviceScene S, const float *rays, const int *li,
-                                                           const int *lo, int n, int *out, int2 *spill) {
-    __shared__ int s_code[kRing * kBlock];
-    __shared__ float s_dist[kRing * kBlock];
-    Stack stk = make_stack(s_code, s_dist, spill);
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    V3 o = v3(rays[6 * i + 0], rays[6 * i + 1], rays[6 * i + 2]);
-    V3 d = v3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
-    float best = kMax;
-    int inst = li[i], obj = lo[i];
-    if (inst != -1) {
-        Hit lh;
-        if (!hit_record(S, inst, obj, o, d, lh)) {
-            out[i] = 1;
-            return;
-        }
-        best = lh.t;
-    }
-    int hi = -1, ho = -1;
-    out[i] = trace_ray<true>(S, o, d, best, hi, ho, inst, obj, stk) ? 1 : 0;
-}
-
-__global__ void k_query_thinfilm(DeviceScene S, int mat, const float *q, const float *pts, int n, float *out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float c = q[4 * i + 0];
-    float lambda = q[4 * i + 1];
-    bool front = q[4 * i + 2] != 0;
-    float s = sqrtf(fmaxf(0.0f, 1 - c * c));
-    V3 p = v3(pts[3 * i + 0], pts[3 * i + 1], pts[3 * i + 2]);
-    int4 mt = S.mat[mat];
-    float thick[kMaxLayers];
-    for (int j = 0; j < mt.y && j < kMaxLayers; j++) thick[j] = scalar_texture(S, S.props[mt.w + j], p);
-    out[i] = thinfilm_reflectance(S, mt, front, v3(0, 0, 1), v3(s, 0, -c), lambda, thick);
-}
 
 // ------------------------------------------------------------ host: scene
 
@@ -1556,17 +395,16 @@ struct EventPair {
 };
 
 
-// Threads of k_persist resident on the device at once.
-template <typename Kernel>
-long long resident_threads(Kernel kernel) {
-    int dev = 0, cus = 0, blocks = 0;
+// Threads resident on the device at once for a kernel with `blocks` blocks
+// of kBlock threads per compute unit.
+long long resident_threads(int blocks) {
+    int dev = 0, cus = 0;
     HIP_OK(hipGetDevice(&dev));
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kBlock, 0));
     return (long long)std::max(1, cus) * std::max(1, blocks) * kBlock;
 }
 
-long long resident_persist_threads() { return resident_threads(k_persist); }
+long long resident_persist_threads() { return resident_threads(persist_blocks_per_cu()); }
 
 Frame make_frame(const lambda_render_params &p) {
     Frame F{};  // every optional pointer (cost, train) starts null
@@ -1586,31 +424,6 @@ Frame make_frame(const lambda_render_params &p) {
     return F;
 }
 
-// Device buffers, stream and counters of one render.
-// Visiting order from measured pixel costs: 8x8 tiles (the unit of the
-// default order, which keeps neighbouring paths in flight together) are
-// summed on the device, sorted on the host (longest first) and expanded back
-// to pixels, row order within a tile.  The order changes when a pixel runs,
-// never what it computes.
-__global__ void k_tile_cost(const int *cost, int w, int h, long long *tile_cost) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= w * h) return;
-    const int tx = (p % w) >> 3, ty = (p / w) >> 3;
-    atomicAdd(reinterpret_cast<unsigned long long *>(tile_cost + size_t(ty) * ((w + 7) >> 3) + tx),
-              (unsigned long long)cost[p]);
-}
-
-// Thread (i, k): pixel k of the tile at position i of the sorted list.
-__global__ void k_tile_expand(const int *tiles, const int *offsets, int ntiles, int w, int h, int *order) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = int(e >> 6), k = int(e & 63);
-    if (i >= ntiles) return;
-    const int t = tiles[i], tiles_x = (w + 7) >> 3;
-    const int tx = t % tiles_x, ty = t / tiles_x;
-    const int tw = min(8, w - 8 * tx), th = min(8, h - 8 * ty);
-    if (k >= tw * th) return;
-    order[offsets[i] + k] = (8 * ty + k / tw) * w + 8 * tx + k % tw;
-}
 
 struct Renderer {
     lambda_hip_scene &g;
@@ -1707,13 +520,13 @@ struct Renderer {
             stats->kernel_ms[k] += ms;
             stats->launches[k]++;
         };
-        if (clear) k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
-        k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
-        k_fill_u64<<<1, kBlock, 0, st>>>(P.rays, 2);
+        if (clear) launch_fill_d(grid_for((long long)F.npix * 3), kBlock, st, P.fb, size_t(F.npix) * 3);
+        launch_fill_i(1, kBlock, st, P.ctr, kCtrCount);
+        launch_fill_u64(1, kBlock, st, P.rays, 2);
         if (guide) tail = fused ? kAutoTailPaths : 0;
-        k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns, 0);
+        launch_start(grid_for(ns), kBlock, st, g.ds, F, P, ns, 0);
         // The work counter starts after the slots' first units.
-        k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrWork, ns);
+        launch_set_i(1, 1, st, P.ctr + kCtrWork, ns);
         HIP_OK(hipGetLastError());
         int n_closest = ns, n_shadow = 0, n_done = 0;
         // Every pass advances each live path by one segment; a slot runs at
@@ -1728,15 +541,15 @@ struct Renderer {
                 // settle pending occlusion rays and ended samples, then run
                 // every live slot, and the units still to start, to the end
                 if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
-                k_trace<<<grid_for(n_shadow), kBlock, 0, st>>>(g.ds, P, 0, n_shadow, B.spill.p, nullptr);
+                launch_trace(grid_for(n_shadow), kBlock, st, g.ds, P, 0, n_shadow, B.spill.p, nullptr);
                 if (stats) HIP_OK(hipEventRecord(ev[0].b, st));
-                k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done, nullptr);
+                launch_finalize(grid_for(n_done), kBlock, st, g.ds, F, P, n_done, nullptr);
                 if (stats) HIP_OK(hipEventRecord(ev[3].a, st));
                 if (guide)
-                    k_persist_guided<<<grid_for((long long)n_closest + n_done), kBlock, 0, st>>>(
+                    launch_persist_guided(grid_for((long long)n_closest + n_done), kBlock, st, 
                         g.ds, F, P, n_closest, B.spill.p, guide->dev());
                 else
-                    k_persist<<<grid_for((long long)n_closest + n_done), kBlock, 0, st>>>(g.ds, F, P, n_closest,
+                    launch_persist(grid_for((long long)n_closest + n_done), kBlock, st, g.ds, F, P, n_closest,
                                                                                           B.spill.p);
                 if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
                 HIP_OK(hipGetLastError());
@@ -1753,22 +566,22 @@ struct Renderer {
                 break;
             }
             if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
-            k_trace<<<grid_for((long long)n_closest + n_shadow), kBlock, 0, st>>>(g.ds, P, n_closest, n_shadow, B.spill.p, nullptr);
+            launch_trace(grid_for((long long)n_closest + n_shadow), kBlock, st, g.ds, P, n_closest, n_shadow, B.spill.p, nullptr);
             if (stats) {
                 HIP_OK(hipEventRecord(ev[0].b, st));
                 HIP_OK(hipEventRecord(ev[2].a, st));
             }
-            k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done, nullptr);
+            launch_finalize(grid_for(n_done), kBlock, st, g.ds, F, P, n_done, nullptr);
             if (stats) {
                 HIP_OK(hipEventRecord(ev[2].b, st));
                 HIP_OK(hipEventRecord(ev[1].a, st));
             }
             if (guide) {
-                k_shade_guided_queue<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, n_closest, *Q);
+                launch_shade_guided_queue(grid_for(n_closest), kBlock, st, g.ds, F, P, n_closest, *Q);
                 guide->forward_dev(Q->feat, P.ctr + kCtrGuided, n_closest, Q->out, st);
-                k_shade_guided<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, *Q);
+                launch_shade_guided(grid_for(n_closest), kBlock, st, g.ds, F, P, *Q);
             } else {
-                k_shade<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, n_closest, nullptr);
+                launch_shade(grid_for(n_closest), kBlock, st, g.ds, F, P, n_closest, nullptr);
             }
             if (stats) HIP_OK(hipEventRecord(ev[1].b, st));
             sync_counters();
@@ -1812,32 +625,32 @@ struct Renderer {
             stats->kernel_ms[k] += ms;
             stats->launches[k]++;
         };
-        if (clear) k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
-        k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
-        k_fill_u64<<<1, kBlock, 0, st>>>(P.rays, 2);
+        if (clear) launch_fill_d(grid_for((long long)F.npix * 3), kBlock, st, P.fb, size_t(F.npix) * 3);
+        launch_fill_i(1, kBlock, st, P.ctr, kCtrCount);
+        launch_fill_u64(1, kBlock, st, P.rays, 2);
         // no unit is started after a batch's first ones
-        k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrWork, int(units));
+        launch_set_i(1, 1, st, P.ctr + kCtrWork, int(units));
         HIP_OK(hipGetLastError());
         for (long long base = 0; base < units; base += ns) {
             const int nb = int(std::min<long long>(ns, units - base));
-            k_start<<<grid_for(nb), kBlock, 0, st>>>(g.ds, F, P, nb, base);
+            launch_start(grid_for(nb), kBlock, st, g.ds, F, P, nb, base);
             int n_closest = nb, n_shadow = 0, n_done = 0;
             for (int pass = 0; pass < F.guide_bounces && n_closest > 0; pass++) {
                 if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
-                k_trace<<<grid_for((long long)n_closest + n_shadow), kBlock, 0, st>>>(g.ds, P, n_closest, n_shadow,
+                launch_trace(grid_for((long long)n_closest + n_shadow), kBlock, st, g.ds, P, n_closest, n_shadow,
                                                                                        B.spill.p, nullptr);
                 if (stats) {
                     HIP_OK(hipEventRecord(ev[0].b, st));
                     HIP_OK(hipEventRecord(ev[2].a, st));
                 }
-                k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done, nullptr);
+                launch_finalize(grid_for(n_done), kBlock, st, g.ds, F, P, n_done, nullptr);
                 if (stats) {
                     HIP_OK(hipEventRecord(ev[2].b, st));
                     HIP_OK(hipEventRecord(ev[1].a, st));
                 }
-                k_shade_guided_queue<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, n_closest, *Q);
+                launch_shade_guided_queue(grid_for(n_closest), kBlock, st, g.ds, F, P, n_closest, *Q);
                 guide->forward_dev(Q->feat, P.ctr + kCtrGuided, n_closest, Q->out, st);
-                k_shade_guided<<<grid_for(n_closest), kBlock, 0, st>>>(g.ds, F, P, *Q);
+                launch_shade_guided(grid_for(n_closest), kBlock, st, g.ds, F, P, *Q);
                 if (stats) HIP_OK(hipEventRecord(ev[1].b, st));
                 sync_counters();
                 if (stats) {
@@ -1854,12 +667,12 @@ struct Renderer {
             // settle pending occlusion rays and ended samples, then run the
             // batch's live paths to their ends
             if (stats) HIP_OK(hipEventRecord(ev[0].a, st));
-            k_trace<<<grid_for(n_shadow), kBlock, 0, st>>>(g.ds, P, 0, n_shadow, B.spill.p, nullptr);
+            launch_trace(grid_for(n_shadow), kBlock, st, g.ds, P, 0, n_shadow, B.spill.p, nullptr);
             if (stats) HIP_OK(hipEventRecord(ev[0].b, st));
-            k_finalize<<<grid_for(n_done), kBlock, 0, st>>>(g.ds, F, P, n_done, nullptr);
+            launch_finalize(grid_for(n_done), kBlock, st, g.ds, F, P, n_done, nullptr);
             if (stats) HIP_OK(hipEventRecord(ev[3].a, st));
             if (n_closest + n_done > 0)
-                k_persist<<<grid_for((long long)n_closest + n_done), kBlock, 0, st>>>(g.ds, F, P, n_closest,
+                launch_persist(grid_for((long long)n_closest + n_done), kBlock, st, g.ds, F, P, n_closest,
                                                                                       B.spill.p);
             if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
             sync_counters();
@@ -1891,12 +704,12 @@ struct Renderer {
         passes = std::max(2, passes + (passes & 1));
         DevBuf<int> counts;
         counts.alloc(3);
-        k_fill_d<<<grid_for((long long)F.npix * 3), kBlock, 0, st>>>(P.fb, size_t(F.npix) * 3);
-        k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
-        k_fill_u64<<<1, kBlock, 0, st>>>(P.rays, 2);
-        k_start<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, P, ns, 0);
-        k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrWork, ns);
-        k_set_i<<<1, 1, 0, st>>>(P.ctr + kCtrNext, ns);  // every slot starts in the closest-hit queue
+        launch_fill_d(grid_for((long long)F.npix * 3), kBlock, st, P.fb, size_t(F.npix) * 3);
+        launch_fill_i(1, kBlock, st, P.ctr, kCtrCount);
+        launch_fill_u64(1, kBlock, st, P.rays, 2);
+        launch_start(grid_for(ns), kBlock, st, g.ds, F, P, ns, 0);
+        launch_set_i(1, 1, st, P.ctr + kCtrWork, ns);
+        launch_set_i(1, 1, st, P.ctr + kCtrNext, ns);  // every slot starts in the closest-hit queue
         HIP_OK(hipGetLastError());
         HIP_OK(hipStreamSynchronize(st));
         hipGraph_t graph = nullptr;
@@ -1912,10 +725,10 @@ struct Renderer {
         HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         Paths Q = P;
         for (int k = 0; k < passes; k++) {
-            k_pass_begin<<<1, 1, 0, st>>>(Q.ctr, counts.p);
-            k_trace<<<grid_for(2LL * ns), kBlock, 0, st>>>(g.ds, Q, 0, 0, B.spill.p, counts.p);
-            k_finalize<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, Q, 0, counts.p);
-            k_shade<<<grid_for(ns), kBlock, 0, st>>>(g.ds, F, Q, 0, counts.p);
+            launch_pass_begin(1, 1, st, Q.ctr, counts.p);
+            launch_trace(grid_for(2LL * ns), kBlock, st, g.ds, Q, 0, 0, B.spill.p, counts.p);
+            launch_finalize(grid_for(ns), kBlock, st, g.ds, F, Q, 0, counts.p);
+            launch_shade(grid_for(ns), kBlock, st, g.ds, F, Q, 0, counts.p);
             std::swap(Q.q_in, Q.q_out);
         }
         HIP_OK(hipStreamEndCapture(st, &graph));
@@ -1957,8 +770,8 @@ struct Renderer {
         offsets.ensure(size_t(ntiles));
         t0.ensure(size_t(ns));
         save.ensure(size_t(F.npix));
-        k_fill_i<<<grid_for(F.npix), kBlock, 0, st>>>(cost.p, F.npix);
-        k_fill_u64<<<grid_for(ntiles), kBlock, 0, st>>>(reinterpret_cast<unsigned long long *>(tile_cost.p),
+        launch_fill_i(grid_for(F.npix), kBlock, st, cost.p, F.npix);
+        launch_fill_u64(grid_for(ntiles), kBlock, st, reinterpret_cast<unsigned long long *>(tile_cost.p),
                                                          size_t(ntiles));
         Frame F1 = F;
         F1.sample_end = first;
@@ -1966,7 +779,7 @@ struct Renderer {
         F1.unit_t0 = t0.p;
         F1.rng_save = save.p;
         run(F1, tail, nullptr, stats, true);
-        k_tile_cost<<<grid_for(F.npix), kBlock, 0, st>>>(cost.p, F.width, F.height, tile_cost.p);
+        launch_tile_cost(grid_for(F.npix), kBlock, st, cost.p, F.width, F.height, tile_cost.p);
         std::vector<long long> tc(static_cast<size_t>(ntiles));
         HIP_OK(hipMemcpyAsync(tc.data(), tile_cost.p, tc.size() * sizeof(long long), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
@@ -1981,7 +794,7 @@ struct Renderer {
         }
         HIP_OK(hipMemcpyAsync(tile_ids.p, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync(offsets.p, offs.data(), offs.size() * sizeof(int), hipMemcpyHostToDevice, st));
-        k_tile_expand<<<grid_for((long long)ntiles * 64), kBlock, 0, st>>>(tile_ids.p, offsets.p, ntiles, F.width,
+        launch_tile_expand(grid_for((long long)ntiles * 64), kBlock, st, tile_ids.p, offsets.p, ntiles, F.width,
                                                                            F.height, order.p);
         HIP_OK(hipGetLastError());
         Frame F2 = F;
@@ -2050,7 +863,7 @@ void render_with(Renderer &R, const Frame &F, const lambda_render_params &p, lon
     if (by_cost) R.run_cost_ordered(F, std::min(p.cost_order > 0 ? p.cost_order : 4, F.spp - 1), tail, progress, stats);
     else if (p.graph_passes > 0) R.run_graph(F, p.graph_passes, progress, stats);
     else R.run(F, tail, progress, stats);
-    k_resolve<<<grid_for(F.npix), kBlock, 0, R.st>>>(F, R.P.fb, R.B.rgb.p);
+    launch_resolve(grid_for(F.npix), kBlock, R.st, F, R.P.fb, R.B.rgb.p);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(out_rgb, R.B.rgb.p, size_t(F.npix) * 3 * sizeof(float), hipMemcpyDeviceToHost, R.st));
     HIP_OK(hipStreamSynchronize(R.st));
@@ -2194,7 +1007,7 @@ int lambda_hip_closest_hit(lambda_hip_scene *scene, const float *rays, int32_t n
         h.alloc(size_t(n));
         DevBuf<int2> spill;
         alloc_spill(*scene, n, spill);
-        k_query_closest<<<grid_for(n), kBlock>>>(scene->ds, r.p, n, h.p, spill.p);
+        launch_query_closest(grid_for(n), kBlock, nullptr, scene->ds, r.p, n, h.p, spill.p);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpy(hits, h.p, size_t(n) * sizeof(lambda_hit), hipMemcpyDeviceToHost));
     });
@@ -2223,7 +1036,7 @@ int lambda_hip_occluded(lambda_hip_scene *scene, const float *rays, const int32_
         out.alloc(size_t(n));
         DevBuf<int2> spill;
         alloc_spill(*scene, n, spill);
-        k_query_occluded<<<grid_for(n), kBlock>>>(scene->ds, r.p, li.p, lo.p, n, out.p, spill.p);
+        launch_query_occluded(grid_for(n), kBlock, nullptr, scene->ds, r.p, li.p, lo.p, n, out.p, spill.p);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpy(occluded, out.p, size_t(n) * sizeof(int), hipMemcpyDeviceToHost));
     });
@@ -2241,7 +1054,7 @@ int lambda_hip_thinfilm_reflectance(lambda_hip_scene *scene, int32_t material, c
         q.upload(queries, size_t(n) * 4);
         p.upload(points, size_t(n) * 3);
         o.alloc(size_t(n));
-        k_query_thinfilm<<<grid_for(n), kBlock>>>(scene->ds, material, q.p, p.p, n, o.p);
+        launch_query_thinfilm(grid_for(n), kBlock, nullptr, scene->ds, material, q.p, p.p, n, o.p);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpy(out, o.p, size_t(n) * sizeof(float), hipMemcpyDeviceToHost));
     });
@@ -2424,8 +1237,8 @@ int lambda_hip_serve(lambda_hip_scene *const *scenes, int32_t num_scenes, const 
             hipStream_t s;
             ~StreamFree() { if (s) (void)hipStreamDestroy(s); }
         } sfree{st};
-        k_fill_i<<<1, kBlock, 0, st>>>(P.ctr, kCtrCount);
-        k_fill_u64<<<1, kBlock, 0, st>>>(P.rays, 2);
+        launch_fill_i(1, kBlock, st, P.ctr, kCtrCount);
+        launch_fill_u64(1, kBlock, st, P.rays, 2);
         int batches = 0;
         double first_start_ms = -1;
         size_t next = 0;
@@ -2456,8 +1269,8 @@ int lambda_hip_serve(lambda_hip_scene *const *scenes, int32_t num_scenes, const 
             DevBuf<unsigned long long> finish;
             done.alloc(batch.size());
             finish.alloc(batch.size());
-            k_fill_d<<<grid_for(px_total * 3), kBlock, 0, st>>>(fb.p, size_t(px_total) * 3);
-            k_fill_i<<<grid_for((long long)batch.size()), kBlock, 0, st>>>(done.p, int(batch.size()));
+            launch_fill_d(grid_for(px_total * 3), kBlock, st, fb.p, size_t(px_total) * 3);
+            launch_fill_i(grid_for((long long)batch.size()), kBlock, st, done.p, int(batch.size()));
             long long off = 0;
             for (size_t k = 0; k < batch.size(); k++) {
                 const lambda_job &jb = jobs[batch[k]];
@@ -2476,14 +1289,14 @@ int lambda_hip_serve(lambda_hip_scene *const *scenes, int32_t num_scenes, const 
             if (units >= (1LL << 31) - (1LL << 28)) throw Failure{LAMBDA_EINVAL, "batch too large"};
             DevBuf<ServeJob> djobs;
             djobs.upload(sj);
-            k_serve_clock<<<1, 1, 0, st>>>(ctl.p);
-            k_serve<<<threads / kBlock, kBlock, 0, st>>>(dscenes.p, djobs.p, int(sj.size()), units, P, ctl.p, done.p,
+            launch_serve_clock(1, 1, st, ctl.p);
+            launch_serve(threads / kBlock, kBlock, st, dscenes.p, djobs.p, int(sj.size()), units, P, ctl.p, done.p,
                                                         finish.p, B.spill.p);
             HIP_OK(hipGetLastError());
             off = 0;
             for (size_t k = 0; k < batch.size(); k++) {
                 const ServeJob &J = sj[k];
-                k_resolve<<<grid_for(J.F.npix), kBlock, 0, st>>>(J.F, J.fb, rgb.p + off * 3);
+                launch_resolve(grid_for(J.F.npix), kBlock, st, J.F, J.fb, rgb.p + off * 3);
                 off += J.F.npix;
             }
             std::vector<float> host_rgb(size_t(px_total) * 3);
@@ -2544,7 +1357,7 @@ int lambda_hip_progressive_create(lambda_hip_scene *scene, const lambda_render_p
         // caller asks for one launch per bounce (tail_paths -1)
         pr->fused = guide && params->tail_paths != -1;
         long long slots = params->path_slots > 0 ? params->path_slots
-                          : pr->fused ? resident_threads(k_persist_guided)
+                          : pr->fused ? resident_threads(persist_guided_blocks_per_cu())
                           : guide     ? (1LL << 20)
                                       : resident_persist_threads();
         slots = std::max<long long>(1, std::min<long long>(slots, 1LL << 26));
@@ -2632,7 +1445,7 @@ int lambda_hip_progressive_image(lambda_hip_progressive *pr, float *out_rgb, cha
         Renderer &R = *pr->R;
         Frame F = make_frame(pr->params);
         F.spp = int(std::min<long long>(pr->total, 1LL << 30));
-        k_resolve<<<grid_for(F.npix), kBlock, 0, R.st>>>(F, R.P.fb, R.B.rgb.p);
+        launch_resolve(grid_for(F.npix), kBlock, R.st, F, R.P.fb, R.B.rgb.p);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(out_rgb, R.B.rgb.p, size_t(F.npix) * 3 * sizeof(float), hipMemcpyDeviceToHost, R.st));
         HIP_OK(hipStreamSynchronize(R.st));
