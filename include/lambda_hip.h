@@ -278,6 +278,22 @@ int lambda_hip_closest_hit(lambda_hip_scene *scene, const float *rays, int32_t n
                            lambda_hit *hits, char *err, size_t err_len);
 
 /*
+ * Closest-hit queries that allocate nothing (no reference counterpart: for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops).  rays[n][6] and hits[n] are device memory
+ * of the current device, which must hold the scene; the kernel is queued on
+ * `stream` (a hipStream_t, NULL for the null stream) and the call returns
+ * without waiting.  Results equal lambda_hip_closest_hit's.  Traversal
+ * stacks deeper than the LDS part use a fixed 16 MiB scratch area per device
+ * (allocated with the library, held until the process ends): scenes whose
+ * traversal needs more than 48 stack entries (lambda_scene_info.stack_need)
+ * are rejected with LAMBDA_EINVAL.  The scratch needs no clearing; calls
+ * that overlap on one device share it and race.
+ */
+int lambda_hip_closest_hit_device(lambda_hip_scene *scene, const float *rays, int32_t n, lambda_hit *hits,
+                                  void *stream, char *err, size_t err_len);
+
+/*
  * Shadow queries: Scene::hit in occlusion mode (scene/scene.h:61-70,99,106)
  * for rays[n][6] toward light (light_instance[i], light_object[i]); -1/-1 is
  * the environment.  occluded[i] = 1 if Scene::hit returned true.

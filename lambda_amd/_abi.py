@@ -153,6 +153,8 @@ def hip():
         lib.lambda_hip_scene_render.argtypes = [vp, rp, _f32p, _i32p, C.c_char_p, C.c_size_t]
         lib.lambda_hip_scene_render_stats.argtypes = [vp, rp, _f32p, C.POINTER(RenderStats), C.c_char_p, C.c_size_t]
         lib.lambda_hip_closest_hit.argtypes = [vp, _f32p, C.c_int32, C.POINTER(Hit), C.c_char_p, C.c_size_t]
+        lib.lambda_hip_closest_hit_device.argtypes = [vp, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                                      C.c_char_p, C.c_size_t]
         lib.lambda_hip_occluded.argtypes = [vp, _f32p, _i32p, _i32p, C.c_int32, _i32p, C.c_char_p, C.c_size_t]
         lib.lambda_hip_thinfilm_reflectance.argtypes = [vp, C.c_int32, _f32p, _f32p, C.c_int32, _f32p,
                                                         C.c_char_p, C.c_size_t]

@@ -139,6 +139,13 @@ constexpr int kPersistWaves = 3;
 // on).  Slots default to the number of persistent threads resident at once.
 constexpr long long kAutoTailPaths = 1LL << 40;
 using Stack = TraversalStack<kRing>;
+// No-allocation closest-hit queries: a fixed grid of this many threads, and
+// scratch for the traversal-stack entries beyond the LDS ring of scenes
+// whose stacks are at most kNoAllocMaxStack entries deep.
+constexpr int kNoAllocThreads = 65536;
+constexpr int kNoAllocMaxStack = 48;
+constexpr int kNoAllocSpill = kNoAllocMaxStack - kRing;
+constexpr int kNoAllocMaxDevices = 64;
 
 __device__ __forceinline__ Stack make_stack(int *s_code, float *s_dist, int2 *spill) {
     Stack s;

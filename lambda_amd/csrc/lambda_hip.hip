@@ -163,6 +163,7 @@ void validate(const lambda_scene_view &v) {
 struct lambda_hip_scene {
     DevBuf<float4> wide, obj, obj_s, inst, stex;
     DevBuf<int> inst_root, obj_mat, props, perm, tlas_rank, obj_rank;
+    int device = 0;         // HIP device holding the scene
     int stack_need = 0;     // deepest traversal stack of the scene
     float box[6] = {0, 0, 0, 0, 0, 0};  // scene bounds (instance-level root)
     DevBuf<float4> worley_points;
@@ -189,6 +190,7 @@ namespace {
 
 void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g) {
     validate(v);
+    HIP_OK(hipGetDevice(&g.device));
     const size_t O = size_t(v.num_objects), I = size_t(v.num_instances);
     WideBuilder wb(v);
     g.stack_need = wb.run();
@@ -1010,6 +1012,32 @@ int lambda_hip_closest_hit(lambda_hip_scene *scene, const float *rays, int32_t n
         launch_query_closest(grid_for(n), kBlock, nullptr, scene->ds, r.p, n, h.p, spill.p);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpy(hits, h.p, size_t(n) * sizeof(lambda_hit), hipMemcpyDeviceToHost));
+    });
+}
+
+int lambda_hip_closest_hit_device(lambda_hip_scene *scene, const float *rays, int32_t n, lambda_hit *hits,
+                                  void *stream, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scene || n < 0 || (n > 0 && (!rays || !hits))) throw Failure{LAMBDA_EINVAL, "invalid arguments"};
+        int dev = 0;
+        HIP_OK(hipGetDevice(&dev));
+        if (dev != scene->device) throw Failure{LAMBDA_EINVAL, "the scene is on another device than the current one"};
+        if (scene->stack_need > kNoAllocMaxStack)
+            throw Failure{LAMBDA_EINVAL, "scene traversal needs a " + std::to_string(scene->stack_need) +
+                                             "-entry stack; the no-allocation query holds at most " +
+                                             std::to_string(kNoAllocMaxStack)};
+        if (n == 0) return;
+        for (const void *p : {static_cast<const void *>(rays), static_cast<const void *>(hits)}) {
+            hipPointerAttribute_t a;
+            if (hipPointerGetAttributes(&a, p) != hipSuccess || a.type != hipMemoryTypeDevice || a.device != dev) {
+                (void)hipGetLastError();
+                throw Failure{LAMBDA_EINVAL, "rays and hits must be device memory of the current device"};
+            }
+        }
+        int2 *spill = noalloc_scratch();
+        launch_query_closest_fixed(kNoAllocThreads / kBlock, kBlock, static_cast<hipStream_t>(stream), scene->ds,
+                                   rays, n, hits, spill);
+        HIP_OK(hipGetLastError());
     });
 }
 

@@ -29,6 +29,10 @@ void launch_persist_guided(dim3 grid, dim3 block, hipStream_t st, DeviceScene S,
 int persist_guided_blocks_per_cu();
 void launch_serve(dim3 grid, dim3 block, hipStream_t st, const DeviceScene *scenes, const ServeJob *jobs, int njobs, long long total_units, Paths P, ServeCtl *ctl, int *done, unsigned long long *finish, int2 *spill);
 void launch_query_closest(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, const float *rays, int n, lambda_hit *out, int2 *spill);
+// Device address of the no-allocation query scratch on the current device
+// (looked up once per device).
+int2 *noalloc_scratch();
+void launch_query_closest_fixed(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, const float *rays, int n, lambda_hit *out, int2 *spill);
 void launch_query_occluded(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, const float *rays, const int *li, const int *lo, int n, int *out, int2 *spill);
 void launch_query_thinfilm(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, int mat, const float *q, const float *pts, int n, float *out);
 
