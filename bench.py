@@ -91,10 +91,23 @@ def cpu_baseline(scene, seconds: float, threads: int) -> dict:
     oracle.render(scene, q, threads=threads, rows=(r0, r0 + rows))
     t = time.perf_counter() - t0
     paths = rows * p.width * spp
-    return {"value": paths / t, "unit": "paths/s", "cores": threads, "kind": "port",
+    return {"value": paths / t, "unit": "paths/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+            "host_cpus": os.cpu_count(),
             "sample": "rows %d-%d of %d (%d pixels) at %d spp, %.1f s; CPU oracle, bit-identical restatement of "
                       "the reference CPU integrator, OpenMP guided schedule over rows" % (
                           r0, r0 + rows, p.height, rows * p.width, spp, t)}
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, warmup: int = 0,

@@ -15,8 +15,12 @@ stores only inputs and outputs:
   ppm_cases.npy, ppm_cases.ppm a float image of edge values (negative, NaN,
                                infinities, the sRGB knee, above 1) and the
                                reference Writer's PPM of it
+  instanced_mesh_1080p_arrays.json  (--full-scale) SHA-256 of every flat
+                               array the reference builds for the 1080p
+                               instanced-mesh scene (1.04M BVH nodes), with
+                               the input files' hashes and its build time
 
-Usage: python tests/golden/make_golden.py [--skip-4096] [--ppm-only]
+Usage: python tests/golden/make_golden.py [--skip-4096] [--ppm-only] [--full-scale]
 """
 from __future__ import annotations
 
@@ -31,6 +35,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import ref_dump  # noqa: E402
 from lambda_amd import scenes  # noqa: E402
@@ -140,9 +145,54 @@ def make_ppm(tmp: str) -> None:
         fh.write(ref_dump.ppm(cases, tmp))
 
 
+def array_hashes(arrs: dict) -> dict:
+    import hashlib
+    from test_host_parity import FIELDS
+    return {f: {"shape": list(np.asarray(arrs[f]).shape), "dtype": str(np.asarray(arrs[f]).dtype),
+                "sha256": hashlib.sha256(np.ascontiguousarray(arrs[f]).tobytes()).hexdigest()} for f in FIELDS}
+
+
+def make_full_scale(tmp: str) -> None:
+    """The reference's arrays of the 1080p instanced-mesh scene, as hashes
+    (the reference's own build takes about ten minutes)."""
+    import hashlib
+    import json
+    import time
+    lrd = scenes.write_instanced_mesh(tmp)
+    cwd = os.getcwd()
+    os.chdir(tmp)
+    try:
+        t0 = time.perf_counter()
+        arrs = ref_dump.dump_scene(os.path.basename(lrd), os.path.join(tmp, "scene.bin"))
+        dt = time.perf_counter() - t0
+    finally:
+        os.chdir(cwd)
+    inputs = {}
+    for fn in sorted(os.listdir(tmp)):
+        if fn.endswith((".lrd", ".obj")):
+            with open(os.path.join(tmp, fn), "rb") as fh:
+                data = fh.read()
+            if fn.endswith(".lrd"):  # the mesh path names the directory
+                data = data.replace((tmp.rstrip("/") + "/").encode(), b"")
+            inputs[fn] = hashlib.sha256(data).hexdigest()
+    out = {"scene": "instanced mesh 1920x1080: lambda_amd.scenes.write_instanced_mesh defaults (level-8 icosphere, "
+                    "64 instances)", "inputs_sha256": inputs, "reference_parse_and_bvh_s": round(dt, 1),
+           "arrays": array_hashes(arrs), "total_light_power": float(arrs["total_light_power"]),
+           "inputs_note": "the .lrd is hashed with its directory prefix (where the .obj was written) removed"}
+    with open(os.path.join(HERE, "instanced_mesh_1080p_arrays.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
 def main(argv) -> None:
     if not ref_dump.available():
         raise SystemExit("reference harness missing: run `make -C oracle ref` first")
+    if "--full-scale" in argv:
+        tmp = tempfile.mkdtemp()
+        try:
+            make_full_scale(tmp)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        return
     if "--ppm-only" in argv:
         tmp = tempfile.mkdtemp()
         try:

@@ -227,3 +227,40 @@ def test_cli_argument_and_file_errors(tmp_path):
     assert run("none.lrd") == (1, "none.lrd: failed to open file (no such file or directory)")
     (tmp_path / "bad.lrd").write_text("Camera (0 0 0) (0 0 0) (1 0 0) (0 1 0)\n")
     assert run("bad.lrd") == (1, 'bad.lrd:1: expected "Render", got "Camera"')
+
+
+@pytest.mark.slow
+def test_full_scale_arrays_match_reference_hashes(tmp_path):
+    """The 1080p instanced-mesh scene (1,310,724 objects, 64 instances of a
+    level-8 icosphere, 1,043,859 BVH nodes): every flat array the host front
+    end builds -- the SAH node array with its std::partition order
+    (scene/bvh.h:61-246), the reordered objects and instances, the light
+    list -- hashes to the bytes the reference's own build produced
+    (tests/golden/instanced_mesh_1080p_arrays.json, written by
+    make_golden.py --full-scale from the reference harness)."""
+    import hashlib
+    import json
+    import time
+    from lambda_amd import scenes as gen
+    want = json.load(open(os.path.join(GOLDEN, "instanced_mesh_1080p_arrays.json")))
+    lrd = gen.write_instanced_mesh(str(tmp_path))
+    for fn, h in want["inputs_sha256"].items():
+        data = (tmp_path / fn).read_bytes()
+        if fn.endswith(".lrd"):
+            data = data.replace((str(tmp_path) + "/").encode(), b"")
+        assert hashlib.sha256(data).hexdigest() == h, fn
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        t0 = time.perf_counter()
+        s = lambda_amd.Scene.load(os.path.basename(lrd))
+        dt = time.perf_counter() - t0
+    finally:
+        os.chdir(cwd)
+    a = s.arrays()
+    print("host parse + BVH %.1f s (reference %.0f s)" % (dt, want["reference_parse_and_bvh_s"]), flush=True)
+    for f, rec in want["arrays"].items():
+        x = np.ascontiguousarray(a[f])
+        assert list(x.shape) == rec["shape"] and str(x.dtype) == rec["dtype"], f
+        assert hashlib.sha256(x.tobytes()).hexdigest() == rec["sha256"], "%s differs from the reference" % f
+    assert np.float32(a["total_light_power"]) == np.float32(want["total_light_power"])
