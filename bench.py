@@ -212,8 +212,10 @@ def guided_leg(args, scene_dir: str) -> dict:
 def serve_leg(args, mesh_dev, scene_dir: str) -> dict:
     """32 seeded jobs, random variants of the closed box, the thin-film grid
     and the instanced mesh (side 128-1024 pixels, depth 4-16, 16-256 spp in
-    perfect squares, 380-780 or 360-830 nm), Poisson arrivals with a mean
-    gap of 50 ms, served in persistent launches admitted by HBM budget."""
+    perfect squares, 380-780 or 360-830 nm), submitted to the live server
+    by a host thread with Poisson arrivals (mean gap 50 ms) and admitted by
+    HBM budget into overlapping persistent launches."""
+    import threading
     import numpy as np
     import lambda_amd
     from lambda_amd import scenes
@@ -229,15 +231,38 @@ def serve_leg(args, mesh_dev, scene_dir: str) -> dict:
         lo, hi = (380.0, 780.0) if rng.random() < 0.5 else (360.0, 830.0)
         jobs.append(dict(scene=int(rng.integers(0, 3)), width=side, height=side, depth=int(rng.integers(4, 17)),
                          samples=root * root, sqrt_samples=root, lambda_min=lo, lambda_max=hi, arrival_ms=t))
-    _, results, st = lambda_amd.serve(devs, jobs, hbm_budget=args.serve_budget)
+    srv = lambda_amd.Server(devs, hbm_budget=args.serve_budget, max_launches=args.serve_launches)
+    tickets = []
+
+    def feed():
+        t0 = time.perf_counter()
+        for jd in jobs:
+            jd = dict(jd)
+            due = jd.pop("arrival_ms") / 1e3
+            delay = due - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(jd.pop("scene"), **jd)[0])
+
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    st = srv.drain()
+    results = [srv.poll(tk)[1] for tk in tickets]
+    srv.close()
     for d in devs[:2]:
         d.close()
     lat = np.array([r["latency_ms"] for r in results])
+    batch_px = {}
+    for jd, r in zip(jobs, results):
+        batch_px[r["batch"]] = batch_px.get(r["batch"], 0) + jd["width"] * jd["height"]
     return {"jobs": len(jobs), "batches": st["batches"], "paths": st["paths"], "wall_ms": st["wall_ms"],
             "paths_per_s": st["paths"] / (st["wall_ms"] / 1e3),
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                            "max": float(lat.max())},
             "last_arrival_ms": jobs[-1]["arrival_ms"], "hbm_budget": args.serve_budget,
+            "max_launches": args.serve_launches, "largest_batch_bytes": 36 * max(batch_px.values()),
+            "mode": "live server: host-timed Poisson submissions, HBM-budget admission, overlapping launches",
             "seed": args.serve_seed}
 
 
@@ -302,7 +327,10 @@ def main() -> int:
                     help="reference streams: time the first N samples of every pixel, then run the slowest tiles "
                          "first (0 automatic: N = 4; -1 off)")
     ap.add_argument("--graph-passes", type=int, default=0, help="capture the bounce loop in a hipGraph of this many passes")
-    ap.add_argument("--autotune", action="store_true", help="pick the schedule by timing short renders first")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the library's default schedule instead of timing short renders first")
+    ap.add_argument("--tune-cache", default=os.environ.get("LAMBDA_AUTOTUNE_CACHE", ""),
+                    help="autotuner cache file (default: the tuner's own)")
     ap.add_argument("--tail-paths", type=int, default=0,
                     help="live paths below which the render finishes in one persistent launch (0 automatic, -1 never)")
     ap.add_argument("--scene-dir", default=os.environ.get("LAMBDA_SCENE_DIR", "/tmp/lambda_bench"))
@@ -312,7 +340,9 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the batched-serving leg")
     ap.add_argument("--serve-jobs", type=int, default=32)
     ap.add_argument("--serve-gap-ms", type=float, default=50.0)
-    ap.add_argument("--serve-budget", type=int, default=0, help="HBM bytes admitted per launch (0 = no limit)")
+    ap.add_argument("--serve-budget", type=int, default=128 << 20,
+                    help="HBM bytes of the jobs in flight (0 = no limit)")
+    ap.add_argument("--serve-launches", type=int, default=2, help="persistent launches in flight")
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--guided-res", default="320x180")
     ap.add_argument("--guided-ref-spp", type=int, default=65536)
@@ -370,37 +400,35 @@ def main() -> int:
     mode = lambda_amd.LAMBDA_RNG_REFERENCE if args.rng == "reference" else lambda_amd.LAMBDA_RNG_THROUGHPUT
     p = scene.params
     tuned = None
-    if args.autotune:
+    sched = {"path_slots": args.path_slots, "tail_paths": args.tail_paths, "cost_order": args.cost_order,
+             "graph_passes": args.graph_passes}
+    if not args.no_autotune:
         from lambda_amd import autotune
         t0 = time.perf_counter()
-        tuned = autotune.tune(dev, cache=None, rng_mode=mode)
-        log("autotune (%.1f s): %s" % (time.perf_counter() - t0, tuned["best"]))
-        best = tuned["best"]
-        args.path_slots = best.get("path_slots", 0)
-        args.tail_paths = best.get("tail_paths", 0)
-        args.cost_order = best.get("cost_order", 0)
-        args.graph_passes = best.get("graph_passes", 0)
+        kw = {"cache": args.tune_cache} if args.tune_cache else {}
+        tuned = autotune.tune(dev, trial_spp=16, rng_mode=mode, **kw)
+        tuned["seconds"] = time.perf_counter() - t0
+        log("autotune (%.1f s): %s" % (tuned["seconds"], tuned["best"]))
+        sched = dict(tuned["best"])
     paths_per_step = p.width * p.height * p.samples
 
     # the output buffer is allocated once, as main.cpp allocates h_output
     img = np.zeros((p.height, p.width, 3), np.float32)
     for i in range(args.warmup):
         t0 = time.perf_counter()
-        dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order,
-                   graph_passes=args.graph_passes, out=img)
+        dev.render(rng_mode=mode, out=img, **sched)
         log("warmup %d: %.3f s" % (i, time.perf_counter() - t0))
 
     t0 = time.perf_counter()
     for i in range(args.steps):
-        dev.render(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order,
-                   graph_passes=args.graph_passes, out=img)
+        dev.render(rng_mode=mode, out=img, **sched)
     elapsed = time.perf_counter() - t0
     value = paths_per_step * args.steps / elapsed
     log("timed %d steps: %.3f s -> %.4g paths/s" % (args.steps, elapsed, value))
     finite = bool(np.all(np.isfinite(img)))
 
     # Kernel timing from HIP events on the render stream.
-    _, st = dev.render_stats(rng_mode=mode, path_slots=args.path_slots, tail_paths=args.tail_paths, cost_order=args.cost_order, graph_passes=args.graph_passes)
+    _, st = dev.render_stats(rng_mode=mode, **sched)
     log("kernel ms: " + ", ".join("%s %.1f" % (k, st[k + "_ms"]) for k in ("trace", "shade", "finalize", "persist")))
 
     # The same frame with independent streams per (pixel, sample): same
@@ -466,9 +494,9 @@ def main() -> int:
 
     if tuned:
         cfg["autotune"] = {"best": tuned["best"], "trial_spp": tuned["trial_spp"],
-                           "table_ms": [round(ms, 2) for _, ms in tuned["table"]]}
-    cfg.update({"rng": args.rng, "path_slots": st["path_slots"], "schedule": {
-        "tail_paths": args.tail_paths, "cost_order": args.cost_order, "graph_passes": args.graph_passes}, "setup_s": {
+                           "seconds": round(tuned.get("seconds", 0.0), 2),
+                           "table_ms": [[c, round(ms, 2)] for c, ms in tuned["table"]]}
+    cfg.update({"rng": args.rng, "path_slots": st["path_slots"], "schedule": sched, "setup_s": {
         "generate": round(t_gen, 2), "parse_and_bvh": round(t_parse, 2), "upload": round(t_upload, 2)}})
     out = {
         "metric": METRIC,

@@ -195,6 +195,10 @@ typedef struct lambda_render_params {
                               loop captured in a hipGraph of this many passes
                               (rounded up to even) and replayed until done;
                               0 = no graph */
+    int32_t tile;          /* side of the square pixel tiles of the visiting
+                              order (4, 8 or 16; 0 = 8) */
+    int32_t occupancy;     /* persistent kernel variant: waves per SIMD its
+                              registers are capped for (2, 3 or 4; 0 = 3) */
 } lambda_render_params;
 
 /*
@@ -414,6 +418,41 @@ typedef struct lambda_serve_stats {
 int lambda_hip_serve(lambda_hip_scene *const *scenes, int32_t num_scenes, const lambda_job *jobs, int32_t num_jobs,
                      int64_t hbm_budget, lambda_job_result *results, lambda_serve_stats *stats, char *err,
                      size_t err_len);
+
+/*
+ * Live serving (no reference counterpart): jobs are submitted while earlier
+ * ones render, as a serving front end receives them.  A scheduler thread
+ * admits queued jobs in submission order while the buffers of the jobs in
+ * flight (36 bytes per pixel) fit hbm_budget (0 = no limit; a job larger
+ * than the budget runs alone), and renders each admitted group as one
+ * persistent launch on one of max_launches streams (0 = 2), so that a new
+ * group starts on the compute units a draining launch frees.  Within a
+ * launch the jobs with the longest per-pixel sample chains (samples x
+ * (depth + 1)) start first.  Every job's image is bit-identical to
+ * lambda_hip_scene_render of the same settings with reference streams.
+ * Latency = the device time the job's last pixel finished - submission.
+ */
+typedef struct lambda_server lambda_server;
+
+enum lambda_job_state { LAMBDA_JOB_QUEUED = 0, LAMBDA_JOB_RUNNING = 1, LAMBDA_JOB_DONE = 2, LAMBDA_JOB_FAILED = 3 };
+
+/* The scenes stay owned by the caller and must outlive the server. */
+int lambda_hip_server_create(lambda_hip_scene *const *scenes, int32_t num_scenes, int64_t hbm_budget,
+                             int32_t max_launches, lambda_server **out, char *err, size_t err_len);
+/* Queues a job (arrival_ms is ignored: it arrives now) and returns its
+ * ticket; job->out_rgb must stay valid until the job is done. */
+int lambda_hip_server_submit(lambda_server *server, const lambda_job *job, int64_t *ticket, char *err,
+                             size_t err_len);
+/* Non-blocking: *state = LAMBDA_JOB_*; with LAMBDA_JOB_DONE the image is in
+ * out_rgb and *result (may be NULL) holds latency, finish time (ms after
+ * the server's creation) and launch index.  A failed job returns its error. */
+int lambda_hip_server_poll(lambda_server *server, int64_t ticket, int32_t *state, lambda_job_result *result,
+                           char *err, size_t err_len);
+/* Waits until every submitted job has finished; *stats (may be NULL): launches,
+ * paths, rays, and the time from the first submission to the last finish. */
+int lambda_hip_server_drain(lambda_server *server, lambda_serve_stats *stats, char *err, size_t err_len);
+/* Finishes the queued jobs, then releases the server. */
+void lambda_hip_server_destroy(lambda_server *server);
 
 /*
  * Progressive rendering (no reference counterpart): a session accumulates

@@ -429,3 +429,66 @@ def serve(devices: "list[DeviceScene]", jobs: "list[dict]", hbm_budget: int = 0)
     stats = {"batches": st.batches, "paths": st.paths, "closest_rays": st.closest_rays,
              "shadow_rays": st.shadow_rays, "wall_ms": st.wall_ms}
     return images, results, stats
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
+
+
+class Server:
+    """Live serving (lambda_hip_server_*): submit jobs while earlier ones
+    render; admission by HBM budget, overlapping persistent launches."""
+
+    def __init__(self, devices: "list[DeviceScene]", hbm_budget: int = 0, max_launches: int = 0):
+        self._lib = _abi.hip()
+        self.devices = devices
+        self._handles = (C.c_void_p * len(devices))(*[d._h for d in devices])
+        self._h = C.c_void_p()
+        self._keep = {}
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_server_create(self._handles, len(devices), hbm_budget, max_launches,
+                                                  C.byref(self._h), err, 4096), err)
+
+    def submit(self, scene: int, **params) -> "tuple[int, np.ndarray]":
+        """Queues a job on devices[scene] with render-parameter overrides;
+        returns (ticket, image buffer filled when the job is done)."""
+        # settings from the job's scene (the library rejects an index out of range)
+        p = self.devices[scene if 0 <= scene < len(self.devices) else 0].scene.render_params(**params)
+        img = np.zeros((p.height, p.width, 3), np.float32)
+        job = _abi.Job()
+        job.scene = scene
+        job.params = p
+        job.arrival_ms = 0.0
+        job.out_rgb = _abi.ptr(img, C.c_float)
+        t = C.c_int64()
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_server_submit(self._h, C.byref(job), C.byref(t), err, 4096), err)
+        self._keep[t.value] = img
+        return t.value, img
+
+    def poll(self, ticket: int) -> "tuple[int, dict | None]":
+        st = C.c_int32()
+        res = _abi.JobResult()
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_server_poll(self._h, ticket, C.byref(st), C.byref(res), err, 4096), err)
+        if st.value != JOB_DONE:
+            return st.value, None
+        return st.value, {"latency_ms": res.latency_ms, "finish_ms": res.finish_ms, "batch": res.batch}
+
+    def drain(self) -> dict:
+        st = _abi.ServeStats()
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_server_drain(self._h, C.byref(st), err, 4096), err)
+        return {"batches": st.batches, "paths": st.paths, "closest_rays": st.closest_rays,
+                "shadow_rays": st.shadow_rays, "wall_ms": st.wall_ms}
+
+    def close(self):
+        if self._h:
+            self._lib.lambda_hip_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

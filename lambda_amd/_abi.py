@@ -48,6 +48,7 @@ class RenderParams(C.Structure):
         ("lambda_min", C.c_float), ("lambda_max", C.c_float),
         ("seed", C.c_uint64), ("rng_mode", C.c_int32), ("path_slots", C.c_int32),
         ("tail_paths", C.c_int32), ("cost_order", C.c_int32), ("graph_passes", C.c_int32),
+        ("tile", C.c_int32), ("occupancy", C.c_int32),
     ]
 
 
@@ -176,6 +177,13 @@ def hip():
         lib.lambda_hip_guide_benchmark.argtypes = [vp, C.c_int32, C.c_int32, f64p, f64p, C.c_char_p, C.c_size_t]
         lib.lambda_hip_serve.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Job), C.c_int32, C.c_int64,
                                          C.POINTER(JobResult), C.POINTER(ServeStats), C.c_char_p, C.c_size_t]
+        lib.lambda_hip_server_create.argtypes = [C.POINTER(vp), C.c_int32, C.c_int64, C.c_int32, C.POINTER(vp),
+                                                 C.c_char_p, C.c_size_t]
+        lib.lambda_hip_server_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_int64), C.c_char_p, C.c_size_t]
+        lib.lambda_hip_server_poll.argtypes = [vp, C.c_int64, _i32p, C.POINTER(JobResult), C.c_char_p, C.c_size_t]
+        lib.lambda_hip_server_drain.argtypes = [vp, C.POINTER(ServeStats), C.c_char_p, C.c_size_t]
+        lib.lambda_hip_server_destroy.argtypes = [vp]
+        lib.lambda_hip_server_destroy.restype = None
         lib.lambda_hip_progressive_create.argtypes = [vp, rp, vp, C.POINTER(vp), C.c_char_p, C.c_size_t]
         lib.lambda_hip_progressive_step.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(ProgressiveStats),
                                                     C.c_char_p, C.c_size_t]

@@ -1,13 +1,15 @@
 """Schedule autotuner: picks, for a scene and a frame, the schedule of the
 integrator -- persistent launch, one launch per bounce with a persistent
 tail, or the bounce loop captured in a hipGraph -- with its path-slot
-count (queue size) and the tile cost order, by timing short renders of the
-same frame at reduced samples per pixel.  The choice is cached in a JSON
-file keyed by the scene's shape and the frame's settings.
+count (queue size), the tile size and cost order of the pixel visiting
+order, and the compiled occupancy variant of the persistent kernel (2, 3 or
+4 waves per SIMD: its registers capped for that many), by timing short
+renders of the same frame at reduced samples per pixel.  The choice is
+cached in a JSON file keyed by the scene's shape, the frame's settings and
+the device.
 
-Block size and the LDS stack ring are compile-time choices of the library
-(256 threads, 16 entries per lane); the tuner covers what a render can
-change at run time.
+The block size (256 threads) and the LDS stack ring (16 entries per lane)
+are fixed at compile time; the occupancy variants are the compiled axis.
 """
 from __future__ import annotations
 
@@ -26,24 +28,38 @@ CACHE = os.environ.get("LAMBDA_AUTOTUNE_CACHE", os.path.join(os.path.expanduser(
 
 def candidates(npix: int, rng_mode: int) -> list[dict]:
     """The search space: one dict of render-parameter overrides each."""
+    base = {"tail_paths": 0, "path_slots": 0, "cost_order": 4, "tile": 8, "occupancy": 3}
     c = [
-        {"tail_paths": 0, "path_slots": 0, "cost_order": 4},        # persistent, resident slots, slowest tiles first
-        {"tail_paths": 0, "path_slots": 0, "cost_order": 8},        # ... timed over the first 8 samples
-        {"tail_paths": 0, "path_slots": 0, "cost_order": -1},       # persistent, tile order
-        {"tail_paths": 65536, "path_slots": npix, "cost_order": -1},  # passes, persistent tail
-        {"tail_paths": -1, "path_slots": npix, "cost_order": -1, "graph_passes": 8},  # graph-captured passes
+        base,                                       # persistent, resident slots, slowest 8x8 tiles first
+        dict(base, cost_order=8),                   # ... timed over the first 8 samples
+        dict(base, cost_order=-1),                  # persistent, tile order
+        dict(base, tile=4),                         # 4x4 tiles
+        dict(base, tile=16),                        # 16x16 tiles
+        dict(base, occupancy=2),                    # kernel variant for 2 waves per SIMD
+        dict(base, occupancy=4),                    # ... for 4
+        {"tail_paths": 65536, "path_slots": npix, "cost_order": -1, "tile": 8, "occupancy": 3},  # passes + tail
+        {"tail_paths": -1, "path_slots": npix, "cost_order": -1, "graph_passes": 8, "tile": 8, "occupancy": 3},
     ]
     if rng_mode != LAMBDA_RNG_REFERENCE:
-        c = [dict(x, cost_order=0) for x in c]
-        c.append({"tail_paths": 0, "path_slots": 1 << 20, "cost_order": 0})
+        c = [dict(x, cost_order=0) for x in c if x["cost_order"] in (4, -1)]
+        c.append(dict(base, cost_order=0, path_slots=1 << 20))
     return c
+
+
+def _device_name() -> str:
+    try:
+        from . import device_info
+        return device_info()[0]
+    except Exception:
+        return "no device"
 
 
 def key(dev: DeviceScene, **params) -> str:
     v = dev.scene.view
     p = dev.scene.render_params(**params)
     shape = (v.num_nodes, v.num_objects, v.num_instances, v.num_materials, p.width, p.height, p.depth,
-             p.samples, p.rng_mode, tuple(np.round(dev.scene.arrays()["node_bounds"][0], 3).tolist()))
+             p.samples, p.rng_mode, tuple(np.round(dev.scene.arrays()["node_bounds"][0], 3).tolist()),
+             _device_name())
     return hashlib.sha1(repr(shape).encode()).hexdigest()[:16]
 
 

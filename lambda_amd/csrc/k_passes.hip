@@ -132,25 +132,27 @@ __global__ void k_fill_i(int *p, int n) {
 // summed on the device, sorted on the host (longest first) and expanded back
 // to pixels, row order within a tile.  The order changes when a pixel runs,
 // never what it computes.
-__global__ void k_tile_cost(const int *cost, int w, int h, long long *tile_cost) {
+__global__ void k_tile_cost(const int *cost, int w, int h, int tile, long long *tile_cost) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= w * h) return;
-    const int tx = (p % w) >> 3, ty = (p / w) >> 3;
-    atomicAdd(reinterpret_cast<unsigned long long *>(tile_cost + size_t(ty) * ((w + 7) >> 3) + tx),
+    const int tx = (p % w) / tile, ty = (p / w) / tile;
+    atomicAdd(reinterpret_cast<unsigned long long *>(tile_cost + size_t(ty) * ((w + tile - 1) / tile) + tx),
               (unsigned long long)cost[p]);
 }
 
 // Thread (i, k): pixel k of the tile at position i of the sorted list.
-__global__ void k_tile_expand(const int *tiles, const int *offsets, int ntiles, int w, int h, int *order) {
+__global__ void k_tile_expand(const int *tiles, const int *offsets, int ntiles, int w, int h, int tile, int *order) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = int(e >> 6), k = int(e & 63);
+    const int area = tile * tile;
+    const int i = int(e / area), k = int(e % area);
     if (i >= ntiles) return;
-    const int t = tiles[i], tiles_x = (w + 7) >> 3;
+    const int t = tiles[i], tiles_x = (w + tile - 1) / tile;
     const int tx = t % tiles_x, ty = t / tiles_x;
-    const int tw = min(8, w - 8 * tx), th = min(8, h - 8 * ty);
+    const int tw = min(tile, w - tile * tx), th = min(tile, h - tile * ty);
     if (k >= tw * th) return;
-    order[offsets[i] + k] = (8 * ty + k / tw) * w + 8 * tx + k % tw;
+    order[offsets[i] + k] = (tile * ty + k / tw) * w + tile * tx + k % tw;
 }
+
 
 }  // namespace
 
@@ -194,12 +196,12 @@ void launch_fill_i(dim3 grid, dim3 block, hipStream_t st, int *p, int n) {
     k_fill_i<<<grid, block, 0, st>>>(p, n);
 }
 
-void launch_tile_cost(dim3 grid, dim3 block, hipStream_t st, const int *cost, int w, int h, long long *tile_cost) {
-    k_tile_cost<<<grid, block, 0, st>>>(cost, w, h, tile_cost);
+void launch_tile_cost(dim3 grid, dim3 block, hipStream_t st, const int *cost, int w, int h, int tile, long long *tile_cost) {
+    k_tile_cost<<<grid, block, 0, st>>>(cost, w, h, tile, tile_cost);
 }
 
-void launch_tile_expand(dim3 grid, dim3 block, hipStream_t st, const int *tiles, const int *offsets, int ntiles, int w, int h, int *order) {
-    k_tile_expand<<<grid, block, 0, st>>>(tiles, offsets, ntiles, w, h, order);
+void launch_tile_expand(dim3 grid, dim3 block, hipStream_t st, const int *tiles, const int *offsets, int ntiles, int w, int h, int tile, int *order) {
+    k_tile_expand<<<grid, block, 0, st>>>(tiles, offsets, ntiles, w, h, tile, order);
 }
 
 }  // namespace lam

@@ -20,6 +20,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -28,6 +30,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "lambda_hip.h"
@@ -338,15 +341,15 @@ void check_device() {
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Failure{LAMBDA_ENODEV, "no HIP device visible"};
 }
 
-// Pixels in 8x8 tiles, tiles row-major: neighbouring lanes trace
+// Pixels in tile x tile squares, tiles row-major: neighbouring lanes trace
 // neighbouring pixels.
-std::vector<int> tile_order(int w, int h) {
+std::vector<int> tile_order(int w, int h, int tile) {
     std::vector<int> order;
     order.reserve(size_t(w) * h);
-    for (int ty = 0; ty < h; ty += 8)
-        for (int tx = 0; tx < w; tx += 8)
-            for (int y = ty; y < std::min(ty + 8, h); y++)
-                for (int x = tx; x < std::min(tx + 8, w); x++) order.push_back(y * w + x);
+    for (int ty = 0; ty < h; ty += tile)
+        for (int tx = 0; tx < w; tx += tile)
+            for (int y = ty; y < std::min(ty + tile, h); y++)
+                for (int x = tx; x < std::min(tx + tile, w); x++) order.push_back(y * w + x);
     return order;
 }
 
@@ -370,6 +373,10 @@ void validate_params(const lambda_render_params &p) {
         throw Failure{LAMBDA_EINVAL, "invalid wavelength range"};
     if (p.rng_mode != LAMBDA_RNG_REFERENCE && p.rng_mode != LAMBDA_RNG_THROUGHPUT)
         throw Failure{LAMBDA_EINVAL, "invalid rng_mode"};
+    if (p.tile != 0 && p.tile != 4 && p.tile != 8 && p.tile != 16)
+        throw Failure{LAMBDA_EINVAL, "invalid tile (4, 8 or 16; 0 = 8)"};
+    if (p.occupancy != 0 && (p.occupancy < 2 || p.occupancy > 4))
+        throw Failure{LAMBDA_EINVAL, "invalid occupancy (2, 3 or 4 waves per SIMD; 0 = 3)"};
     if ((long long)p.width * p.height > (1LL << 28)) throw Failure{LAMBDA_EINVAL, "image too large"};
     // work-unit counters are 32-bit and may run past the last unit by the
     // number of slots (at most 2^28)
@@ -406,7 +413,7 @@ long long resident_threads(int blocks) {
     return (long long)std::max(1, cus) * std::max(1, blocks) * kBlock;
 }
 
-long long resident_persist_threads() { return resident_threads(persist_blocks_per_cu()); }
+long long resident_persist_threads(int occupancy = 0) { return resident_threads(persist_blocks_per_cu(occupancy)); }
 
 Frame make_frame(const lambda_render_params &p) {
     Frame F{};  // every optional pointer (cost, train) starts null
@@ -438,6 +445,8 @@ struct Renderer {
     } C;                                  // two-launch cost order (allocated on first use)
     Paths P{};
     int ns = 0;
+    int tile = 8;        // side of the tiles of the visiting order
+    int occupancy = 0;   // persistent kernel variant (waves per SIMD, 0 = default)
     int *host_ctr = nullptr;
     hipStream_t st = nullptr;
     EventPair ev[4];
@@ -451,7 +460,7 @@ struct Renderer {
         B.rays.alloc(2);
         B.fb.alloc(size_t(F.npix) * 3);
         B.rgb.alloc(size_t(F.npix) * 3);
-        B.order.upload(tile_order(F.width, F.height));
+        B.order.upload(tile_order(F.width, F.height, tile));
         alloc_spill(g, 2LL * ns, B.spill);
         P.o = B.o.p; P.d = B.d.p; P.lam = B.lam.p; P.beta = B.beta.p; P.L = B.L.p;
         P.rng = B.rng.p; P.info = B.info.p; P.hit = B.hit.p;
@@ -552,7 +561,7 @@ struct Renderer {
                         g.ds, F, P, n_closest, B.spill.p, guide->dev());
                 else
                     launch_persist(grid_for((long long)n_closest + n_done), kBlock, st, g.ds, F, P, n_closest,
-                                                                                          B.spill.p);
+                                                                                          B.spill.p, occupancy);
                 if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
                 HIP_OK(hipGetLastError());
                 wait_with_progress(F, progress);
@@ -675,7 +684,7 @@ struct Renderer {
             if (stats) HIP_OK(hipEventRecord(ev[3].a, st));
             if (n_closest + n_done > 0)
                 launch_persist(grid_for((long long)n_closest + n_done), kBlock, st, g.ds, F, P, n_closest,
-                                                                                      B.spill.p);
+                                                                                      B.spill.p, occupancy);
             if (stats) HIP_OK(hipEventRecord(ev[3].b, st));
             sync_counters();
             if (stats) {
@@ -760,7 +769,8 @@ struct Renderer {
     // each pixel's samples still run in order on its own stream.
     void run_cost_ordered(const Frame &F, int first, long long tail, volatile int32_t *progress,
                           lambda_render_stats *stats) {
-        const int tiles_x = (F.width + 7) / 8, tiles_y = (F.height + 7) / 8, ntiles = tiles_x * tiles_y;
+        const int tiles_x = (F.width + tile - 1) / tile, tiles_y = (F.height + tile - 1) / tile;
+        const int ntiles = tiles_x * tiles_y;
         DevBuf<int> &cost = C.cost, &order = C.order, &tile_ids = C.tile_ids, &offsets = C.offsets;
         DevBuf<long long> &tile_cost = C.tile_cost;
         DevBuf<unsigned long long> &t0 = C.t0;
@@ -781,7 +791,7 @@ struct Renderer {
         F1.unit_t0 = t0.p;
         F1.rng_save = save.p;
         run(F1, tail, nullptr, stats, true);
-        launch_tile_cost(grid_for(F.npix), kBlock, st, cost.p, F.width, F.height, tile_cost.p);
+        launch_tile_cost(grid_for(F.npix), kBlock, st, cost.p, F.width, F.height, tile, tile_cost.p);
         std::vector<long long> tc(static_cast<size_t>(ntiles));
         HIP_OK(hipMemcpyAsync(tc.data(), tile_cost.p, tc.size() * sizeof(long long), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
@@ -792,12 +802,12 @@ struct Renderer {
         for (int i = 0; i < ntiles; i++) {
             const int t = ids[size_t(i)];
             offs[size_t(i)] = pos;
-            pos += std::min(8, F.width - 8 * (t % tiles_x)) * std::min(8, F.height - 8 * (t / tiles_x));
+            pos += std::min(tile, F.width - tile * (t % tiles_x)) * std::min(tile, F.height - tile * (t / tiles_x));
         }
         HIP_OK(hipMemcpyAsync(tile_ids.p, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync(offsets.p, offs.data(), offs.size() * sizeof(int), hipMemcpyHostToDevice, st));
-        launch_tile_expand(grid_for((long long)ntiles * 64), kBlock, st, tile_ids.p, offsets.p, ntiles, F.width,
-                                                                           F.height, order.p);
+        launch_tile_expand(grid_for((long long)ntiles * tile * tile), kBlock, st, tile_ids.p, offsets.p, ntiles, F.width,
+                                                                           F.height, tile, order.p);
         HIP_OK(hipGetLastError());
         Frame F2 = F;
         F2.sample_begin = first;
@@ -825,7 +835,7 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
     // Slots: paths in flight.  Reference mode: at most one per pixel (a slot
     // owns a pixel for all its samples).
     const long long units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
-    long long slots = p.path_slots > 0 ? p.path_slots : resident_persist_threads();
+    long long slots = p.path_slots > 0 ? p.path_slots : resident_persist_threads(p.occupancy);
     slots = std::max<long long>(1, std::min<long long>(slots, units));
     if (slots > (1LL << 28)) throw Failure{LAMBDA_EINVAL, "too many path slots"};
     // Below this many live paths the rest of the render runs in one
@@ -857,6 +867,13 @@ void render_with(Renderer &R, const Frame &F, const lambda_render_params &p, lon
         std::memset(stats, 0, sizeof(*stats));
         stats->path_slots = R.ns;
     }
+    const int tile = p.tile > 0 ? p.tile : 8;
+    if (tile != R.tile) {
+        R.B.order.upload(tile_order(F.width, F.height, tile));
+        R.P.order = R.B.order.p;
+        R.tile = tile;
+    }
+    R.occupancy = p.occupancy;
     // reference streams, persistent schedule, more pixels than slots: the
     // two-launch cost order (cost_order samples first; 0 = automatic: 4, the
     // best measured on the 1080p instanced-mesh frame against 1, 2 and 8)
@@ -946,6 +963,333 @@ int guarded(char *err, size_t len, F &&fn) {
         set_err(err, len, "unknown failure");
         return LAMBDA_EINVAL;
     }
+}
+
+}  // namespace
+
+
+// ------------------------------------------------------------ live serving
+//
+// A scheduler thread owns every HIP call of a server.  Submitted jobs wait
+// in a FIFO; a group of them is admitted while the in-flight jobs' buffers
+// (36 bytes per pixel: the FP64 XYZ sums and the RGB result) fit the HBM
+// budget, and runs as one k_serve launch on one of the server's launch
+// slots (stream, path state, buffers that only grow).  Launches on
+// different slots overlap: a launch sized to fewer pixels than the resident
+// threads, or one draining its last pixels, leaves compute units to the
+// next.  Buffers are never freed while launches run (hipFree may wait for
+// the device); they are reused by the slot's next launch.
+
+struct lambda_server {
+    struct Job {
+        lambda_job job;
+        int64_t ticket = 0;
+        double submit_ms = 0;
+        int32_t state = 0;          // LAMBDA_JOB_*
+        lambda_job_result result{};
+        std::string error;
+    };
+    struct Slot {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        RenderBuffers B;            // path state of the launch's threads
+        DevBuf<ServeCtl> ctl;
+        DevBuf<double> fb;
+        DevBuf<float> rgb;
+        DevBuf<int> done;
+        DevBuf<unsigned long long> finish;
+        DevBuf<ServeJob> djobs;
+        std::vector<ServeJob> hjobs;
+        float *staging = nullptr;   // pinned: RGB of the launch's jobs
+        size_t staging_n = 0;
+        unsigned long long *hfinish = nullptr;  // pinned: finish ticks, then t0, error flag, rays
+        size_t hfinish_n = 0;
+        std::vector<int64_t> tickets;
+        long long bytes = 0;        // budget held by the launch
+        int launch_index = -1;
+        bool busy = false;
+    };
+    std::vector<DeviceScene> table;
+    DevBuf<DeviceScene> dscenes;
+    int stack_need = 0;
+    long long threads = 0;
+    int64_t budget = 0;
+    double ticks_per_ms = 100000.0;
+    unsigned long long cal_tick = 0;
+    double cal_ms = 0;
+    std::chrono::steady_clock::time_point t0;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int64_t> pending;
+    std::unordered_map<int64_t, Job> jobs;
+    int64_t next_ticket = 1;
+    long long inflight_bytes = 0;
+    int launches = 0;
+    long long paths = 0, closest = 0, shadow = 0;
+    double last_finish_ms = 0;
+    std::string fatal;
+    bool stop = false;
+    std::thread worker;
+
+    double now_ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    double tick_ms(unsigned long long tick) const { return cal_ms + (double(tick) - double(cal_tick)) / ticks_per_ms; }
+    void run();
+    void launch(Slot &sl, std::vector<int64_t> batch);
+    void finalize(Slot &sl);
+    ~lambda_server() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        if (worker.joinable()) worker.join();
+        for (auto &sl : slots) {
+            if (sl->staging) (void)hipHostFree(sl->staging);
+            if (sl->hfinish) (void)hipHostFree(sl->hfinish);
+            if (sl->ev) (void)hipEventDestroy(sl->ev);
+            if (sl->st) (void)hipStreamDestroy(sl->st);
+        }
+    }
+};
+
+namespace {
+
+constexpr long long kServeBytesPerPixel = 3 * 8 + 3 * 4;
+
+template <typename T>
+void grow(DevBuf<T> &b, size_t n) {
+    if (b.n < n) b.alloc(n);
+}
+
+template <typename T>
+void grow_pinned(T *&p, size_t &cap, size_t n) {
+    if (cap >= n) return;
+    if (p) HIP_OK(hipHostFree(p));
+    p = nullptr;
+    cap = 0;
+    HIP_OK(hipHostMalloc((void **)&p, n * sizeof(T), hipHostMallocDefault));
+    cap = n;
+}
+
+}  // namespace
+
+// One launch for the admitted jobs `batch` (the caller has taken their
+// budget): the jobs with the longest per-pixel chains first.
+void lambda_server::launch(Slot &sl, std::vector<int64_t> batch) {
+    std::vector<const Job *> js;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        for (int64_t t : batch) js.push_back(&jobs.at(t));
+    }
+    std::vector<size_t> order(js.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        const lambda_render_params &pa = js[a]->job.params, &pb = js[b]->job.params;
+        return (long long)pa.samples * (pa.depth + 1) > (long long)pb.samples * (pb.depth + 1);
+    });
+    sl.tickets.clear();
+    long long px_total = 0;
+    for (size_t k : order) {
+        sl.tickets.push_back(js[k]->ticket);
+        px_total += (long long)js[k]->job.params.width * js[k]->job.params.height;
+    }
+    const size_t nj = sl.tickets.size();
+    grow(sl.fb, size_t(px_total) * 3);
+    grow(sl.rgb, size_t(px_total) * 3);
+    grow(sl.done, nj);
+    grow(sl.finish, nj);
+    grow(sl.djobs, nj);
+    grow_pinned(sl.staging, sl.staging_n, size_t(px_total) * 3);
+    grow_pinned(sl.hfinish, sl.hfinish_n, nj + 4);
+    sl.hjobs.assign(nj, ServeJob{});
+    long long units = 0;
+    for (size_t k = 0; k < nj; k++) {
+        lambda_render_params p = js[order[k]]->job.params;
+        p.rng_mode = LAMBDA_RNG_REFERENCE;
+        ServeJob &J = sl.hjobs[k];
+        J.F = make_frame(p);
+        J.fb = sl.fb.p + units * 3;
+        J.first_unit = units;
+        J.arrival = 0;
+        J.scene = js[order[k]]->job.scene;
+        J.units = J.F.npix;
+        units += J.units;
+    }
+    Paths P{};
+    RenderBuffers &B = sl.B;
+    P.o = B.o.p; P.d = B.d.p; P.lam = B.lam.p; P.beta = B.beta.p; P.L = B.L.p;
+    P.rng = B.rng.p; P.info = B.info.p; P.hit = B.hit.p; P.ctr = B.ctr.p; P.rays = B.rays.p;
+    const hipStream_t st = sl.st;
+    HIP_OK(hipMemcpyAsync(sl.djobs.p, sl.hjobs.data(), nj * sizeof(ServeJob), hipMemcpyHostToDevice, st));
+    launch_fill_d(grid_for(px_total * 3), kBlock, st, sl.fb.p, size_t(px_total) * 3);
+    launch_fill_i(grid_for((long long)nj), kBlock, st, sl.done.p, int(nj));
+    launch_fill_i(1, kBlock, st, P.ctr, kCtrCount);
+    launch_fill_u64(1, kBlock, st, P.rays, 2);
+    launch_serve_clock(1, 1, st, sl.ctl.p);
+    const long long grid_threads = std::min<long long>(threads, (units + kBlock - 1) / kBlock * kBlock);
+    launch_serve(int(grid_threads / kBlock), kBlock, st, dscenes.p, sl.djobs.p, int(nj), units, P, sl.ctl.p,
+                 sl.done.p, sl.finish.p, B.spill.p);
+    long long off = 0;
+    for (size_t k = 0; k < nj; k++) {
+        launch_resolve(grid_for(sl.hjobs[k].F.npix), kBlock, st, sl.hjobs[k].F, sl.hjobs[k].fb, sl.rgb.p + off * 3);
+        off += sl.hjobs[k].F.npix;
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(sl.staging, sl.rgb.p, size_t(px_total) * 3 * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(sl.hfinish, sl.finish.p, nj * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(sl.hfinish + nj, &sl.ctl.p->t0, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(sl.hfinish + nj + 1, P.ctr + kCtrError, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(sl.hfinish + nj + 2, P.rays, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipEventRecord(sl.ev, st));
+    std::lock_guard<std::mutex> g(mu);
+    sl.launch_index = launches++;
+    for (int64_t t : sl.tickets) jobs.at(t).state = LAMBDA_JOB_RUNNING;
+}
+
+// Results of a finished launch: images to the callers' buffers, latencies
+// from the device finish ticks.
+void lambda_server::finalize(Slot &sl) {
+    const size_t nj = sl.tickets.size();
+    const bool bad = (sl.hfinish[nj + 1] & 0xffffffffull) != 0;
+    long long off = 0;
+    std::lock_guard<std::mutex> g(mu);
+    for (size_t k = 0; k < nj; k++) {
+        Job &jb = jobs.at(sl.tickets[k]);
+        const ServeJob &J = sl.hjobs[k];
+        if (bad) {
+            jb.state = LAMBDA_JOB_FAILED;
+            jb.error = "device reported an inconsistent hit record";
+        } else {
+            std::memcpy(jb.job.out_rgb, sl.staging + off * 3, size_t(J.F.npix) * 3 * sizeof(float));
+            jb.result.finish_ms = tick_ms(sl.hfinish[k]);
+            jb.result.latency_ms = jb.result.finish_ms - jb.submit_ms;
+            jb.result.batch = sl.launch_index;
+            jb.state = LAMBDA_JOB_DONE;
+            last_finish_ms = std::max(last_finish_ms, jb.result.finish_ms);
+        }
+        off += J.F.npix;
+        paths += (long long)J.F.npix * J.F.spp;
+    }
+    closest += (long long)sl.hfinish[nj + 2];
+    shadow += (long long)sl.hfinish[nj + 3];
+    inflight_bytes -= sl.bytes;
+    sl.bytes = 0;
+    sl.busy = false;
+    cv.notify_all();
+}
+
+void lambda_server::run() {
+    try {
+        while (true) {
+            bool progressed = false;
+            for (auto &slp : slots) {
+                Slot &sl = *slp;
+                if (!sl.busy) continue;
+                const hipError_t q = hipEventQuery(sl.ev);
+                if (q == hipErrorNotReady) continue;
+                HIP_OK(q);
+                finalize(sl);
+                progressed = true;
+            }
+            Slot *free_slot = nullptr;
+            std::vector<int64_t> batch;
+            {
+                std::unique_lock<std::mutex> g(mu);
+                bool any_busy = false;
+                for (auto &slp : slots) {
+                    if (slp->busy) any_busy = true;
+                    else if (!free_slot) free_slot = slp.get();
+                }
+                if (stop && pending.empty() && !any_busy) return;
+                if (free_slot) {
+                    long long bytes = 0;
+                    while (!pending.empty()) {
+                        const Job &jb = jobs.at(pending.front());
+                        const long long jbytes =
+                            (long long)jb.job.params.width * jb.job.params.height * kServeBytesPerPixel;
+                        const bool alone = !any_busy && batch.empty();
+                        if (budget > 0 && inflight_bytes + bytes + jbytes > budget && !alone) break;
+                        bytes += jbytes;
+                        batch.push_back(pending.front());
+                        pending.pop_front();
+                    }
+                    if (!batch.empty()) {
+                        // the slot is taken now, so that a drain cannot see
+                        // the group neither queued nor running
+                        inflight_bytes += bytes;
+                        free_slot->bytes = bytes;
+                        free_slot->busy = true;
+                    }
+                }
+                if (batch.empty() && !progressed) {
+                    // nothing to start: wait for a submission, or poll the launches
+                    cv.wait_for(g, std::chrono::microseconds(any_busy ? 200 : 20000));
+                    continue;
+                }
+            }
+            if (!batch.empty()) launch(*free_slot, std::move(batch));
+        }
+    } catch (const Failure &f) {
+        std::lock_guard<std::mutex> g(mu);
+        fatal = f.msg;
+    } catch (const std::exception &e) {
+        std::lock_guard<std::mutex> g(mu);
+        fatal = e.what();
+    }
+    std::lock_guard<std::mutex> g(mu);
+    for (auto &kv : jobs)
+        if (kv.second.state != LAMBDA_JOB_DONE) {
+            kv.second.state = LAMBDA_JOB_FAILED;
+            kv.second.error = fatal;
+        }
+    for (auto &slp : slots) slp->busy = false;
+    pending.clear();
+    inflight_bytes = 0;
+    cv.notify_all();
+}
+
+namespace {
+
+void server_setup(lambda_server &srv, lambda_hip_scene *const *scenes, int32_t num_scenes, int32_t max_launches) {
+    srv.t0 = std::chrono::steady_clock::now();
+    srv.table.resize(size_t(num_scenes));
+    for (int k = 0; k < num_scenes; k++) {
+        if (!scenes[k]) throw Failure{LAMBDA_EINVAL, "null scene"};
+        srv.table[size_t(k)] = scenes[k]->ds;
+        srv.stack_need = std::max(srv.stack_need, scenes[k]->stack_need);
+    }
+    srv.dscenes.upload(srv.table);
+    srv.threads = resident_persist_threads();
+    int dev = 0, rate_khz = 0;
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+    srv.ticks_per_ms = double(std::max(rate_khz, 1));
+    for (int k = 0; k < max_launches; k++) {
+        auto sl = std::make_unique<lambda_server::Slot>();
+        HIP_OK(hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&sl->ev, hipEventDisableTiming));
+        const size_t n = size_t(srv.threads);
+        RenderBuffers &B = sl->B;
+        B.o.alloc(n); B.d.alloc(n); B.lam.alloc(n); B.beta.alloc(n); B.L.alloc(n); B.rng.alloc(n);
+        B.info.alloc(n); B.hit.alloc(n);
+        B.ctr.alloc(kCtrCount);
+        B.rays.alloc(2);
+        if (srv.stack_need > kRing) B.spill.alloc(size_t(srv.stack_need - kRing) * n);
+        sl->ctl.alloc(1);
+        srv.slots.push_back(std::move(sl));
+    }
+    // device clock -> server milliseconds: one clock read bracketed by host times
+    lambda_server::Slot &s0 = *srv.slots[0];
+    unsigned long long tick = 0;
+    const double before = srv.now_ms();
+    launch_serve_clock(1, 1, s0.st, s0.ctl.p);
+    HIP_OK(hipMemcpyAsync(&tick, &s0.ctl.p->t0, sizeof tick, hipMemcpyDeviceToHost, s0.st));
+    HIP_OK(hipStreamSynchronize(s0.st));
+    srv.cal_ms = 0.5 * (before + srv.now_ms());
+    srv.cal_tick = tick;
 }
 
 }  // namespace
@@ -1210,6 +1554,85 @@ int lambda_hip_scene_info(const lambda_hip_scene *scene, lambda_scene_info *info
         for (int k = 0; k < 6; k++) info->bounds[k] = scene->box[k];
     });
 }
+
+int lambda_hip_server_create(lambda_hip_scene *const *scenes, int32_t num_scenes, int64_t hbm_budget,
+                             int32_t max_launches, lambda_server **out, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scenes || num_scenes < 1 || !out || hbm_budget < 0 || max_launches < 0 || max_launches > 8)
+            throw Failure{LAMBDA_EINVAL, "invalid arguments"};
+        *out = nullptr;
+        check_device();
+        auto srv = std::make_unique<lambda_server>();
+        srv->budget = hbm_budget;
+        server_setup(*srv, scenes, num_scenes, max_launches > 0 ? max_launches : 2);
+        lambda_server *p = srv.get();
+        srv->worker = std::thread([p] { p->run(); });
+        *out = srv.release();
+    });
+}
+
+int lambda_hip_server_submit(lambda_server *srv, const lambda_job *job, int64_t *ticket, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!srv || !job || !ticket) throw Failure{LAMBDA_EINVAL, "null argument"};
+        if (job->scene < 0 || job->scene >= int(srv->table.size())) throw Failure{LAMBDA_EINVAL, "job scene out of range"};
+        if (!job->out_rgb) throw Failure{LAMBDA_EINVAL, "job without output buffer"};
+        lambda_render_params p = job->params;
+        p.rng_mode = LAMBDA_RNG_REFERENCE;
+        validate_params(p);
+        std::lock_guard<std::mutex> g(srv->mu);
+        if (!srv->fatal.empty()) throw Failure{LAMBDA_EHIP, "server stopped: " + srv->fatal};
+        if (srv->pending.size() >= 1u << 20) throw Failure{LAMBDA_EBUSY, "job queue full"};
+        lambda_server::Job jb;
+        jb.job = *job;
+        jb.job.params = p;
+        jb.ticket = srv->next_ticket++;
+        jb.submit_ms = srv->now_ms();
+        jb.state = LAMBDA_JOB_QUEUED;
+        *ticket = jb.ticket;
+        srv->jobs.emplace(jb.ticket, jb);
+        srv->pending.push_back(jb.ticket);
+        srv->cv.notify_all();
+    });
+}
+
+int lambda_hip_server_poll(lambda_server *srv, int64_t ticket, int32_t *state, lambda_job_result *result, char *err,
+                           size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!srv || !state) throw Failure{LAMBDA_EINVAL, "null argument"};
+        std::lock_guard<std::mutex> g(srv->mu);
+        auto it = srv->jobs.find(ticket);
+        if (it == srv->jobs.end()) throw Failure{LAMBDA_EINVAL, "unknown ticket"};
+        *state = it->second.state;
+        if (it->second.state == LAMBDA_JOB_FAILED) throw Failure{LAMBDA_EHIP, it->second.error};
+        if (result && it->second.state == LAMBDA_JOB_DONE) *result = it->second.result;
+    });
+}
+
+int lambda_hip_server_drain(lambda_server *srv, lambda_serve_stats *stats, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!srv) throw Failure{LAMBDA_EINVAL, "null argument"};
+        std::unique_lock<std::mutex> g(srv->mu);
+        srv->cv.wait(g, [&] {
+            if (!srv->fatal.empty()) return true;
+            if (!srv->pending.empty()) return false;
+            for (auto &sl : srv->slots)
+                if (sl->busy) return false;
+            return true;
+        });
+        if (!srv->fatal.empty()) throw Failure{LAMBDA_EHIP, "server stopped: " + srv->fatal};
+        if (stats) {
+            stats->batches = srv->launches;
+            stats->paths = srv->paths;
+            stats->closest_rays = srv->closest;
+            stats->shadow_rays = srv->shadow;
+            double first = -1;
+            for (auto &kv : srv->jobs) first = first < 0 ? kv.second.submit_ms : std::min(first, kv.second.submit_ms);
+            stats->wall_ms = first < 0 ? 0 : srv->last_finish_ms - first;
+        }
+    });
+}
+
+void lambda_hip_server_destroy(lambda_server *srv) { delete srv; }
 
 int lambda_hip_serve(lambda_hip_scene *const *scenes, int32_t num_scenes, const lambda_job *jobs, int32_t num_jobs,
                      int64_t hbm_budget, lambda_job_result *results, lambda_serve_stats *stats, char *err,

@@ -17,14 +17,19 @@ void launch_set_i(dim3 grid, dim3 block, hipStream_t st, int *p, int v);
 void launch_fill_d(dim3 grid, dim3 block, hipStream_t st, double *p, size_t n);
 void launch_fill_u64(dim3 grid, dim3 block, hipStream_t st, unsigned long long *p, size_t n);
 void launch_fill_i(dim3 grid, dim3 block, hipStream_t st, int *p, int n);
-void launch_tile_cost(dim3 grid, dim3 block, hipStream_t st, const int *cost, int w, int h, long long *tile_cost);
-void launch_tile_expand(dim3 grid, dim3 block, hipStream_t st, const int *tiles, const int *offsets, int ntiles, int w, int h, int *order);
+void launch_tile_cost(dim3 grid, dim3 block, hipStream_t st, const int *cost, int w, int h, int tile, long long *tile_cost);
+void launch_tile_expand(dim3 grid, dim3 block, hipStream_t st, const int *tiles, const int *offsets, int ntiles, int w, int h, int tile, int *order);
 void launch_shade(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, int count, const int *counts);
 void launch_shade_guided_queue(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, int count, GuideQueue Q);
 void launch_shade_guided(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, GuideQueue Q);
-void launch_persist(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, int n_in, int2 *spill);
-// Blocks of kBlock threads of each persistent kernel resident per compute unit.
-int persist_blocks_per_cu();
+// The persistent kernel variant with `occupancy` waves per SIMD (0: the
+// default, kPersistWaves); blocks of kBlock threads resident per compute unit.
+void launch_persist(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, int n_in, int2 *spill,
+                    int occupancy);
+int persist_blocks_per_cu(int occupancy);
+void launch_persist_occ(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, int n_in, int2 *spill,
+                        int occupancy);
+int persist_occ_blocks_per_cu(int occupancy);
 void launch_persist_guided(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, int n_in, int2 *spill, GuideDev G);
 int persist_guided_blocks_per_cu();
 void launch_serve(dim3 grid, dim3 block, hipStream_t st, const DeviceScene *scenes, const ServeJob *jobs, int njobs, long long total_units, Paths P, ServeCtl *ctl, int *done, unsigned long long *finish, int2 *spill);

@@ -25,6 +25,9 @@ def test_candidates_cover_the_schedules():
     assert any(c.get("graph_passes", 0) > 0 for c in ref)
     # the cost order applies to reference streams only
     assert all(c["cost_order"] == 0 for c in thr)
+    # tile sizes and the compiled occupancy variants
+    assert {c["tile"] for c in ref} == {4, 8, 16}
+    assert {c["occupancy"] for c in ref} == {2, 3, 4}
 
 
 def test_key_depends_on_scene_and_frame_and_cache_is_used(tmp_path):
