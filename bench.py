@@ -315,6 +315,54 @@ def copy_probe(nbytes: int) -> None:
     log("copy probe: %d bytes read, %d bytes written" % (nbytes, nbytes))
 
 
+def kernel_source_hash() -> str:
+    """SHA-256 over the device sources (lambda_amd/csrc): a traffic record is
+    used only with the sources it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "lambda_amd", "csrc")
+    for fn in sorted(os.listdir(d)):
+        with open(os.path.join(d, fn), "rb") as fh:
+            h.update(fn.encode() + b"\0" + fh.read())
+    return h.hexdigest()
+
+
+def traffic_summary(fetch_csv: str, write_csv: str, kernel: str, copy_bytes: int, out: str) -> dict:
+    """Writes the traffic record of `kernel` from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) of the same command, each
+    scaled by what the counter reads for the copy probe's known bytes."""
+    import csv
+
+    def read(path, counter):
+        vals, copy = [], []
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            v = float(r["Counter_Value"]) * 1024.0
+            name = r["Kernel_Name"]
+            if kernel in name:
+                vals.append(v)
+            elif "copyBuffer" in name:
+                copy.append(v)
+        if not vals or not copy:
+            raise SystemExit("%s: no %s rows for %s and the copy probe" % (path, counter, kernel))
+        return vals, max(copy)
+
+    f, fc = read(fetch_csv, "FETCH_SIZE")
+    w, wc = read(write_csv, "WRITE_SIZE")
+    fs, ws = copy_bytes / fc, copy_bytes / wc
+    rec = {"kernel": kernel, "workload": "instanced_mesh_1080p",
+           "bytes_per_launch": (sum(f) / len(f)) * fs + (sum(w) / len(w)) * ws,
+           "fetch_bytes_per_launch": (sum(f) / len(f)) * fs, "write_bytes_per_launch": (sum(w) / len(w)) * ws,
+           "fetch_scale": fs, "write_scale": ws, "dispatches_averaged": len(f),
+           "calibration": "one %d-byte device-to-device hipMemcpy (__amd_rocclr_copyBuffer) in the same process: "
+                          "FETCH_SIZE %.1f KiB, WRITE_SIZE %.1f KiB" % (copy_bytes, fc / 1024, wc / 1024),
+           "source_sha256": kernel_source_hash()}
+    with open(out, "w") as fh:
+        json.dump(rec, fh, indent=1)
+    return rec
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,7 +412,15 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (a plain copy kernel of known traffic, "
                          "the calibration point for hardware byte counters)")
+    ap.add_argument("--traffic-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_BYTES"),
+                    help="write the k_persist traffic record from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
+    if args.traffic_summary:
+        fc, wc, nb = args.traffic_summary
+        rec = traffic_summary(fc, wc, "k_persist", int(nb),
+                              os.path.join(ROOT, "profiles", "persist_traffic_instanced_mesh_1080p.json"))
+        print(json.dumps(rec))
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the renderer has no multi-GPU path (frames shard by pixel bands; "
@@ -466,12 +522,18 @@ def main() -> int:
     log("traversal work from %d sampled pixels (%.1f s): %.0f B per closest ray, %.0f B per shadow ray" % (
         ps["paths"] // max(1, p.samples), t_stats, b_closest, b_shadow))
 
-    traffic = None
+    # HBM bytes per launch from the copy-calibrated FETCH_SIZE + WRITE_SIZE
+    # counters (profiles/), valid only for the kernel sources they were
+    # measured on
+    traffic, traffic_note = None, "no counter record for this kernel and workload"
     tpath = os.path.join(ROOT, "profiles", "%s_traffic_%s.json" % (kern, cfg["workload"]))
     if os.path.exists(tpath):
         with open(tpath) as fh:
             tj = json.load(fh)
-        traffic = tj.get("bytes_per_launch")
+        if tj.get("source_sha256") == kernel_source_hash():
+            traffic, traffic_note = tj.get("bytes_per_launch"), "counters of these kernel sources (%s)" % os.path.basename(tpath)
+        else:
+            traffic_note = "counter record is stale (kernel sources changed since it was measured)"
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -514,7 +576,7 @@ def main() -> int:
         "config": cfg,
         "roofline": {
             "bound": "hbm", "kernel": "k_" + kern, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
             "bytes_per_closest_ray": b_closest, "bytes_per_shadow_ray": b_shadow, "launches": launches,
             "avg_launch_ms": avg_launch_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch,
         },
