@@ -219,6 +219,22 @@ typedef struct lambda_hip_scene lambda_hip_scene;
 int lambda_hip_scene_create(const lambda_scene_view *scene, lambda_hip_scene **out,
                             char *err, size_t err_len);
 
+/*
+ * The same, with the BVHs built on the GPU (no reference counterpart: the
+ * reference builds a SAH BVH on the host, scene/bvh.h:61-246; this is for
+ * scenes whose geometry changes between frames).  The node arrays of the
+ * view are not read: each object group (the object range of the instances)
+ * and the instance level get a linear BVH (Morton codes, radix sort, binary
+ * radix tree, one primitive per leaf).  Objects and instances are renumbered
+ * in the new trees' leaf order: object_order[k] (O entries, may be NULL) and
+ * instance_order[k] (I entries) give the view's index of the new object and
+ * instance k; hit IDs are in the new numbering, hit distances are those of
+ * the host-built BVH.  Instance object ranges must tile the object array.
+ * *build_ms (may be NULL): the BVH build's wall time.
+ */
+int lambda_hip_scene_create_lbvh(const lambda_scene_view *scene, lambda_hip_scene **out, int32_t *object_order,
+                                 int32_t *instance_order, double *build_ms, char *err, size_t err_len);
+
 /* Releases every device allocation of the scene. */
 void lambda_hip_scene_destroy(lambda_hip_scene *scene);
 

@@ -138,12 +138,28 @@ class DeviceScene:
     counterpart of the arrays main.cpp copies to device memory
     (src/main.cpp:84-103)."""
 
-    def __init__(self, scene: Scene):
+    def __init__(self, scene: Scene, gpu_bvh: bool = False):
+        """gpu_bvh: build the BVHs on the GPU (lambda_hip_scene_create_lbvh)
+        instead of using the host-built ones; object_order and
+        instance_order then map the renumbered objects and instances to the
+        scene's, and bvh_build_ms is the build's time."""
         self._lib = _abi.hip()
         self._h = C.c_void_p()
         self.scene = scene
+        self.object_order = self.instance_order = None
+        self.bvh_build_ms = None
         err = C.create_string_buffer(4096)
-        _check(self._lib.lambda_hip_scene_create(C.byref(scene.view), C.byref(self._h), err, 4096), err)
+        if gpu_bvh:
+            v = scene.view
+            self.object_order = np.zeros(v.num_objects, np.int32)
+            self.instance_order = np.zeros(v.num_instances, np.int32)
+            ms = C.c_double()
+            _check(self._lib.lambda_hip_scene_create_lbvh(
+                C.byref(v), C.byref(self._h), _abi.ptr(self.object_order, C.c_int32),
+                _abi.ptr(self.instance_order, C.c_int32), C.byref(ms), err, 4096), err)
+            self.bvh_build_ms = ms.value
+        else:
+            _check(self._lib.lambda_hip_scene_create(C.byref(scene.view), C.byref(self._h), err, 4096), err)
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -149,6 +149,8 @@ def hip():
         rp = C.POINTER(RenderParams)
         lib.lambda_hip_render.argtypes = [sv, rp, _f32p, _i32p, C.c_char_p, C.c_size_t]
         lib.lambda_hip_scene_create.argtypes = [sv, C.POINTER(vp), C.c_char_p, C.c_size_t]
+        lib.lambda_hip_scene_create_lbvh.argtypes = [sv, C.POINTER(vp), _i32p, _i32p, C.POINTER(C.c_double),
+                                                     C.c_char_p, C.c_size_t]
         lib.lambda_hip_scene_destroy.argtypes = [vp]
         lib.lambda_hip_scene_destroy.restype = None
         lib.lambda_hip_scene_render.argtypes = [vp, rp, _f32p, _i32p, C.c_char_p, C.c_size_t]

@@ -37,6 +37,7 @@
 #include "guide_net.h"
 #include "integrator.h"
 #include "launch.h"
+#include "lbvh.h"
 #include "scene_prep.h"
 
 namespace lam {
@@ -85,7 +86,7 @@ int tree_stack_depth(const lambda_scene_view &v, int root, int leaf_limit, bool 
 
 
 
-void validate(const lambda_scene_view &v) {
+void validate(const lambda_scene_view &v, int max_depth = kStack) {
     require(v.num_nodes > 0 && v.num_objects > 0 && v.num_instances > 0, "empty scene");
     require(v.num_lights >= 1, "light list must end with the environment entry");
     require(v.num_spectra > 0 && v.num_spectrum_textures > 0 && v.num_scalar_textures > 0 && v.num_materials > 0,
@@ -155,10 +156,10 @@ void validate(const lambda_scene_view &v) {
     }
     require(v.light_instances[v.num_lights - 1] == -1, "last light must be the environment");
     require(v.background_texture >= 0 && v.background_texture < v.num_spectrum_textures, "background texture");
-    require(tree_stack_depth(v, 0, v.num_instances, true) <= kStack, "instance BVH deeper than 20");
+    require(tree_stack_depth(v, 0, v.num_instances, true) <= max_depth, "instance BVH too deep");
     for (int i = 0; i < v.num_instances; i++)
-        require(tree_stack_depth(v, v.instance_ints[4 * i + 2], v.num_objects, false) <= kStack,
-                "object BVH deeper than 20");
+        require(tree_stack_depth(v, v.instance_ints[4 * i + 2], v.num_objects, false) <= max_depth,
+                "object BVH too deep");
 }
 
 }  // namespace
@@ -191,8 +192,12 @@ struct lambda_guide {
 
 namespace {
 
-void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g) {
-    validate(v);
+// Deepest tree the GPU-built BVHs may have (binary radix trees of 30-bit
+// codes); the traversal's spill area is sized from the scene's own need.
+constexpr int kLbvhMaxDepth = 96;
+
+void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g, int max_depth = kStack) {
+    validate(v, max_depth);
     HIP_OK(hipGetDevice(&g.device));
     const size_t O = size_t(v.num_objects), I = size_t(v.num_instances);
     WideBuilder wb(v);
@@ -1313,6 +1318,71 @@ int lambda_hip_scene_create(const lambda_scene_view *scene, lambda_hip_scene **o
 }
 
 void lambda_hip_scene_destroy(lambda_hip_scene *scene) { delete scene; }
+
+int lambda_hip_scene_create_lbvh(const lambda_scene_view *scene, lambda_hip_scene **out, int32_t *object_order,
+                                 int32_t *instance_order, double *build_ms, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!scene || !out) throw Failure{LAMBDA_EINVAL, "null argument"};
+        *out = nullptr;
+        check_device();
+        const lambda_scene_view &v = *scene;
+        if (v.num_objects <= 0 || v.num_instances <= 0 || !v.object_type || !v.object_geometry || !v.instance_ints ||
+            !v.instance_matrices || !v.object_material || !v.object_shading || v.num_lights < 1 || !v.light_instances ||
+            !v.light_objects)
+            throw Failure{LAMBDA_EINVAL, "invalid scene: empty scene or null array"};
+        const auto t0 = std::chrono::steady_clock::now();
+        LbvhScene L;
+        build_lbvh(v, L);
+        HIP_OK(hipDeviceSynchronize());
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        const size_t O = size_t(v.num_objects), I = size_t(v.num_instances);
+        std::vector<int> inv_obj(O), inv_inst(I);
+        for (size_t k = 0; k < O; k++) inv_obj[size_t(L.object_order[k])] = int(k);
+        for (size_t k = 0; k < I; k++) inv_inst[size_t(L.instance_order[k])] = int(k);
+        std::vector<int> otype(O), omat(O), iints(I * 4), lins(size_t(v.num_lights)), lobj(size_t(v.num_lights));
+        std::vector<float> ogeom(O * 12), oshade(O * 15), imats(I * 32);
+        for (size_t k = 0; k < O; k++) {
+            const size_t src = size_t(L.object_order[k]);
+            otype[k] = v.object_type[src];
+            omat[k] = v.object_material[src];
+            std::memcpy(&ogeom[k * 12], v.object_geometry + src * 12, 12 * sizeof(float));
+            std::memcpy(&oshade[k * 15], v.object_shading + src * 15, 15 * sizeof(float));
+        }
+        for (size_t k = 0; k < I; k++) {
+            const size_t src = size_t(L.instance_order[k]);
+            iints[k * 4 + 0] = v.instance_ints[src * 4 + 0];
+            iints[k * 4 + 1] = v.instance_ints[src * 4 + 1];
+            iints[k * 4 + 2] = L.blas_root[src];
+            iints[k * 4 + 3] = v.instance_ints[src * 4 + 3];
+            std::memcpy(&imats[k * 32], v.instance_matrices + src * 32, 32 * sizeof(float));
+        }
+        for (int k = 0; k < v.num_lights; k++) {
+            const int li = v.light_instances[k], lo = v.light_objects[k];
+            if ((li != -1 && (li < 0 || size_t(li) >= I)) || (lo != -1 && (lo < 0 || size_t(lo) >= O)))
+                throw Failure{LAMBDA_EINVAL, "invalid scene: light index out of range"};
+            lins[size_t(k)] = li == -1 ? -1 : inv_inst[size_t(li)];
+            lobj[size_t(k)] = lo == -1 ? -1 : inv_obj[size_t(lo)];
+        }
+        lambda_scene_view w = v;
+        w.object_type = otype.data();
+        w.object_material = omat.data();
+        w.object_geometry = ogeom.data();
+        w.object_shading = oshade.data();
+        w.instance_ints = iints.data();
+        w.instance_matrices = imats.data();
+        w.num_nodes = int32_t(L.node_ints.size() / 2);
+        w.node_bounds = L.node_bounds.data();
+        w.node_ints = L.node_ints.data();
+        w.light_instances = lins.data();
+        w.light_objects = lobj.data();
+        auto g = std::make_unique<lambda_hip_scene>();
+        upload_scene(w, *g, kLbvhMaxDepth);
+        if (object_order) std::memcpy(object_order, L.object_order.data(), O * sizeof(int32_t));
+        if (instance_order) std::memcpy(instance_order, L.instance_order.data(), I * sizeof(int32_t));
+        if (build_ms) *build_ms = ms;
+        *out = g.release();
+    });
+}
 
 int lambda_hip_scene_render(lambda_hip_scene *scene, const lambda_render_params *params, float *out_rgb,
                             volatile int32_t *progress, char *err, size_t err_len) {
