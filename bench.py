@@ -111,17 +111,20 @@ def cpu_model() -> str:
 
 
 def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, warmup: int = 0,
-                   train_steps: int = 1 << 30, epochs: int = 1, batch: int = 1 << 14, guided_bounces: int = 0):
+                   train_steps: int = 1 << 30, epochs: int = 1, batch: int = 1 << 14, guided_bounces: int = 0,
+                   table_cells: int = 0):
     """Progressive render until relMSE against ref <= target; returns the
     render-and-train wall time at which the target was crossed (log-linear
     interpolation between steps), the samples per pixel then, and the curve.
     With a guide: `warmup` steps train without adding to the image, then
-    the first `train_steps` kept steps train too.  relMSE evaluation is not
-    timed."""
+    the first `train_steps` kept steps train too; table_cells > 0 guides
+    from the tabulated network.  relMSE evaluation is not timed."""
     import lambda_amd
     p = lambda_amd.Progressive(dev, guide)
     if guide is not None:
         p.set_training(epochs, batch, 1e-3, guided_bounces)
+        if table_cells:
+            p.set_guide_table(table_cells)
     elapsed, spp, curve = 0.0, 0, []
     prev = None
     hit = None

@@ -488,6 +488,8 @@ typedef struct lambda_progressive_stats {
     double render_ms;       /* device time of the last step's rendering */
     double train_ms;        /* device time of its training (0 if none) */
     double loss;            /* mean training loss of that training */
+    int64_t closest_rays;   /* closest-hit rays traced by the last step */
+    int64_t shadow_rays;    /* its next-event occlusion rays */
 } lambda_progressive_stats;
 
 /* params: size, depth, wavelength range, seed, path_slots and tail_paths
@@ -518,6 +520,17 @@ int lambda_hip_progressive_image(lambda_hip_progressive *session, float *out_rgb
  * passes and the rest in one persistent launch. */
 int lambda_hip_progressive_set_training(lambda_hip_progressive *session, int32_t epochs, int32_t batch, float lr,
                                         int32_t guided_bounces, char *err, size_t err_len);
+/* Tabulated guide (guided persistent sessions, before the first step): after
+ * each training, the network is evaluated once at the centres of a grid of
+ * cubic cells over the scene's bounding box (`cells` along its longest
+ * side) for each of the six axis directions as the surface normal, at the
+ * middle of the session's wavelength range; a guided vertex then samples the
+ * mixture of its cell and of the axis nearest its normal instead of running
+ * the network, in a kernel with the plain kernel's occupancy.  Steps stay
+ * unbiased (sampling and both MIS densities use the same entry).  cells = 0
+ * returns to the network inside the kernel; at most 1024 cells and 2^24
+ * entries (cells x 6), 160 bytes of device memory each. */
+int lambda_hip_progressive_set_guide_table(lambda_hip_progressive *session, int32_t cells, char *err, size_t err_len);
 void lambda_hip_progressive_destroy(lambda_hip_progressive *session);
 
 /* Device name and compute-unit count of the current HIP device. */

@@ -370,7 +370,8 @@ class Progressive:
         flags = (1 if train else 0) | (0 if keep else 2)
         _check(self._lib.lambda_hip_progressive_step(self._h, samples, flags, C.byref(st), err, 4096), err)
         return {"samples": st.samples, "train_samples": st.train_samples, "render_ms": st.render_ms,
-                "train_ms": st.train_ms, "loss": st.loss}
+                "train_ms": st.train_ms, "loss": st.loss, "closest_rays": st.closest_rays,
+                "shadow_rays": st.shadow_rays}
 
     def set_training(self, epochs: int = 1, batch: int = 1 << 14, lr: float = 1e-3, guided_bounces: int = 0) -> None:
         """Adam epochs per step, batch, rate, and the vertices guided: those
@@ -379,6 +380,15 @@ class Progressive:
         err = C.create_string_buffer(4096)
         _check(self._lib.lambda_hip_progressive_set_training(self._h, epochs, batch, lr, guided_bounces, err, 4096),
                err)
+
+    def set_guide_table(self, cells: int) -> None:
+        """Guide from a table of the network's mixtures, re-evaluated after
+        every training, over cubic cells (`cells` along the scene box's
+        longest side) and the six axis normals, instead of running the
+        network at every guided vertex; 0 = the network.  Before the first
+        step."""
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_progressive_set_guide_table(self._h, cells, err, 4096), err)
 
     def image(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 3), np.float32)

@@ -94,7 +94,7 @@ class ServeStats(C.Structure):
 class ProgressiveStats(C.Structure):
     _fields_ = [
         ("samples", C.c_int64), ("train_samples", C.c_int64), ("render_ms", C.c_double),
-        ("train_ms", C.c_double), ("loss", C.c_double),
+        ("train_ms", C.c_double), ("loss", C.c_double), ("closest_rays", C.c_int64), ("shadow_rays", C.c_int64),
     ]
 
 
@@ -192,6 +192,7 @@ def hip():
         lib.lambda_hip_progressive_image.argtypes = [vp, _f32p, C.c_char_p, C.c_size_t]
         lib.lambda_hip_progressive_set_training.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_int32,
                                                             C.c_char_p, C.c_size_t]
+        lib.lambda_hip_progressive_set_guide_table.argtypes = [vp, C.c_int32, C.c_char_p, C.c_size_t]
         lib.lambda_hip_progressive_destroy.argtypes = [vp]
         lib.lambda_hip_progressive_destroy.restype = None
         _hip = lib

@@ -181,21 +181,96 @@ LAM_FN void guide_compact(const float *o, float *c) {
     }
 }
 
+// The compact mixture's 40 floats in ten 16-byte loads (c is 16-byte
+// aligned: rows of kLobeOut floats from an aligned base).
+LAM_FN void guide_load_c(const float *c, float (&v)[kLobeOut]) {
+    const float4 *c4 = reinterpret_cast<const float4 *>(c);
+#pragma unroll
+    for (int i = 0; i < kLobeOut / 4; i++) {
+        const float4 q = c4[i];
+        v[4 * i] = q.x;
+        v[4 * i + 1] = q.y;
+        v[4 * i + 2] = q.z;
+        v[4 * i + 3] = q.w;
+    }
+}
+
 LAM_FN float guide_pdf_c(const float *c, V3 w) {
+    float v[kLobeOut];
+    guide_load_c(c, v);
     float p = 0;
 #pragma unroll
-    for (int k = 0; k < kLobes; k++) p += expf(c[5 * k + 3] + (c[5 * k] * w.x + c[5 * k + 1] * w.y + c[5 * k + 2] * w.z));
+    for (int k = 0; k < kLobes; k++) p += expf(v[5 * k + 3] + (v[5 * k] * w.x + v[5 * k + 1] * w.y + v[5 * k + 2] * w.z));
     return p;
 }
 
 LAM_FN V3 guide_sample_c(const float *c, float u0, float u1, float u2) {
+    // lobe k: the number of cumulative weights at or below u0 (they rise
+    // with k), counted without a data-dependent loop
     int k = 0;
-    while (k < kLobes - 1 && u0 >= c[5 * k + 4]) k++;
+#pragma unroll
+    for (int j = 0; j < kLobes - 1; j++) k += u0 >= c[5 * j + 4] ? 1 : 0;
     Lobe L;
     const V3 km = v3(c[5 * k], c[5 * k + 1], c[5 * k + 2]);
     L.kappa = len(km);
     L.mu = km * (1 / L.kappa);
     return vmf_sample(L, u1, u2);
+}
+
+// Tabulated guide: the network evaluated once per training step at the
+// centres of a grid of cubic cells over the scene box, for each of the six
+// axis directions as the normal, and stored in compact form (kLobeOut floats
+// per entry, entries [z][y][x][normal]).  A vertex takes the entry of the
+// cell that holds it and of the axis direction nearest its normal; the
+// mixture it samples and the densities its MIS weights use are both that
+// entry's, so the estimator stays unbiased whatever the table holds.
+constexpr int kTableNormals = 6;
+
+struct GuideTableDev {
+    const float *rows;
+    float lo[3];
+    float inv_cell;  // cells per unit length
+    int n[3];
+    float lambda;    // wavelength the entries were evaluated at
+};
+
+// Normal bin: 2 * axis + (component < 0) for the axis of the largest
+// component (ties to the lower axis).
+LAM_FN int guide_table_normal_bin(V3 n) {
+    const float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+    if (ax >= ay && ax >= az) return n.x < 0 ? 1 : 0;
+    if (ay >= az) return n.y < 0 ? 3 : 2;
+    return n.z < 0 ? 5 : 4;
+}
+
+LAM_FN V3 guide_table_normal(int bin) {
+    const float s = (bin & 1) ? -1.0f : 1.0f;
+    return bin < 2 ? v3(s, 0, 0) : bin < 4 ? v3(0, s, 0) : v3(0, 0, s);
+}
+
+LAM_FN int guide_table_cell(float x, float lo, float inv_cell, int n) {
+    const float f = floorf((x - lo) * inv_cell);
+    return !(f > 0) ? 0 : f >= float(n - 1) ? n - 1 : int(f);  // NaN -> 0
+}
+
+LAM_FN long long guide_table_entry(const GuideTableDev &T, V3 p, V3 n) {
+    const int ix = guide_table_cell(p.x, T.lo[0], T.inv_cell, T.n[0]);
+    const int iy = guide_table_cell(p.y, T.lo[1], T.inv_cell, T.n[1]);
+    const int iz = guide_table_cell(p.z, T.lo[2], T.inv_cell, T.n[2]);
+    return ((((long long)iz * T.n[1] + iy) * T.n[0] + ix) * kTableNormals + guide_table_normal_bin(n));
+}
+
+// Centre and normal of entry e.
+LAM_FN void guide_table_query(const GuideTableDev &T, long long e, V3 &p, V3 &n) {
+    const int bin = int(e % kTableNormals);
+    long long c = e / kTableNormals;
+    const int ix = int(c % T.n[0]);
+    c /= T.n[0];
+    const int iy = int(c % T.n[1]);
+    const int iz = int(c / T.n[1]);
+    const float cell = 1 / T.inv_cell;
+    p = v3(T.lo[0] + (ix + 0.5f) * cell, T.lo[1] + (iy + 0.5f) * cell, T.lo[2] + (iz + 0.5f) * cell);
+    n = guide_table_normal(bin);
 }
 
 // Loss of one training sample, -c log q(w), and its gradient with respect to

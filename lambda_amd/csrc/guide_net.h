@@ -33,6 +33,10 @@ struct GuideNet {
     // Minibatch Adam over samples [0, n): steps of `batch` samples, weights
     // capped at kWeightCap x their mean over [0, n).
     void train(const float *samples, int n, int batch, hipStream_t st);
+    // The table's entries (compact mixtures) from the current weights; feat
+    // is scratch of entries x kGuideScalars floats.
+    void tabulate(const GuideTableDev &T, float3 box_lo, float3 box_hi, float *feat, float *rows, long long entries,
+                  hipStream_t st) const;
 };
 
 }  // namespace lam

@@ -1,5 +1,5 @@
 // The guiding network's kernels -- inference, gradient, Adam, weight
-// packing -- and the host methods of GuideNet that launch them.
+// packing, tabulation -- and the host methods of GuideNet that launch them.
 #include <cmath>
 #include <vector>
 
@@ -202,6 +202,27 @@ __global__ void __launch_bounds__(1024) k_weight_cap(const float *samples, int n
 }
 
 
+// Features of table entries [0, n): each cell's centre with one of the six
+// axis directions as the normal, at the table's wavelength.
+__global__ void k_guide_table_features(GuideTableDev T, float3 box_lo, float3 box_hi, float *feat, long long n) {
+    const V3 lo = v3(box_lo.x, box_lo.y, box_lo.z), hi = v3(box_hi.x, box_hi.y, box_hi.z);
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        V3 p, nn;
+        guide_table_query(T, e, p, nn);
+        const GuideFeatures g = guide_features(p, lo, hi, nn, T.lambda);
+#pragma unroll
+        for (int k = 0; k < kGuideScalars; k++) feat[e * kGuideScalars + k] = g.f[k];
+    }
+}
+
+// Raw network outputs of rows [0, n) (stride kLobeOut) to the compact
+// mixture, in place.
+__global__ void k_guide_compact_rows(float *rows, long long n) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+        guide_compact(rows + e * kLobeOut, rows + e * kLobeOut);
+}
+
+
 }  // namespace
 
 void GuideNet::alloc() {
@@ -294,6 +315,20 @@ void GuideNet::train(const float *samples, int n, int batch, hipStream_t st) {
         gradient(samples + size_t(b0) * kGuideSample, batch, st, cap.p);
         adam(1.0f / float(batch), st);
     }
+}
+
+// The table's entries from the current weights: features of every entry
+// into feat (entries x kGuideScalars floats), the network into rows
+// (entries x kLobeOut), then the compact form in place.
+void GuideNet::tabulate(const GuideTableDev &T, float3 box_lo, float3 box_hi, float *feat, float *rows,
+                        long long entries, hipStream_t st) const {
+    if (entries <= 0) return;
+    const int blocks = int(std::min<long long>((entries + 255) / 256, 4096));
+    k_guide_table_features<<<blocks, 256, 0, st>>>(T, box_lo, box_hi, feat, entries);
+    HIP_OK(hipGetLastError());
+    forward(feat, kGuideScalars, int(entries), rows, kLobeOut, st);
+    k_guide_compact_rows<<<blocks, 256, 0, st>>>(rows, entries);
+    HIP_OK(hipGetLastError());
 }
 
 }  // namespace lam

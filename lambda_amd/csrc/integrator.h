@@ -219,6 +219,10 @@ struct ShadowRay {
 // -2 = no record, -1 = awaiting the next vertex.
 constexpr float kRecNone = -2.0f, kRecPending = -1.0f;
 
+// Probability of sampling the guide at a guided vertex; the cosine lobe
+// takes the rest, and both MIS densities use this one-sample mixture.
+constexpr float kGuideFrac = 0.5f;
+
 // One iteration body of Renderer::trace (render/renderer.h:102-192) for the
 // path in slot s after its closest-hit query (P.hit[s]).  Updates the slot's
 // state; returns true when the sample's path has ended.  *push_shadow tells
@@ -229,11 +233,14 @@ constexpr float kRecNone = -2.0f, kRecPending = -1.0f;
 // the compact form of guide_compact, features gfeat); the mixture density is the
 // scatter pdf of both MIS weights (next-event estimation and emitter hits),
 // so the estimator stays unbiased.  The vertex is recorded for training in
-// P.grec.  Without gout the vertex is shaded as in the plain tracer.
+// P.grec.  Without gout the vertex is shaded as in the plain tracer.  With
+// a table instead, a Lambertian vertex with bounce index below
+// F.guide_bounces takes its mixture from its table entry, and its features
+// are computed only when the vertex is recorded.
 template <bool kGuided = false>
 __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F, const Paths &P, int s,
                                            bool &push_shadow, ShadowRay &sh, const float *gout = nullptr,
-                                           const float *gfeat = nullptr) {
+                                           const float *gfeat = nullptr, const GuideTableDev *table = nullptr) {
     push_shadow = false;
     {
         float4 o4 = P.o[s], d4 = P.d[s];
@@ -291,6 +298,10 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     const bool lambertian = mt.x == LAMBDA_MATERIAL_LAMBERTIAN;
                     S4 albedo = s4(0);
                     if (lambertian) albedo = lambertian_albedo(S, mt, h, lam);
+                    if constexpr (kGuided) {
+                        if (table && lambertian && bounce < F.guide_bounces)
+                            gout = table->rows + guide_table_entry(*table, h.p, h.n) * kLobeOut;
+                    }
                     const bool guided = kGuided && lambertian && gout != nullptr;
                     if (lambertian && total > 0) {
                         // Next-event estimation (render/renderer.h:133-175).
@@ -333,7 +344,7 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                                 if (dot(h.n, ldir) > 0) att = albedo / kPi;
                                 float sp = ct > 0 ? ct / kPi : 0;
                                 if constexpr (kGuided) {
-                                    if (guided) sp = 0.5f * sp + 0.5f * guide_pdf_c(gout, ldir);
+                                    if (guided) sp = (1 - kGuideFrac) * sp + kGuideFrac * guide_pdf_c(gout, ldir);
                                 }
                                 float w = power_heuristic(lprob, sp);
                                 S4 c = beta * att * ct * em * w / lprob;
@@ -351,7 +362,7 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     if (lambertian) {
                         bool from_guide = false;
                         if constexpr (kGuided) {
-                            if (guided && rng.next() >= 0.5f) {
+                            if (guided && rng.next() >= 1 - kGuideFrac) {
                                 float u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
                                 nd = guide_sample_c(gout, u0, u1, u2);
                                 from_guide = true;
@@ -414,7 +425,7 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     prev_pdf = lambertian ? (ct > 0 ? ct / kPi : 0) : 0;
                     if constexpr (kGuided) {
                         if (guided) {
-                            prev_pdf = 0.5f * prev_pdf + 0.5f * guide_pdf_c(gout, d);
+                            prev_pdf = (1 - kGuideFrac) * prev_pdf + kGuideFrac * guide_pdf_c(gout, d);
                             // below the surface the contribution is zero
                             if (!(ct > 0) || !(prev_pdf > 0)) finished = true;
                         }
@@ -437,6 +448,12 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     if (bounce >= F.depth) finished = true;
                     if constexpr (kGuided) {
                         if (guided && !finished && bounce < F.grec_stride) {
+                            GuideFeatures tf;
+                            if (table) {
+                                tf = guide_features(h.p, v3(F.box_lo.x, F.box_lo.y, F.box_lo.z),
+                                                    v3(F.box_hi.x, F.box_hi.y, F.box_hi.z), h.n, lam.v[0]);
+                                gfeat = tf.f;
+                            }
                             float4 *r = P.grec + (size_t(s) * F.grec_stride + bounce) * 3;
                             r[0] = make_float4(gfeat[0], gfeat[1], gfeat[2], gfeat[3]);
                             r[1] = make_float4(gfeat[4], gfeat[5], d.x, d.y);

@@ -520,9 +520,11 @@ struct Renderer {
     // then one persistent launch.  With a guide: fused, one launch of the
     // guided persistent kernel (the network inside it); otherwise every pass
     // runs the network on the paths at guided vertices (no persistent
-    // launch).
+    // launch).  With a table instead of a guide: the persistent launch is
+    // the table-guided kernel.
     void run(const Frame &F, long long tail, volatile int32_t *progress, lambda_render_stats *stats,
-             bool clear = true, const GuideNet *guide = nullptr, GuideQueue *Q = nullptr, bool fused = false) {
+             bool clear = true, const GuideNet *guide = nullptr, GuideQueue *Q = nullptr, bool fused = false,
+             const GuideTableDev *table = nullptr) {
         const long long units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
         if (stats) {
             for (auto &e : ev) {
@@ -564,6 +566,9 @@ struct Renderer {
                 if (guide)
                     launch_persist_guided(grid_for((long long)n_closest + n_done), kBlock, st, 
                         g.ds, F, P, n_closest, B.spill.p, guide->dev());
+                else if (table)
+                    launch_persist_table(grid_for((long long)n_closest + n_done), kBlock, st, g.ds, F, P, n_closest,
+                                         B.spill.p, *table);
                 else
                     launch_persist(grid_for((long long)n_closest + n_done), kBlock, st, g.ds, F, P, n_closest,
                                                                                           B.spill.p, occupancy);
@@ -904,6 +909,10 @@ void render_with(Renderer &R, const Frame &F, const lambda_render_params &p, lon
 // eight bounces (on the guiding room, 2, 4, 6, 10, 12, 16 or every bounce
 // gave less variance reduction per unit time).
 constexpr int kDefaultGuidedBounces = 8;
+// Tabulated guide limits: cells along the longest side, entries in all
+// (2^24 entries of 40 floats = 2.7 GB).
+constexpr int kMaxTableCells = 1024;
+constexpr long long kMaxTableEntries = 1LL << 24;
 
 struct lambda_hip_progressive {
     lambda_hip_scene *scene = nullptr;
@@ -923,6 +932,12 @@ struct lambda_hip_progressive {
     bool fused = false;          // guided: network inside the persistent kernel
     double seen_per_sample = 0;  // training candidates per sample per pixel, last step
     DevBuf<double> scratch_fb;   // framebuffer of steps that are not kept
+    // tabulated guide (cells along the box's longest side; 0 = off)
+    int table_cells = 0;
+    long long table_entries = 0;
+    GuideTableDev table{};
+    DevBuf<float> table_rows, table_feat;
+    bool table_stale = true;     // the weights changed since the table was built
 };
 
 namespace {
@@ -1860,6 +1875,33 @@ int lambda_hip_serve(lambda_hip_scene *const *scenes, int32_t num_scenes, const 
     });
 }
 
+namespace {
+
+// Path slots and per-slot guide buffers of a session, for the kernel that
+// renders its steps (path_slots > 0: the caller's count).
+void size_session(lambda_hip_progressive &pr, int path_slots) {
+    const bool guided = pr.net != nullptr;
+    long long slots = path_slots > 0                  ? path_slots
+                      : guided && pr.table_cells > 0  ? resident_threads(persist_table_blocks_per_cu())
+                      : pr.fused                      ? resident_threads(persist_guided_blocks_per_cu())
+                      : guided                        ? (1LL << 20)
+                                                      : resident_persist_threads();
+    slots = std::max<long long>(1, std::min<long long>(slots, 1LL << 26));
+    pr.R.reset();
+    pr.R = std::make_unique<Renderer>(*pr.scene, make_frame(pr.params), int(slots));
+    if (guided) {
+        const size_t ns = size_t(slots);
+        pr.q_slot.alloc(ns);
+        pr.q_feat.alloc(ns * kGuideScalars);
+        pr.q_out.alloc(ns * kLobeOut);
+        pr.grec.alloc(ns * size_t(pr.params.depth + 1) * 3);
+        pr.Q = GuideQueue{pr.q_slot.p, pr.q_feat.p, pr.q_out.p};
+        pr.R->P.grec = pr.grec.p;
+    }
+}
+
+}  // namespace
+
 int lambda_hip_progressive_create(lambda_hip_scene *scene, const lambda_render_params *params, lambda_guide *guide,
                                   lambda_hip_progressive **out, char *err, size_t err_len) {
     return guarded(err, err_len, [&] {
@@ -1873,26 +1915,11 @@ int lambda_hip_progressive_create(lambda_hip_scene *scene, const lambda_render_p
         pr->scene = scene;
         pr->params = p;
         pr->net = guide ? &guide->net : nullptr;
-        const Frame F = make_frame(p);
         // guided: the network inside the persistent kernel unless the
         // caller asks for one launch per bounce (tail_paths -1)
         pr->fused = guide && params->tail_paths != -1;
-        long long slots = params->path_slots > 0 ? params->path_slots
-                          : pr->fused ? resident_threads(persist_guided_blocks_per_cu())
-                          : guide     ? (1LL << 20)
-                                      : resident_persist_threads();
-        slots = std::max<long long>(1, std::min<long long>(slots, 1LL << 26));
-        pr->R = std::make_unique<Renderer>(*scene, F, int(slots));
-        if (guide) {
-            const size_t ns = size_t(slots);
-            pr->q_slot.alloc(ns);
-            pr->q_feat.alloc(ns * kGuideScalars);
-            pr->q_out.alloc(ns * kLobeOut);
-            pr->grec.alloc(ns * size_t(p.depth + 1) * 3);
-            pr->train.alloc(size_t(pr->train_cap) * kGuideSample);
-            pr->Q = GuideQueue{pr->q_slot.p, pr->q_feat.p, pr->q_out.p};
-            pr->R->P.grec = pr->grec.p;
-        }
+        size_session(*pr, params->path_slots);
+        if (guide) pr->train.alloc(size_t(pr->train_cap) * kGuideSample);
         *out = pr.release();
     });
 }
@@ -1917,14 +1944,24 @@ int lambda_hip_progressive_step(lambda_hip_progressive *pr, int32_t samples, int
             if (!pr->scratch_fb.p) pr->scratch_fb.alloc(size_t(F.npix) * 3);
             R.P.fb = pr->scratch_fb.p;
         }
+        const bool use_table = pr->net && pr->table_cells > 0;
+        if (use_table && pr->table_stale) {
+            const float *b = pr->scene->box;
+            pr->net->tabulate(pr->table, make_float3(b[0], b[1], b[2]), make_float3(b[3], b[4], b[5]),
+                              pr->table_feat.p, pr->table_rows.p, pr->table_entries, R.st);
+            pr->table_stale = false;
+        }
         HIP_OK(hipEventRecord(ev.a, R.st));
         const bool clear = !keep || pr->total == 0;
-        if (pr->net && pr->fused)
-            R.run(F, kAutoTailPaths, nullptr, nullptr, clear, pr->net, &pr->Q, true);
+        lambda_render_stats rs{};
+        if (use_table)
+            R.run(F, kAutoTailPaths, nullptr, &rs, clear, nullptr, nullptr, false, &pr->table);
+        else if (pr->net && pr->fused)
+            R.run(F, kAutoTailPaths, nullptr, &rs, clear, pr->net, &pr->Q, true);
         else if (pr->net && pr->guide_bounces <= pr->params.depth)
-            R.run_guided_batches(F, nullptr, nullptr, clear, pr->net, &pr->Q);
+            R.run_guided_batches(F, nullptr, &rs, clear, pr->net, &pr->Q);
         else
-            R.run(F, pr->net ? 0 : kAutoTailPaths, nullptr, nullptr, clear, pr->net, pr->net ? &pr->Q : nullptr);
+            R.run(F, pr->net ? 0 : kAutoTailPaths, nullptr, &rs, clear, pr->net, pr->net ? &pr->Q : nullptr);
         HIP_OK(hipEventRecord(ev.b, R.st));
         R.P.fb = fb;
         HIP_OK(hipEventSynchronize(ev.b));
@@ -1939,6 +1976,7 @@ int lambda_hip_progressive_step(lambda_hip_progressive *pr, int32_t samples, int
             HIP_OK(hipMemsetAsync(net.loss.p, 0, sizeof(float), R.st));
             HIP_OK(hipEventRecord(ev.a, R.st));
             for (int e = 0; e < pr->epochs; e++) net.train(pr->train.p, int(collected), pr->batch, R.st);
+            pr->table_stale = true;
             HIP_OK(hipEventRecord(ev.b, R.st));
             HIP_OK(hipEventSynchronize(ev.b));
             HIP_OK(hipEventElapsedTime(&train_ms, ev.a, ev.b));
@@ -1955,6 +1993,8 @@ int lambda_hip_progressive_step(lambda_hip_progressive *pr, int32_t samples, int
             stats->render_ms = render_ms;
             stats->train_ms = train_ms;
             stats->loss = loss;
+            stats->closest_rays = rs.closest_rays;
+            stats->shadow_rays = rs.shadow_rays;
         }
     });
 }
@@ -1983,6 +2023,50 @@ int lambda_hip_progressive_set_training(lambda_hip_progressive *pr, int32_t epoc
         if (guided_bounces < 0) throw Failure{LAMBDA_EINVAL, "invalid training settings"};
         pr->guide_bounces = guided_bounces > 0 ? guided_bounces : kDefaultGuidedBounces;
         if (pr->net) pr->net->lr = lr;
+    });
+}
+
+int lambda_hip_progressive_set_guide_table(lambda_hip_progressive *pr, int32_t cells, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!pr) throw Failure{LAMBDA_EINVAL, "null argument"};
+        if (!pr->net) throw Failure{LAMBDA_EINVAL, "the session has no guide"};
+        if (cells < 0 || cells > kMaxTableCells) throw Failure{LAMBDA_EINVAL, "invalid table size"};
+        if (cells > 0 && !pr->fused) throw Failure{LAMBDA_EINVAL, "a table needs the persistent schedule"};
+        GuideTableDev T{};
+        long long entries = 0;
+        if (cells > 0) {
+            const float *b = pr->scene->box;
+            float ext[3], longest = 0;
+            for (int a = 0; a < 3; a++) {
+                ext[a] = std::max(0.0f, b[3 + a] - b[a]);
+                longest = std::max(longest, ext[a]);
+            }
+            const float cell = longest > 0 ? longest / float(cells) : 1.0f;
+            entries = kTableNormals;
+            for (int a = 0; a < 3; a++) {
+                T.lo[a] = b[a];
+                T.n[a] = std::max(1, int(std::ceil(ext[a] / cell)));
+                entries *= T.n[a];
+            }
+            if (entries > kMaxTableEntries) throw Failure{LAMBDA_EINVAL, "table too large"};
+            T.inv_cell = 1 / cell;
+            T.lambda = 0.5f * (pr->params.lambda_min + pr->params.lambda_max);
+        }
+        const bool resize = (cells > 0) != (pr->table_cells > 0);
+        if (resize && pr->steps > 0) throw Failure{LAMBDA_EINVAL, "switch to or from a table before the first step"};
+        pr->table_rows.release();
+        pr->table_feat.release();
+        pr->table_cells = cells;
+        pr->table_entries = entries;
+        if (cells > 0) {
+            pr->table_rows.alloc(size_t(entries) * kLobeOut);
+            pr->table_feat.alloc(size_t(entries) * kGuideScalars);
+            T.rows = pr->table_rows.p;
+        }
+        pr->table = T;
+        pr->table_stale = true;
+        // the table kernel holds more paths in flight than the network's
+        if (resize) size_session(*pr, pr->params.path_slots);
     });
 }
 

@@ -32,6 +32,9 @@ void launch_persist_occ(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Fr
 int persist_occ_blocks_per_cu(int occupancy);
 void launch_persist_guided(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, int n_in, int2 *spill, GuideDev G);
 int persist_guided_blocks_per_cu();
+void launch_persist_table(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, int n_in, int2 *spill,
+                          GuideTableDev T);
+int persist_table_blocks_per_cu();
 void launch_serve(dim3 grid, dim3 block, hipStream_t st, const DeviceScene *scenes, const ServeJob *jobs, int njobs, long long total_units, Paths P, ServeCtl *ctl, int *done, unsigned long long *finish, int2 *spill);
 void launch_query_closest(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, const float *rays, int n, lambda_hit *out, int2 *spill);
 // Device address of the no-allocation query scratch on the current device

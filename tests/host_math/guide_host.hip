@@ -8,6 +8,8 @@
 //       grad     <outputs.f32 [n][40]> <dirs.f32 [n][3]> <c.f32 [n]> <out.f32 [n][41] = loss, d[40]>
 //       sample   <outputs.f32 [40]> <count> <seed>   <out.f32 [count][3]>
 //       samplec  (as sample, through the compact form)
+//       table    <geometry.f32 [7] = lo xyz, inv_cell, nx, ny, nz> <points.f32 [n][6] = p xyz, normal xyz>
+//                <out.f32 [n][7] = entry, then the entry's centre xyz and normal xyz>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -72,7 +74,7 @@ int main(int argc, char **argv) {
                 guide_lobes(&o[r * kLobeOut], L);
                 out[r] = guide_pdf(L, dir);
             } else {
-                float c[kLobeOut];
+                alignas(16) float c[kLobeOut];
                 guide_compact(&o[r * kLobeOut], c);
                 out[r] = guide_pdf_c(c, dir);
             }
@@ -94,7 +96,7 @@ int main(int argc, char **argv) {
         Pcg rng = pcg_seed(uint64_t(std::atoll(argv[4])), 7);
         Lobe L[kLobes];
         guide_lobes(o.data(), L);
-        float c[kLobeOut];
+        alignas(16) float c[kLobeOut];
         guide_compact(o.data(), c);
         std::vector<float> out(size_t(count) * 3);
         for (int i = 0; i < count; i++) {
@@ -105,6 +107,31 @@ int main(int argc, char **argv) {
             out[3 * size_t(i) + 2] = d.z;
         }
         write_f32(argv[5], out);
+    } else if (mode == "table" && argc >= 5) {
+        std::vector<float> g = read_f32(argv[2]), pts = read_f32(argv[3]);
+        GuideTableDev T{};
+        for (int a = 0; a < 3; a++) {
+            T.lo[a] = g[size_t(a)];
+            T.n[a] = int(g[size_t(4 + a)]);
+        }
+        T.inv_cell = g[3];
+        const size_t n = pts.size() / 6;
+        std::vector<float> out(n * 7);
+        for (size_t i = 0; i < n; i++) {
+            const float *q = &pts[i * 6];
+            const long long e = guide_table_entry(T, v3(q[0], q[1], q[2]), v3(q[3], q[4], q[5]));
+            V3 c, nn;
+            guide_table_query(T, e, c, nn);
+            float *o = &out[i * 7];
+            o[0] = float(e);
+            o[1] = c.x;
+            o[2] = c.y;
+            o[3] = c.z;
+            o[4] = nn.x;
+            o[5] = nn.y;
+            o[6] = nn.z;
+        }
+        write_f32(argv[4], out);
     } else {
         return 2;
     }

@@ -40,12 +40,14 @@ def test_struct_layouts_match_header(tmp_path):
     """The ctypes mirrors agree with the C compiler's layout of the header."""
     src = tmp_path / "sizes.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "lambda_hip.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu", sizeof(lambda_hit), sizeof(lambda_render_params),'
-                   ' sizeof(lambda_scene_view), offsetof(lambda_scene_view, camera), offsetof(lambda_render_params, seed));'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu", sizeof(lambda_hit), sizeof(lambda_render_params),'
+                   ' sizeof(lambda_scene_view), offsetof(lambda_scene_view, camera), offsetof(lambda_render_params, seed),'
+                   ' sizeof(lambda_progressive_stats), offsetof(lambda_progressive_stats, shadow_rays));'
                    'return 0;}\n')
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [C.sizeof(abi.Hit), C.sizeof(abi.RenderParams), C.sizeof(abi.SceneView),
-            abi.SceneView.camera.offset, abi.RenderParams.seed.offset]
+            abi.SceneView.camera.offset, abi.RenderParams.seed.offset, C.sizeof(abi.ProgressiveStats),
+            abi.ProgressiveStats.shadow_rays.offset]
     assert got == want

@@ -366,3 +366,39 @@ def test_device_sampling_follows_pdf(driver, tmp_path, mode):
     sigma = np.sqrt(cell * (1 - cell) / count) + 1e-7
     z = np.abs(hist - cell) / sigma
     assert z.max() < 6, z.max()
+
+
+def test_table_entries_cells_and_normal_bins(driver, tmp_path):
+    """guide_table_entry / guide_table_query: a point's cell is the floor of
+    its offset over the cell size, clamped into the grid (NaN to cell 0);
+    the normal bin is 2 x the axis of the largest |component| (ties to the
+    lower axis) + 1 for a negative component; an entry's centre lies in its
+    cell, its normal is the bin's axis direction, and looking the centre up
+    with that normal gives the entry back."""
+    lo = np.array([-1.0, 0.5, 2.0], np.float32)
+    inv_cell, dims = np.float32(4.0), np.array([7, 3, 5])
+    geom = np.concatenate([lo, [inv_cell], dims]).astype(np.float32)
+    rng = np.random.default_rng(7)
+    m = 4000
+    p = lo + rng.uniform(-0.3, 1.3, (m, 3)).astype(np.float32) * dims / inv_cell
+    nrm = rng.normal(size=(m, 3)).astype(np.float32)
+    nrm[:8] = [[1, 1, 0], [0, -1, -1], [1, -1, 1], [0, 0, -2], [-3, 0, 0], [0, 0.5, 0], [1, 1, 1], [-1, -1, 0]]
+    p[8] = [np.nan, 0.7, 2.1]
+    got = run(driver, tmp_path, ["table"], [geom, np.concatenate([p, nrm], axis=1)], (m, 7))
+    with np.errstate(invalid="ignore"):
+        f = np.floor((p - lo) * inv_cell)
+    cell = np.where(np.isnan(f), 0, np.clip(f, 0, dims - 1)).astype(np.int64)
+    a = np.abs(nrm)
+    axis = np.argmax(a, axis=1)  # first maximum: ties to the lower axis
+    neg = nrm[np.arange(m), axis] < 0
+    want = ((cell[:, 2] * dims[1] + cell[:, 1]) * dims[0] + cell[:, 0]) * 6 + 2 * axis + neg
+    np.testing.assert_array_equal(got[:, 0].astype(np.int64), want)
+    assert list(got[:8, 0].astype(np.int64) % 6) == [0, 3, 0, 5, 1, 2, 0, 1]
+    # centres inside their cells, normals on their axes
+    centre, axis_dir = got[:, 1:4], got[:, 4:7]
+    np.testing.assert_array_equal(np.floor((centre - lo) * inv_cell).astype(np.int64), cell)
+    sign = np.where(neg, -1.0, 1.0)
+    np.testing.assert_array_equal(axis_dir, np.eye(3)[axis] * sign[:, None])
+    # round trip
+    back = run(driver, tmp_path, ["table"], [geom, got[:, 1:7]], (m, 7))
+    np.testing.assert_array_equal(back[:, 0], got[:, 0])

@@ -51,12 +51,14 @@ def test_reference_is_mostly_finite(reference):
     assert fmean(reference) > 0
 
 
-def step_means(room, guide, steps, spp, train, bounces=0, **overrides):
+def step_means(room, guide, steps, spp, train, bounces=0, table=0, **overrides):
     """Image mean of each step's own samples (from the running images) and
     the last step's statistics."""
     p = lambda_amd.Progressive(room, guide, path_slots=1 << 18, **overrides)
     if guide is not None:
         p.set_training(4, 4096, 1e-3, bounces)
+        if table:
+            p.set_guide_table(table)
     prev, means, st = None, [], None
     for k in range(1, steps + 1):
         st = p.step(spp, train=train)
@@ -83,6 +85,52 @@ def test_guided_estimator_is_unbiased(room):
         z = (m.mean() - plain.mean()) / se
         print("%s guide mean %.5g, plain %.5g, z = %.2f" % (name, m.mean(), plain.mean(), z), flush=True)
         assert abs(z) < 4
+
+
+@pytest.mark.parametrize("bounces", [0, 64])
+def test_table_guided_estimator_is_unbiased(room, bounces):
+    """The tabulated guide (48 cells along the room's longest side, rebuilt
+    after every training; the default 8 or every guided vertex): it trains,
+    and per-step image means agree with the plain tracer's within 4
+    standard errors."""
+    plain, _ = step_means(room, None, 16, 64, False)
+    g = lambda_amd.Guide(seed=3)
+    m, st = step_means(room, g, 16, 64, True, bounces, table=48)
+    g.close()
+    assert st["train_samples"] > 10000
+    se = np.sqrt(m.var(ddof=1) / len(m) + plain.var(ddof=1) / len(plain))
+    z = (m.mean() - plain.mean()) / se
+    print("table-guided (bounces %d) mean %.5g, plain %.5g, z = %.2f" % (bounces, m.mean(), plain.mean(), z),
+          flush=True)
+    assert abs(z) < 4
+
+
+def test_guide_table_settings_are_checked(room):
+    """A table needs a guided persistent session, a size within the limits,
+    and is chosen before the first step."""
+    p = lambda_amd.Progressive(room, None)
+    with pytest.raises(lambda_amd.LambdaError, match="no guide"):
+        p.set_guide_table(32)
+    p.close()
+    g = lambda_amd.Guide(seed=1)
+    p = lambda_amd.Progressive(room, g, tail_paths=-1)
+    with pytest.raises(lambda_amd.LambdaError, match="persistent"):
+        p.set_guide_table(32)
+    p.close()
+    p = lambda_amd.Progressive(room, g)
+    for bad in (-1, 1025):
+        with pytest.raises(lambda_amd.LambdaError, match="invalid table size"):
+            p.set_guide_table(bad)
+    p.set_guide_table(16)
+    p.set_guide_table(24)          # another size keeps the table schedule
+    st = p.step(16, train=True)
+    assert st["closest_rays"] > 0 and st["shadow_rays"] > 0
+    with pytest.raises(lambda_amd.LambdaError, match="before the first step"):
+        p.set_guide_table(0)
+    p.set_guide_table(32)          # resizing the table itself is allowed
+    p.step(16, train=False)
+    p.close()
+    g.close()
 
 
 @pytest.mark.parametrize("bounces", [4, 64])
@@ -137,7 +185,9 @@ def test_trained_guide_samples_toward_the_gap(room):
 def test_trained_guide_against_plain_at_equal_samples(tmp_path_factory):
     """320x180, 1024 spp, 3 independent sessions each against a 16384-spp
     plain reference: the guide, trained for 48 steps and then frozen, lowers
-    the relMSE at equal samples by at least 1.2x (measured 1.5-1.8x)."""
+    the relMSE at equal samples by at least 1.2x (measured 1.5-1.8x), both
+    evaluated in the kernel (8 guided vertices) and tabulated (32 cells,
+    every vertex guided)."""
     d = tmp_path_factory.mktemp("room320")
     dev = lambda_amd.DeviceScene(lambda_amd.Scene.load(
         gen.write_guiding_room(str(d), width=320, height=180, depth=16, samples=1)))
@@ -147,10 +197,12 @@ def test_trained_guide_against_plain_at_equal_samples(tmp_path_factory):
     R = ref.image()
     ref.close()
 
-    def err(guide, seed):
+    def err(guide, seed, table=0):
         p = lambda_amd.Progressive(dev, guide, seed=seed)
         if guide is not None:
-            p.set_training(4, 4096, 1e-3)
+            p.set_training(4, 4096, 1e-3, 17 if table else 0)
+            if table:
+                p.set_guide_table(table)
             for _ in range(48):
                 p.step(64, train=True, keep=False)
         for _ in range(16):
@@ -160,13 +212,19 @@ def test_trained_guide_against_plain_at_equal_samples(tmp_path_factory):
         return e
 
     plain = [err(None, 1000 + k) for k in range(3)]
-    guided = []
+    guided, tabled = [], []
     for k in range(3):
         g = lambda_amd.Guide(seed=1 + k)
         guided.append(err(g, 2000 + k))
         g.close()
+        g = lambda_amd.Guide(seed=1 + k)
+        tabled.append(err(g, 3000 + k, table=32))
+        g.close()
     dev.close()
-    print("1024 spp relMSE: plain %s (mean %.4g), guided %s (mean %.4g), plain/guided %.2f" % (
-        " ".join("%.3g" % e for e in plain), np.mean(plain), " ".join("%.3g" % e for e in guided),
-        np.mean(guided), np.mean(plain) / np.mean(guided)), flush=True)
+    print("1024 spp relMSE: plain %s (mean %.4g), guided %s (mean %.4g), plain/guided %.2f, "
+          "table-guided %s (mean %.4g), plain/table-guided %.2f" % (
+              " ".join("%.3g" % e for e in plain), np.mean(plain), " ".join("%.3g" % e for e in guided),
+              np.mean(guided), np.mean(plain) / np.mean(guided), " ".join("%.3g" % e for e in tabled),
+              np.mean(tabled), np.mean(plain) / np.mean(tabled)), flush=True)
     assert np.mean(plain) / np.mean(guided) >= 1.2
+    assert np.mean(plain) / np.mean(tabled) >= 1.2
