@@ -22,7 +22,9 @@ Printed JSON line (rank 0, one GPU):
   throughput_streams  the same frame with independent streams per sample
   guided_vs_plain     wall time to relMSE 1e-3 against a 65536-spp plain
                reference on the room lit through a door gap (320x180, depth
-               16): the plain tracer, and the median of 3 guided runs
+               16): the plain tracer, and the median of 3 guided runs (the
+               guide's mixtures tabulated over 48 cells along the room's
+               longest side, one-sample mixture 0.5 cos + 0.5 guide)
   guide_mlp_roofline  the guiding network's forward and gradient kernels,
                algorithmic FLOP per second against 2.5 PFLOP/s dense BF16
   batched_serving     32 seeded jobs with Poisson arrivals in persistent
@@ -112,7 +114,7 @@ def cpu_model() -> str:
 
 def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, warmup: int = 0,
                    train_steps: int = 1 << 30, epochs: int = 1, batch: int = 1 << 14, guided_bounces: int = 0,
-                   table_cells: int = 0):
+                   table_cells: int = 0, mix: float = 0.5):
     """Progressive render until relMSE against ref <= target; returns the
     render-and-train wall time at which the target was crossed (log-linear
     interpolation between steps), the samples per pixel then, and the curve.
@@ -125,6 +127,7 @@ def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, 
         p.set_training(epochs, batch, 1e-3, guided_bounces)
         if table_cells:
             p.set_guide_table(table_cells)
+        p.set_guide_mix(mix)
     elapsed, spp, curve = 0.0, 0, []
     prev = None
     hit = None
@@ -187,7 +190,8 @@ def guided_leg(args, scene_dir: str) -> dict:
         hit, curve = time_to_target(dev, guide, ref, 1e-3, args.guided_step, args.guided_max_spp,
                                     warmup=args.guided_warmup, train_steps=args.guided_train_steps,
                                     epochs=args.guided_epochs, batch=args.guided_batch,
-                                    guided_bounces=args.guided_bounces)
+                                    guided_bounces=args.guided_bounces, table_cells=args.guided_table_cells,
+                                    mix=args.guided_mix)
         guide.close()
         log("guided leg: guided run %d reaches relMSE 1e-3 at %s" % (k, hit))
         runs.append((hit, curve))
@@ -199,8 +203,11 @@ def guided_leg(args, scene_dir: str) -> dict:
            "reference_spp": args.guided_ref_spp, "target_rel_mse": 1e-3, "step_spp": args.guided_step,
            "guided_warmup_steps": args.guided_warmup, "guided_train_steps": min(args.guided_train_steps, 1 << 30),
            "guided_epochs": args.guided_epochs, "guided_batch": args.guided_batch, "adam_lr": 1e-3,
-           "guided_bounces": args.guided_bounces or 8,
-           "guided_schedule": "persistent kernel, network evaluated per wave on the matrix cores",
+           "guided_bounces": args.guided_bounces or 8, "guide_fraction": args.guided_mix,
+           "guided_schedule": ("persistent kernel, mixtures looked up in a table of the network's outputs "
+                               "(%d cells along the longest side x 6 axis normals), rebuilt after each training"
+                               % args.guided_table_cells) if args.guided_table_cells else
+                              "persistent kernel, network evaluated per wave on the matrix cores",
            "plain_s": plain[0] if plain else None, "plain_spp": plain[1] if plain else None,
            "guided_runs": len(runs), "guided_s_runs": [r[0][0] if r[0] else None for r in runs],
            "guided_spp_runs": [r[0][1] if r[0] else None for r in runs],
@@ -409,6 +416,11 @@ def main() -> int:
     ap.add_argument("--guided-batch", type=int, default=4096)
     ap.add_argument("--guided-bounces", type=int, default=0,
                     help="vertices guided per path (bounce index below this; 0 = the library default, 8)")
+    ap.add_argument("--guided-table-cells", type=int, default=48,
+                    help="guide from a table of the network's mixtures with this many cells along the scene's "
+                         "longest side (0: the network inside the persistent kernel)")
+    ap.add_argument("--guided-mix", type=float, default=0.5,
+                    help="probability of sampling the guide at a guided vertex (the cosine lobe takes the rest)")
     ap.add_argument("--throughput-steps", type=int, default=2,
                     help="also time this many renders with throughput streams (0 to skip)")
     ap.add_argument("--stats-pixels", type=int, default=20000, help="pixels sampled to count traversal work")

@@ -531,6 +531,10 @@ int lambda_hip_progressive_set_training(lambda_hip_progressive *session, int32_t
  * returns to the network inside the kernel; at most 1024 cells and 2^24
  * entries (cells x 6), 160 bytes of device memory each. */
 int lambda_hip_progressive_set_guide_table(lambda_hip_progressive *session, int32_t cells, char *err, size_t err_len);
+/* Probability of sampling the guide at a guided vertex (default 0.5, the
+ * one-sample mixture 0.5 cos/pi + 0.5 q); frac in (0, 0.95], the cosine
+ * lobe takes the rest and both MIS densities use the same mixture. */
+int lambda_hip_progressive_set_guide_mix(lambda_hip_progressive *session, float frac, char *err, size_t err_len);
 void lambda_hip_progressive_destroy(lambda_hip_progressive *session);
 
 /* Device name and compute-unit count of the current HIP device. */

@@ -390,6 +390,12 @@ class Progressive:
         err = C.create_string_buffer(4096)
         _check(self._lib.lambda_hip_progressive_set_guide_table(self._h, cells, err, 4096), err)
 
+    def set_guide_mix(self, frac: float) -> None:
+        """Probability of sampling the guide at a guided vertex (default
+        0.5; at most 0.95, the cosine lobe takes the rest)."""
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_progressive_set_guide_mix(self._h, frac, err, 4096), err)
+
     def image(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 3), np.float32)
         err = C.create_string_buffer(4096)

@@ -306,13 +306,16 @@ void GuideNet::adam(float scale, hipStream_t st) {
 }
 
 // Minibatch Adam over samples [0, n): steps of `batch` samples, weights
-// capped at kWeightCap x their mean over [0, n).
-void GuideNet::train(const float *samples, int n, int batch, hipStream_t st) {
+// capped at cap_factor x their mean over [0, n) when cap_factor > 0.
+void GuideNet::train(const float *samples, int n, int batch, hipStream_t st, float cap_factor) {
     if (n <= 0) return;
-    k_weight_cap<<<1, 1024, 0, st>>>(samples, n, kWeightCap, cap.p);
-    HIP_OK(hipGetLastError());
+    const bool capped = cap_factor > 0;
+    if (capped) {
+        k_weight_cap<<<1, 1024, 0, st>>>(samples, n, cap_factor, cap.p);
+        HIP_OK(hipGetLastError());
+    }
     for (int b0 = 0; b0 + batch <= n; b0 += batch) {
-        gradient(samples + size_t(b0) * kGuideSample, batch, st, cap.p);
+        gradient(samples + size_t(b0) * kGuideSample, batch, st, capped ? cap.p : nullptr);
         adam(1.0f / float(batch), st);
     }
 }

@@ -87,6 +87,7 @@ struct Frame {
     float train_keep;         // probability of keeping a training candidate
     int grec_stride;          // guided vertices per slot (depth + 1)
     int guide_bounces;        // vertices with bounce index below this are guided
+    float guide_frac;         // probability of sampling the guide at a guided vertex
     float3 box_lo, box_hi;    // scene box for the guiding features
 };
 
@@ -219,9 +220,13 @@ struct ShadowRay {
 // -2 = no record, -1 = awaiting the next vertex.
 constexpr float kRecNone = -2.0f, kRecPending = -1.0f;
 
-// Probability of sampling the guide at a guided vertex; the cosine lobe
-// takes the rest, and both MIS densities use this one-sample mixture.
+// Default probability of sampling the guide at a guided vertex (Frame::
+// guide_frac); the cosine lobe takes the rest, and both MIS densities use
+// this one-sample mixture.
 constexpr float kGuideFrac = 0.5f;
+// Largest accepted guide fraction: the cosine lobe keeps at least 5 % so that
+// directions where every vMF lobe's density underflows are still sampled.
+constexpr float kGuideFracMax = 0.95f;
 
 // One iteration body of Renderer::trace (render/renderer.h:102-192) for the
 // path in slot s after its closest-hit query (P.hit[s]).  Updates the slot's
@@ -229,7 +234,7 @@ constexpr float kGuideFrac = 0.5f;
 // whether next-event estimation produced an occlusion ray.
 //
 // kGuided with gout set: a Lambertian vertex samples the next direction from
-// the one-sample mixture 0.5 cos/pi + 0.5 q (q: the guiding distribution in
+// the one-sample mixture (1 - a) cos/pi + a q (a = F.guide_frac, q: the guiding distribution in
 // the compact form of guide_compact, features gfeat); the mixture density is the
 // scatter pdf of both MIS weights (next-event estimation and emitter hits),
 // so the estimator stays unbiased.  The vertex is recorded for training in
@@ -344,7 +349,7 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                                 if (dot(h.n, ldir) > 0) att = albedo / kPi;
                                 float sp = ct > 0 ? ct / kPi : 0;
                                 if constexpr (kGuided) {
-                                    if (guided) sp = (1 - kGuideFrac) * sp + kGuideFrac * guide_pdf_c(gout, ldir);
+                                    if (guided) sp = (1 - F.guide_frac) * sp + F.guide_frac * guide_pdf_c(gout, ldir);
                                 }
                                 float w = power_heuristic(lprob, sp);
                                 S4 c = beta * att * ct * em * w / lprob;
@@ -362,7 +367,7 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     if (lambertian) {
                         bool from_guide = false;
                         if constexpr (kGuided) {
-                            if (guided && rng.next() >= 1 - kGuideFrac) {
+                            if (guided && rng.next() >= 1 - F.guide_frac) {
                                 float u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
                                 nd = guide_sample_c(gout, u0, u1, u2);
                                 from_guide = true;
@@ -425,7 +430,7 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                     prev_pdf = lambertian ? (ct > 0 ? ct / kPi : 0) : 0;
                     if constexpr (kGuided) {
                         if (guided) {
-                            prev_pdf = (1 - kGuideFrac) * prev_pdf + kGuideFrac * guide_pdf_c(gout, d);
+                            prev_pdf = (1 - F.guide_frac) * prev_pdf + F.guide_frac * guide_pdf_c(gout, d);
                             // below the surface the contribution is zero
                             if (!(ct > 0) || !(prev_pdf > 0)) finished = true;
                         }

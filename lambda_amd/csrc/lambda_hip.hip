@@ -435,6 +435,7 @@ Frame make_frame(const lambda_render_params &p) {
     F.seed = p.seed;
     F.total_work = (long long)F.npix * F.spp;
     F.cost = nullptr;
+    F.guide_frac = kGuideFrac;
     return F;
 }
 
@@ -929,6 +930,7 @@ struct lambda_hip_progressive {
     int batch = 1 << 14;
     int epochs = 1;
     int guide_bounces = kDefaultGuidedBounces;
+    float guide_frac = kGuideFrac;  // probability of sampling the guide
     bool fused = false;          // guided: network inside the persistent kernel
     double seen_per_sample = 0;  // training candidates per sample per pixel, last step
     DevBuf<double> scratch_fb;   // framebuffer of steps that are not kept
@@ -953,7 +955,10 @@ Frame progressive_frame(const lambda_hip_progressive &pr, int samples, int root,
     F.seed = pr.params.seed + 0x9e3779b97f4a7c15ull * uint64_t(pr.steps + 1);
     F.box_lo = make_float3(pr.scene->box[0], pr.scene->box[1], pr.scene->box[2]);
     F.box_hi = make_float3(pr.scene->box[3], pr.scene->box[4], pr.scene->box[5]);
-    if (pr.net) F.guide_bounces = pr.guide_bounces;
+    if (pr.net) {
+        F.guide_bounces = pr.guide_bounces;
+        F.guide_frac = pr.guide_frac;
+    }
     if (pr.net && train) {
         F.train = pr.train.p;
         F.train_cap = pr.train_cap;
@@ -2067,6 +2072,15 @@ int lambda_hip_progressive_set_guide_table(lambda_hip_progressive *pr, int32_t c
         pr->table_stale = true;
         // the table kernel holds more paths in flight than the network's
         if (resize) size_session(*pr, pr->params.path_slots);
+    });
+}
+
+int lambda_hip_progressive_set_guide_mix(lambda_hip_progressive *pr, float frac, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!pr) throw Failure{LAMBDA_EINVAL, "null argument"};
+        if (!pr->net) throw Failure{LAMBDA_EINVAL, "the session has no guide"};
+        if (!(frac > 0 && frac <= kGuideFracMax)) throw Failure{LAMBDA_EINVAL, "invalid guide fraction"};
+        pr->guide_frac = frac;
     });
 }
 

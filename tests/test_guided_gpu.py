@@ -51,7 +51,7 @@ def test_reference_is_mostly_finite(reference):
     assert fmean(reference) > 0
 
 
-def step_means(room, guide, steps, spp, train, bounces=0, table=0, **overrides):
+def step_means(room, guide, steps, spp, train, bounces=0, table=0, mix=None, **overrides):
     """Image mean of each step's own samples (from the running images) and
     the last step's statistics."""
     p = lambda_amd.Progressive(room, guide, path_slots=1 << 18, **overrides)
@@ -59,6 +59,8 @@ def step_means(room, guide, steps, spp, train, bounces=0, table=0, **overrides):
         p.set_training(4, 4096, 1e-3, bounces)
         if table:
             p.set_guide_table(table)
+        if mix is not None:
+            p.set_guide_mix(mix)
     prev, means, st = None, [], None
     for k in range(1, steps + 1):
         st = p.step(spp, train=train)
@@ -103,6 +105,39 @@ def test_table_guided_estimator_is_unbiased(room, bounces):
     print("table-guided (bounces %d) mean %.5g, plain %.5g, z = %.2f" % (bounces, m.mean(), plain.mean(), z),
           flush=True)
     assert abs(z) < 4
+
+
+@pytest.mark.parametrize("mix", [0.2, 0.9])
+def test_guide_mix_estimator_is_unbiased(room, mix):
+    """Other guide fractions of the one-sample mixture (table-guided,
+    trained): per-step image means agree with the plain tracer's within 4
+    standard errors."""
+    plain, _ = step_means(room, None, 16, 64, False)
+    g = lambda_amd.Guide(seed=4)
+    m, st = step_means(room, g, 16, 64, True, table=48, mix=mix)
+    g.close()
+    assert st["train_samples"] > 10000
+    se = np.sqrt(m.var(ddof=1) / len(m) + plain.var(ddof=1) / len(plain))
+    z = (m.mean() - plain.mean()) / se
+    print("guide fraction %.2f: mean %.5g, plain %.5g, z = %.2f" % (mix, m.mean(), plain.mean(), z), flush=True)
+    assert abs(z) < 4
+
+
+def test_guide_mix_settings_are_checked(room):
+    """The guide fraction needs a guided session and lies in (0, 0.95]."""
+    p = lambda_amd.Progressive(room, None)
+    with pytest.raises(lambda_amd.LambdaError, match="no guide"):
+        p.set_guide_mix(0.5)
+    p.close()
+    g = lambda_amd.Guide(seed=1)
+    p = lambda_amd.Progressive(room, g)
+    for bad in (0.0, -0.5, 0.96, 1.0, float("nan")):
+        with pytest.raises(lambda_amd.LambdaError, match="invalid guide fraction"):
+            p.set_guide_mix(bad)
+    p.set_guide_mix(0.95)
+    p.set_guide_mix(0.5)
+    p.close()
+    g.close()
 
 
 def test_guide_table_settings_are_checked(room):
