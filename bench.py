@@ -137,9 +137,12 @@ def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, 
             p.step(step_spp, train=True, keep=False)
             elapsed += time.perf_counter() - t0
     kept = 0
+    train_ms = []
     while spp < max_spp:
         t0 = time.perf_counter()
-        p.step(step_spp, train=guide is not None and kept < train_steps)
+        st = p.step(step_spp, train=guide is not None and kept < train_steps)
+        if st["train_ms"] > 0:
+            train_ms.append(st["train_ms"])
         elapsed += time.perf_counter() - t0
         kept += 1
         spp += step_spp
@@ -156,6 +159,8 @@ def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, 
             break
         prev = (elapsed, spp, err)
     p.close()
+    if train_ms:
+        curve.append(("train_ms_per_training_step", sum(train_ms) / len(train_ms), len(train_ms)))
     return hit, curve
 
 
@@ -199,6 +204,9 @@ def guided_leg(args, scene_dir: str) -> dict:
     done = sorted((r for r in runs if r[0]), key=lambda r: r[0][0])
     # the median run; none when a run did not reach the target
     guided, guided_curve = done[(len(done) - 1) // 2] if len(done) == len(runs) else (None, runs[0][1])
+    train_ms = [c[-1][1] for _, c in runs if c and c[-1][0] == "train_ms_per_training_step"]
+    curves = [[x for x in c if x[0] != "train_ms_per_training_step"] for c in (plain_curve, guided_curve)]
+    plain_curve, guided_curve = curves
     out = {"workload": "guiding_room", "width": w, "height": h, "depth": 16,
            "reference_spp": args.guided_ref_spp, "target_rel_mse": 1e-3, "step_spp": args.guided_step,
            "guided_warmup_steps": args.guided_warmup, "guided_train_steps": min(args.guided_train_steps, 1 << 30),
@@ -212,6 +220,7 @@ def guided_leg(args, scene_dir: str) -> dict:
            "guided_runs": len(runs), "guided_s_runs": [r[0][0] if r[0] else None for r in runs],
            "guided_spp_runs": [r[0][1] if r[0] else None for r in runs],
            "guided_s": guided[0] if guided else None, "guided_spp": guided[1] if guided else None,
+           "train_ms_per_training_step": sum(train_ms) / len(train_ms) if train_ms else None,
            "plain_curve": plain_curve[::max(1, len(plain_curve) // 12)],
            "guided_curve": guided_curve[::max(1, len(guided_curve) // 12)]}
     if plain and guided:
@@ -412,8 +421,8 @@ def main() -> int:
     ap.add_argument("--guided-warmup", type=int, default=0, help="guided steps that train without adding samples")
     ap.add_argument("--guided-train-steps", type=int, default=48,
                     help="first kept guided steps that also train (their samples are unbiased and count)")
-    ap.add_argument("--guided-epochs", type=int, default=4, help="Adam epochs over each step's samples")
-    ap.add_argument("--guided-batch", type=int, default=4096)
+    ap.add_argument("--guided-epochs", type=int, default=2, help="Adam epochs over each step's samples")
+    ap.add_argument("--guided-batch", type=int, default=16384)
     ap.add_argument("--guided-bounces", type=int, default=0,
                     help="vertices guided per path (bounce index below this; 0 = the library default, 8)")
     ap.add_argument("--guided-table-cells", type=int, default=48,

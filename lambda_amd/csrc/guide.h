@@ -273,40 +273,68 @@ LAM_FN void guide_table_query(const GuideTableDev &T, long long e, V3 &p, V3 &n)
     n = guide_table_normal(bin);
 }
 
-// Loss of one training sample, -c log q(w), and its gradient with respect to
-// the 40 raw outputs (d[40]); returns the loss.
+// Loss of one training sample, -c log q(w), and its gradient, split by lobe
+// so that the lobes of one row can be spread over lanes: guide_lobe_density
+// gives lobe k's vMF density at w (with its mean and concentration), the
+// caller forms the softmax weights pi_k and q = sum_k pi_k vMF_k(w), and
+// guide_lobe_grad gives the gradient with respect to lobe k's 5 raw outputs.
+struct LobeEval {
+    V3 mu;
+    float kappa, comp;  // concentration, vMF_k(w)
+};
+
+LAM_FN LobeEval guide_lobe_density(const float *r, V3 w) {
+    LobeEval e;
+    V3 m = v3(r[0], r[1], r[2]);
+    float l = fmaxf(len(m), 1e-6f);
+    e.mu = m * (1 / l);
+    e.kappa = fminf(fmaxf(expf(r[3]), kKappaMin), kKappaMax);
+    e.comp = vmf_norm(e.kappa) * expf(e.kappa * (dot(e.mu, w) - 1));
+    return e;
+}
+
+// r: lobe k's raw outputs; pi = its softmax weight; q = the mixture density
+// at w (at least 1e-30).
+LAM_FN void guide_lobe_grad(const float *r, V3 w, float c, const LobeEval &e, float pi, float q, float *g) {
+    const float resp = pi * e.comp / q;  // posterior of lobe k
+    // d(-c log q)/d logit_k = -c (resp_k - pi_k)
+    g[4] = -c * (resp - pi);
+    // d log vMF / d kappa = 1/kappa - 2 exp(-2k)/(1 - exp(-2k)) + (mu.w - 1)
+    const float kap = e.kappa;
+    const float ex = expf(-2 * kap);
+    const float dlog_dk = 1 / kap - 2 * ex / (-expm1f(-2 * kap)) + (dot(e.mu, w) - 1);
+    const float ek = expf(r[3]);
+    const float dk_draw = (ek > kKappaMin && ek < kKappaMax) ? ek : 0.0f;
+    g[3] = -c * resp * dlog_dk * dk_draw;
+    // d log vMF / d mu = kappa w; mu = m / |m|: d/dm = (I - mu mu^T) / |m|
+    V3 m = v3(r[0], r[1], r[2]);
+    const float l = fmaxf(len(m), 1e-6f);
+    V3 gm = w * kap;
+    gm = (gm - e.mu * dot(e.mu, gm)) * (1 / l);
+    g[0] = -c * resp * gm.x;
+    g[1] = -c * resp * gm.y;
+    g[2] = -c * resp * gm.z;
+}
+
+// The whole row: loss and gradient with respect to the 40 raw outputs
+// (d[40]); returns the loss.
 LAM_FN float guide_loss_grad(const float *o, V3 w, float c, float *d) {
-    Lobe L[kLobes];
-    guide_lobes(o, L);
-    float comp[kLobes];
+    LobeEval e[kLobes];
+    float mx = -INFINITY;
+    for (int k = 0; k < kLobes; k++) mx = fmaxf(mx, o[5 * k + 4]);
+    float pi[kLobes], sum = 0;
+    for (int k = 0; k < kLobes; k++) {
+        e[k] = guide_lobe_density(o + 5 * k, w);
+        pi[k] = expf(o[5 * k + 4] - mx);
+        sum += pi[k];
+    }
     float q = 0;
     for (int k = 0; k < kLobes; k++) {
-        comp[k] = L[k].norm * expf(L[k].kappa * (dot(L[k].mu, w) - 1));
-        q += L[k].weight * comp[k];
+        pi[k] /= sum;
+        q += pi[k] * e[k].comp;
     }
     q = fmaxf(q, 1e-30f);
-    for (int k = 0; k < kLobes; k++) {
-        const float *r = o + 5 * k;
-        float *g = d + 5 * k;
-        const float resp = L[k].weight * comp[k] / q;  // posterior of lobe k
-        // d(-c log q)/d logit_k = -c (resp_k - pi_k)
-        g[4] = -c * (resp - L[k].weight);
-        // d log vMF / d kappa = 1/kappa - 2 exp(-2k)/(1 - exp(-2k)) + (mu.w - 1)
-        const float kap = L[k].kappa;
-        const float e = expf(-2 * kap);
-        const float dlog_dk = 1 / kap - 2 * e / (-expm1f(-2 * kap)) + (dot(L[k].mu, w) - 1);
-        const float ek = expf(r[3]);
-        const float dk_draw = (ek > kKappaMin && ek < kKappaMax) ? ek : 0.0f;
-        g[3] = -c * resp * dlog_dk * dk_draw;
-        // d log vMF / d mu = kappa w; mu = m / |m|: d/dm = (I - mu mu^T) / |m|
-        V3 m = v3(r[0], r[1], r[2]);
-        const float l = fmaxf(len(m), 1e-6f);
-        V3 gm = w * kap;
-        gm = (gm - L[k].mu * dot(L[k].mu, gm)) * (1 / l);
-        g[0] = -c * resp * gm.x;
-        g[1] = -c * resp * gm.y;
-        g[2] = -c * resp * gm.z;
-    }
+    for (int k = 0; k < kLobes; k++) guide_lobe_grad(o + 5 * k, w, c, e[k], pi[k], q, d + 5 * k);
     return -c * logf(q);
 }
 

@@ -96,7 +96,8 @@ __device__ __forceinline__ void encode_rows(const float *feat, int fstride, int 
 }
 
 // One layer for a wave's 16 rows: H_out = act(H_in W_l + b_l).  relu: hidden
-// layers; otherwise the fp32 result goes to out (16 x 64, stride 64).
+// layers; otherwise the fp32 result's first kLobeOut columns go to out
+// (16 x kLobeOut, stride kLobeOut).
 __device__ __forceinline__ void layer_rows16(const GuideDev &G, int layer, const __bf16 *Hin, int ldin,
                                              __bf16 *Hout, float *out) {
     const int l = threadIdx.x & 63;
@@ -115,7 +116,7 @@ __device__ __forceinline__ void layer_rows16(const GuideDev &G, int layer, const
             const int row = 4 * (l >> 4) + rr;
             const float v = acc[nt][rr] + bc;
             if (Hout) Hout[row * kLdH + c] = __bf16(fmaxf(v, 0.0f));
-            else out[row * 64 + c] = v;
+            else if (c < kLobeOut) out[row * kLobeOut + c] = v;
         }
     }
 }

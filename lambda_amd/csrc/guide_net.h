@@ -30,9 +30,10 @@ struct GuideNet {
     // at *c_cap when given).
     void gradient(const float *samples, int n, hipStream_t st, const float *c_cap = nullptr);
     void adam(float scale, hipStream_t st);
-    // Minibatch Adam over samples [0, n): steps of `batch` samples, weights
-    // capped at cap_factor x their mean over [0, n) when cap_factor > 0.
-    void train(const float *samples, int n, int batch, hipStream_t st, float cap_factor = kWeightCap);
+    // `epochs` passes of minibatch Adam over samples [0, n): steps of `batch`
+    // samples, weights capped at cap_factor x their mean over [0, n) when
+    // cap_factor > 0.
+    void train(const float *samples, int n, int batch, hipStream_t st, float cap_factor = kWeightCap, int epochs = 1);
     // The table's entries (compact mixtures) from the current weights; feat
     // is scratch of entries x kGuideScalars floats.
     void tabulate(const GuideTableDev &T, float3 box_lo, float3 box_hi, float *feat, float *rows, long long entries,

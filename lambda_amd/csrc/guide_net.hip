@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(kFwdThreads, 3) k_guide_forward(GuideDev G, co
 // Gradient of sum_i -c_i log q(w_i) over samples [0, n) into grads (fp32,
 // parameter layout, accumulated with atomics) and the loss into *loss.
 // Each block takes 64 rows: forward with stored activations, the loss
-// gradient per row, then backward through the five layers.  Weight
+// gradient (four lanes per row), then backward through the five layers.  Weight
 // gradients dW_l = H_{l-1}^T dZ_l sum over the block's rows on the MFMA
 // (16 x 16 tiles shared among the block's waves).
 __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float *samples, int n, float *grads,
@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
     __shared__ __attribute__((aligned(16))) __bf16 sX[kGuideRowsPerBlock * kLdX];
     __shared__ __attribute__((aligned(16))) __bf16 sH[4][kGuideRowsPerBlock * kLdH];   // H1..H4
     __shared__ __attribute__((aligned(16))) __bf16 sD[2][kGuideRowsPerBlock * kLdH];   // dZ ping-pong
-    __shared__ float sO[kGuideRowsPerBlock * 64];
+    __shared__ __attribute__((aligned(16))) float sO[kGuideRowsPerBlock * kLobeOut];  // raw outputs (40 used)
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int r16 = l & 15, q = l >> 4;
     const int row0 = blockIdx.x * kGuideRowsPerBlock;
@@ -63,26 +63,42 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
     __syncthreads();
     layer_rows16(G, 3, sH[2] + wr * kLdH, kLdH, sH[3] + wr * kLdH, nullptr);
     __syncthreads();
-    layer_rows16(G, 4, sH[3] + wr * kLdH, kLdH, nullptr, sO + wr * 64);
+    layer_rows16(G, 4, sH[3] + wr * kLdH, kLdH, nullptr, sO + wr * kLobeOut);
     __syncthreads();
 
-    // loss gradient per row (one lane per row), dZ_4 in bf16, padding zero
-    float lsum = 0;
-    if (threadIdx.x < kGuideRowsPerBlock) {
-        const int r = threadIdx.x;
-        float d[kLobeOut];
+    // loss gradient: four lanes per row, two lobes each (the softmax and the
+    // mixture density reduced over the four lanes; every thread of the
+    // block takes part); dZ_4 in bf16, padding columns and rows >= n zero
+    {
+        const int r = threadIdx.x >> 2, part = threadIdx.x & 3;
+        const bool valid = row0 + r < n;
+        const float *sm = samples + size_t(valid ? row0 + r : 0) * kGuideSample;
+        const V3 wd = v3(sm[6], sm[7], sm[8]);
+        const float c = c_cap ? fminf(sm[9], *c_cap) : sm[9];
+        const float *o0 = sO + r * kLobeOut + 10 * part, *o1 = o0 + 5;
+        float mx = fmaxf(o0[4], o1[4]);
+        mx = fmaxf(mx, __shfl_xor(mx, 1));
+        mx = fmaxf(mx, __shfl_xor(mx, 2));
+        const LobeEval e0 = guide_lobe_density(o0, wd), e1 = guide_lobe_density(o1, wd);
+        float p0 = expf(o0[4] - mx), p1 = expf(o1[4] - mx);
+        float sum = p0 + p1;
+        sum += __shfl_xor(sum, 1);
+        sum += __shfl_xor(sum, 2);
+        p0 /= sum;
+        p1 /= sum;
+        float q = p0 * e0.comp + p1 * e1.comp;
+        q += __shfl_xor(q, 1);
+        q += __shfl_xor(q, 2);
+        q = fmaxf(q, 1e-30f);
+        float g[10];
+        guide_lobe_grad(o0, wd, c, e0, p0, q, g);
+        guide_lobe_grad(o1, wd, c, e1, p1, q, g + 5);
+        __bf16 *dz = sD[0] + r * kLdH;
 #pragma unroll
-        for (int c = 0; c < kLobeOut; c++) d[c] = 0;
-        if (row0 + r < n) {
-            const float *sm = samples + size_t(row0 + r) * kGuideSample;
-            const float c = c_cap ? fminf(sm[9], *c_cap) : sm[9];
-            lsum = guide_loss_grad(sO + r * 64, v3(sm[6], sm[7], sm[8]), c, d);
-        }
+        for (int j = 0; j < 10; j++) dz[10 * part + j] = __bf16(valid ? g[j] : 0.0f);
 #pragma unroll
-        for (int c = 0; c < 64; c++) sD[0][r * kLdH + c] = __bf16(c < kLobeOut ? d[c] : 0.0f);
-    }
-    // wave 0 holds all 64 rows' losses
-    if (w == 0) {
+        for (int j = 0; j < 6; j++) dz[kLobeOut + 6 * part + j] = __bf16(0.0f);
+        float lsum = valid && part == 0 ? -c * logf(q) : 0.0f;
         for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
         if (l == 0) atomicAdd(loss, lsum);
     }
@@ -187,19 +203,22 @@ __global__ void k_guide_pack(const float *params, __bf16 *w_nk, __bf16 *w_kn) {
 }
 
 
-// *cap = factor x the mean weight of samples [0, n) (one block).
-__global__ void __launch_bounds__(1024) k_weight_cap(const float *samples, int n, float factor, float *cap) {
-    __shared__ float part[1024];
+// *sum += the weights of samples [0, n) (block sums, one atomic per block;
+// *sum cleared before), then k_weight_cap turns it into factor x the mean.
+__global__ void __launch_bounds__(256) k_weight_sum(const float *samples, int n, float *sum) {
+    __shared__ float part[256];
     float s = 0;
-    for (int i = threadIdx.x; i < n; i += 1024) s += samples[size_t(i) * kGuideSample + 9];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += samples[size_t(i) * kGuideSample + 9];
     part[threadIdx.x] = s;
     __syncthreads();
-    for (int off = 512; off > 0; off >>= 1) {
+    for (int off = 128; off > 0; off >>= 1) {
         if (int(threadIdx.x) < off) part[threadIdx.x] += part[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *cap = factor * part[0] / float(n > 0 ? n : 1);
+    if (threadIdx.x == 0) atomicAdd(sum, part[0]);
 }
+
+__global__ void k_weight_cap(float *cap, int n, float factor) { *cap = factor * *cap / float(n > 0 ? n : 1); }
 
 
 // Features of table entries [0, n): each cell's centre with one of the six
@@ -305,19 +324,25 @@ void GuideNet::adam(float scale, hipStream_t st) {
     HIP_OK(hipGetLastError());
 }
 
-// Minibatch Adam over samples [0, n): steps of `batch` samples, weights
-// capped at cap_factor x their mean over [0, n) when cap_factor > 0.
-void GuideNet::train(const float *samples, int n, int batch, hipStream_t st, float cap_factor) {
+// `epochs` passes of minibatch Adam over samples [0, n): steps of `batch`
+// samples, weights capped at cap_factor x their mean over [0, n) when
+// cap_factor > 0 (the cap computed once, before the first pass).
+void GuideNet::train(const float *samples, int n, int batch, hipStream_t st, float cap_factor, int epochs) {
     if (n <= 0) return;
-    const bool capped = cap_factor > 0;
-    if (capped) {
-        k_weight_cap<<<1, 1024, 0, st>>>(samples, n, cap_factor, cap.p);
+    const float *capp = nullptr;
+    if (cap_factor > 0) {
+        HIP_OK(hipMemsetAsync(cap.p, 0, sizeof(float), st));
+        k_weight_sum<<<std::min((n + 255) / 256, 512), 256, 0, st>>>(samples, n, cap.p);
         HIP_OK(hipGetLastError());
+        k_weight_cap<<<1, 1, 0, st>>>(cap.p, n, cap_factor);
+        HIP_OK(hipGetLastError());
+        capp = cap.p;
     }
-    for (int b0 = 0; b0 + batch <= n; b0 += batch) {
-        gradient(samples + size_t(b0) * kGuideSample, batch, st, capped ? cap.p : nullptr);
-        adam(1.0f / float(batch), st);
-    }
+    for (int e = 0; e < epochs; e++)
+        for (int b0 = 0; b0 + batch <= n; b0 += batch) {
+            gradient(samples + size_t(b0) * kGuideSample, batch, st, capp);
+            adam(1.0f / float(batch), st);
+        }
 }
 
 // The table's entries from the current weights: features of every entry

@@ -1980,7 +1980,7 @@ int lambda_hip_progressive_step(lambda_hip_progressive *pr, int32_t samples, int
             GuideNet &net = *pr->net;
             HIP_OK(hipMemsetAsync(net.loss.p, 0, sizeof(float), R.st));
             HIP_OK(hipEventRecord(ev.a, R.st));
-            for (int e = 0; e < pr->epochs; e++) net.train(pr->train.p, int(collected), pr->batch, R.st);
+            net.train(pr->train.p, int(collected), pr->batch, R.st, kWeightCap, pr->epochs);
             pr->table_stale = true;
             HIP_OK(hipEventRecord(ev.b, R.st));
             HIP_OK(hipEventSynchronize(ev.b));
