@@ -114,7 +114,7 @@ def cpu_model() -> str:
 
 def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, warmup: int = 0,
                    train_steps: int = 1 << 30, epochs: int = 1, batch: int = 1 << 14, guided_bounces: int = 0,
-                   table_cells: int = 0, mix: float = 0.5):
+                   table_cells: int = 0, mix: float = 0.5, rr_start: int = 0):
     """Progressive render until relMSE against ref <= target; returns the
     render-and-train wall time at which the target was crossed (log-linear
     interpolation between steps), the samples per pixel then, and the curve.
@@ -123,6 +123,8 @@ def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, 
     from the tabulated network.  relMSE evaluation is not timed."""
     import lambda_amd
     p = lambda_amd.Progressive(dev, guide)
+    if rr_start:
+        p.set_roulette(rr_start)
     if guide is not None:
         p.set_training(epochs, batch, 1e-3, guided_bounces)
         if table_cells:
@@ -186,6 +188,10 @@ def guided_leg(args, scene_dir: str) -> dict:
     log("guided leg: %dx%d reference at %d spp in %.1f s" % (w, h, args.guided_ref_spp, t_ref))
     plain, plain_curve = time_to_target(dev, None, ref, 1e-3, args.guided_step, args.guided_max_spp)
     log("guided leg: plain reaches relMSE 1e-3 at %s" % (plain,))
+    # the plain tracer without Russian roulette (as the guided sessions run):
+    # the share of the guided gain that comes from the roulette setting alone
+    plain_nr, _ = time_to_target(dev, None, ref, 1e-3, args.guided_step, args.guided_max_spp, rr_start=17)
+    log("guided leg: plain without roulette reaches relMSE 1e-3 at %s" % (plain_nr,))
     # Training is not deterministic (the order of the gradient atomics and of
     # the training-sample buffer), so the guided time varies from run to run:
     # several runs with their own guide seeds, the median reported.
@@ -217,6 +223,9 @@ def guided_leg(args, scene_dir: str) -> dict:
                                % args.guided_table_cells) if args.guided_table_cells else
                               "persistent kernel, network evaluated per wave on the matrix cores",
            "plain_s": plain[0] if plain else None, "plain_spp": plain[1] if plain else None,
+           "plain_no_roulette_s": plain_nr[0] if plain_nr else None,
+           "plain_no_roulette_spp": plain_nr[1] if plain_nr else None,
+           "roulette": "plain: from bounce 5 (the reference's); guided: off",
            "guided_runs": len(runs), "guided_s_runs": [r[0][0] if r[0] else None for r in runs],
            "guided_spp_runs": [r[0][1] if r[0] else None for r in runs],
            "guided_s": guided[0] if guided else None, "guided_spp": guided[1] if guided else None,
@@ -225,6 +234,8 @@ def guided_leg(args, scene_dir: str) -> dict:
            "guided_curve": guided_curve[::max(1, len(guided_curve) // 12)]}
     if plain and guided:
         out["guided_over_plain_time"] = guided[0] / plain[0]
+    if plain_nr and guided:
+        out["guided_over_plain_no_roulette_time"] = guided[0] / plain_nr[0]
     return out
 
 

@@ -535,6 +535,15 @@ int lambda_hip_progressive_set_guide_table(lambda_hip_progressive *session, int3
  * one-sample mixture 0.5 cos/pi + 0.5 q); frac in (0, 0.95], the cosine
  * lobe takes the rest and both MIS densities use the same mixture. */
 int lambda_hip_progressive_set_guide_mix(lambda_hip_progressive *session, float frac, char *err, size_t err_len);
+/* Bounce index from which Russian roulette applies: plain sessions default
+ * to 5, the reference's (render/renderer.h:185-191); guided sessions default
+ * to off (depth + 1: every path runs to the depth or to a miss or an
+ * emitter), since roulette on the throughput ends most of the paths the
+ * guide sends toward the light (their throughput is small where the guide's
+ * density is high).  Any value keeps the estimator unbiased.
+ * start_bounce >= 1. */
+int lambda_hip_progressive_set_roulette(lambda_hip_progressive *session, int32_t start_bounce, char *err,
+                                        size_t err_len);
 void lambda_hip_progressive_destroy(lambda_hip_progressive *session);
 
 /* Device name and compute-unit count of the current HIP device. */

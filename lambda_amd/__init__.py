@@ -396,6 +396,13 @@ class Progressive:
         err = C.create_string_buffer(4096)
         _check(self._lib.lambda_hip_progressive_set_guide_mix(self._h, frac, err, 4096), err)
 
+    def set_roulette(self, start_bounce: int) -> None:
+        """Bounce index from which Russian roulette applies (plain sessions:
+        5, the reference's; guided sessions: off); above the path depth it
+        is off."""
+        err = C.create_string_buffer(4096)
+        _check(self._lib.lambda_hip_progressive_set_roulette(self._h, start_bounce, err, 4096), err)
+
     def image(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 3), np.float32)
         err = C.create_string_buffer(4096)

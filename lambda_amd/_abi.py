@@ -194,6 +194,7 @@ def hip():
                                                             C.c_char_p, C.c_size_t]
         lib.lambda_hip_progressive_set_guide_table.argtypes = [vp, C.c_int32, C.c_char_p, C.c_size_t]
         lib.lambda_hip_progressive_set_guide_mix.argtypes = [vp, C.c_float, C.c_char_p, C.c_size_t]
+        lib.lambda_hip_progressive_set_roulette.argtypes = [vp, C.c_int32, C.c_char_p, C.c_size_t]
         lib.lambda_hip_progressive_destroy.argtypes = [vp]
         lib.lambda_hip_progressive_destroy.restype = None
         _hip = lib

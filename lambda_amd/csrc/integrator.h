@@ -88,6 +88,7 @@ struct Frame {
     int grec_stride;          // guided vertices per slot (depth + 1)
     int guide_bounces;        // vertices with bounce index below this are guided
     float guide_frac;         // probability of sampling the guide at a guided vertex
+    int rr_start;             // bounce index from which Russian roulette applies
     float3 box_lo, box_hi;    // scene box for the guiding features
 };
 
@@ -442,7 +443,7 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                         S4 f = att * ct / prev_pdf;
                         beta = beta * f;
                     }
-                    if (bounce >= kRRStart) {
+                    if (bounce >= F.rr_start) {
                         float qq = 1 - clampf(s4max(beta), float(0.05), float(0.95));
                         if (rng.next() < qq) finished = true;
                         else {

@@ -436,6 +436,7 @@ Frame make_frame(const lambda_render_params &p) {
     F.total_work = (long long)F.npix * F.spp;
     F.cost = nullptr;
     F.guide_frac = kGuideFrac;
+    F.rr_start = kRRStart;
     return F;
 }
 
@@ -931,6 +932,9 @@ struct lambda_hip_progressive {
     int epochs = 1;
     int guide_bounces = kDefaultGuidedBounces;
     float guide_frac = kGuideFrac;  // probability of sampling the guide
+    // bounce index from which Russian roulette applies: the reference's for
+    // plain sessions; guided sessions run without it (set at creation)
+    int rr_start = kRRStart;
     bool fused = false;          // guided: network inside the persistent kernel
     double seen_per_sample = 0;  // training candidates per sample per pixel, last step
     DevBuf<double> scratch_fb;   // framebuffer of steps that are not kept
@@ -959,6 +963,7 @@ Frame progressive_frame(const lambda_hip_progressive &pr, int samples, int root,
         F.guide_bounces = pr.guide_bounces;
         F.guide_frac = pr.guide_frac;
     }
+    F.rr_start = pr.rr_start;
     if (pr.net && train) {
         F.train = pr.train.p;
         F.train_cap = pr.train_cap;
@@ -1920,6 +1925,13 @@ int lambda_hip_progressive_create(lambda_hip_scene *scene, const lambda_render_p
         pr->scene = scene;
         pr->params = p;
         pr->net = guide ? &guide->net : nullptr;
+        // Throughput-based roulette works against a guide: after a guided
+        // bounce toward bright directions the throughput is small because
+        // the sampling density there is high, so roulette would end most of
+        // exactly those paths and weight the survivors up to 20x.  Guided
+        // sessions therefore run every path to the depth (or to a miss or an
+        // emitter) unless the caller sets a start.
+        if (guide) pr->rr_start = p.depth + 1;
         // guided: the network inside the persistent kernel unless the
         // caller asks for one launch per bounce (tail_paths -1)
         pr->fused = guide && params->tail_paths != -1;
@@ -2081,6 +2093,14 @@ int lambda_hip_progressive_set_guide_mix(lambda_hip_progressive *pr, float frac,
         if (!pr->net) throw Failure{LAMBDA_EINVAL, "the session has no guide"};
         if (!(frac > 0 && frac <= kGuideFracMax)) throw Failure{LAMBDA_EINVAL, "invalid guide fraction"};
         pr->guide_frac = frac;
+    });
+}
+
+int lambda_hip_progressive_set_roulette(lambda_hip_progressive *pr, int32_t start_bounce, char *err, size_t err_len) {
+    return guarded(err, err_len, [&] {
+        if (!pr) throw Failure{LAMBDA_EINVAL, "null argument"};
+        if (start_bounce < 1) throw Failure{LAMBDA_EINVAL, "invalid roulette start"};
+        pr->rr_start = start_bounce;
     });
 }
 
