@@ -150,6 +150,8 @@ def time_to_target(dev, guide, ref, target: float, step_spp: int, max_spp: int, 
         spp += step_spp
         err = lambda_amd.rel_mse(p.image(), ref)
         curve.append((round(elapsed, 4), spp, err))
+        if kept % 50 == 0:
+            log("  %s: %d spp, relMSE %.3g, %.1f s" % ("guided" if guide is not None else "plain", spp, err, elapsed))
         if err <= target:
             if prev is None:
                 hit = (elapsed, spp)
@@ -178,14 +180,31 @@ def guided_leg(args, scene_dir: str) -> dict:
     scene = lambda_amd.Scene.load(lrd)
     dev = lambda_amd.DeviceScene(scene)
     t0 = time.perf_counter()
-    # the reference's streams are independent of the measured runs
-    ref_session = lambda_amd.Progressive(dev, None, seed=0x5EED0F2EF)
-    for _ in range(args.guided_ref_spp // 256):
-        ref_session.step(256, train=False)
-    ref = ref_session.image()
-    ref_session.close()
-    t_ref = time.perf_counter() - t0
-    log("guided leg: %dx%d reference at %d spp in %.1f s" % (w, h, args.guided_ref_spp, t_ref))
+    if args.guided_ref_load:
+        import numpy as np
+        ref = np.load(args.guided_ref_load)
+        if ref.shape != (h, w, 3):
+            raise SystemExit("%s holds a %s image, not %dx%d" % (args.guided_ref_load, ref.shape, w, h))
+        log("guided leg: %dx%d reference loaded from %s" % (w, h, args.guided_ref_load))
+    else:
+        # the reference's streams are independent of the measured runs
+        ref_session = lambda_amd.Progressive(dev, None, seed=0x5EED0F2EF)
+        steps = args.guided_ref_spp // 256
+        for k in range(steps):
+            ref_session.step(256, train=False)
+            if (k + 1) % 32 == 0:
+                log("guided leg: reference %d of %d spp" % (256 * (k + 1), args.guided_ref_spp))
+        ref = ref_session.image()
+        ref_session.close()
+        t_ref = time.perf_counter() - t0
+        log("guided leg: %dx%d reference at %d spp in %.1f s" % (w, h, args.guided_ref_spp, t_ref))
+        if args.guided_ref_save:
+            import numpy as np
+            np.save(args.guided_ref_save, ref)
+            if args.guided_ref_only:
+                dev.close()
+                return {"reference_saved": args.guided_ref_save, "width": w, "height": h,
+                        "reference_spp": args.guided_ref_spp, "seconds": t_ref}
     plain, plain_curve = time_to_target(dev, None, ref, 1e-3, args.guided_step, args.guided_max_spp)
     log("guided leg: plain reaches relMSE 1e-3 at %s" % (plain,))
     # the plain tracer without Russian roulette (as the guided sessions run):
@@ -436,6 +455,12 @@ def main() -> int:
     ap.add_argument("--guided-batch", type=int, default=16384)
     ap.add_argument("--guided-bounces", type=int, default=0,
                     help="vertices guided per path (bounce index below this; 0 = the library default, 8)")
+    ap.add_argument("--guided-ref-save", default="", help="save the guided leg's reference image (.npy)")
+    ap.add_argument("--guided-ref-load", default="", help="use a reference image saved by --guided-ref-save")
+    ap.add_argument("--guided-ref-only", action="store_true",
+                    help="with --guided-ref-save: render and save the reference, then stop")
+    ap.add_argument("--guided-only", action="store_true",
+                    help="run only the guided-vs-plain leg and print its record (no paths/s line)")
     ap.add_argument("--guided-table-cells", type=int, default=48,
                     help="guide from a table of the network's mixtures with this many cells along the scene's "
                          "longest side (0: the network inside the persistent kernel)")
@@ -468,6 +493,11 @@ def main() -> int:
 
     name, cus = lambda_amd.device_info()
     log("device %s, %d CUs" % (name, cus))
+    if args.guided_only:
+        rec = guided_leg(args, args.scene_dir)
+        rec.update({"device": name, "metric": "wall time to relMSE 1e-3, guided vs plain"})
+        print(json.dumps(rec), flush=True)
+        return 0
     if args.copy_probe:
         copy_probe(args.copy_probe)
     t0 = time.perf_counter()
