@@ -908,8 +908,9 @@ void render_with(Renderer &R, const Frame &F, const lambda_render_params &p, lon
 // vertices sample the guiding mixture and each step can train the network on
 // the radiance its own paths gathered.
 // Guided vertices per sample when a session does not set them: the first
-// eight bounces (on the guiding room, 2, 4, 6, 10, 12, 16 or every bounce
-// gave less variance reduction per unit time).
+// eight bounces (on the guiding room with roulette on, 2, 4, 6, 10, 12 or 16
+// gave less variance reduction per unit time; with roulette off, as guided
+// sessions run, 12 and every bounce were within the run-to-run spread).
 constexpr int kDefaultGuidedBounces = 8;
 // Tabulated guide limits: cells along the longest side, entries in all
 // (2^24 entries of 40 floats = 2.7 GB).
