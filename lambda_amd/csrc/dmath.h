@@ -14,6 +14,12 @@
 // Callable from host code too, so the device arithmetic can be exercised on
 // the CPU by test programs.
 #define LAM_FN __host__ __device__ __forceinline__
+// Heavy functions that few vertices reach (thin-film transfer matrices,
+// procedural textures): kept out of line, so that each kernel carries one
+// copy instead of one per call site (the persistent kernel's code shrinks
+// from 546 KB to 100 KB plus 56 KB for the two callees; 1080p frame
+// 2.5 % faster).
+#define LAM_OUTLINE inline __host__ __device__ __noinline__
 
 namespace lam {
 

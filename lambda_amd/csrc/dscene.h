@@ -443,7 +443,7 @@ LAM_FN bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_ins
 
 // --------------------------------------------------------------- textures
 
-LAM_FN float scalar_texture(const DeviceScene &S, int t, V3 p) {
+LAM_OUTLINE float scalar_texture(const DeviceScene &S, int t, V3 p) {
     float4 x = S.stex[t];
     int type = as_int(x.x);
     if (type == LAMBDA_SCALAR_CONSTANT) return x.y;
@@ -453,19 +453,24 @@ LAM_FN float scalar_texture(const DeviceScene &S, int t, V3 p) {
     return x.y + (x.z - x.y) * worley(p * x.w, tab);
 }
 
-// Spectrum texture at one wavelength (scene/texture.h:178-210).  Checker
-// cells are in (u, v) units.
-LAM_FN float spectrum_texture(const DeviceScene &S, int t, const Hit &h, float lambda) {
-    int4 x = S.xtex[t];
-    if (x.x == LAMBDA_SPECTRUM_CONSTANT) return dense(S.spectra + size_t(x.y) * kBins, lambda);
+// Constant or checker spectrum texture x at one wavelength
+// (scene/texture.h:178-210); checker cells are in (u, v) units.
+LAM_FN float spectrum_texture_tabulated(const DeviceScene &S, int4 x, const Hit &h, float lambda) {
+    int s = x.y;
     if (x.x == LAMBDA_SPECTRUM_CHECKER) {
         float scale = as_float(x.w);
         int cx = int(floorf(h.u / scale));
         int cy = int(floorf(h.v / scale));
-        int s = (cx + cy) % 2 == 0 ? x.y : x.z;
-        return dense(S.spectra + size_t(s) * kBins, lambda);
+        s = (cx + cy) % 2 == 0 ? x.y : x.z;
     }
-    return scalar_texture(S, x.y, h.p);
+    return dense(S.spectra + size_t(s) * kBins, lambda);
+}
+
+// Spectrum texture at one wavelength (scene/texture.h:178-210).
+LAM_FN float spectrum_texture(const DeviceScene &S, int t, const Hit &h, float lambda) {
+    int4 x = S.xtex[t];
+    if (x.x == LAMBDA_SPECTRUM_SCALAR) return scalar_texture(S, x.y, h.p);
+    return spectrum_texture_tabulated(S, x, h, lambda);
 }
 
 // Texture that does not depend on the wavelength: evaluated once per vertex.
@@ -473,14 +478,15 @@ LAM_FN bool spectrum_texture_is_scalar(const DeviceScene &S, int t) {
     return S.xtex[t].x == LAMBDA_SPECTRUM_SCALAR;
 }
 
+// The four wavelengths of a vertex; a scalar texture is evaluated once (it
+// does not depend on the wavelength), so only one copy of its code is
+// inlined per call site.
 LAM_FN S4 spectrum_texture4(const DeviceScene &S, int t, const Hit &h, const S4 &lam) {
     S4 r;
-    if (spectrum_texture_is_scalar(S, t)) {
-        float v = scalar_texture(S, S.xtex[t].y, h.p);
-        return s4(v);
-    }
+    const int4 x = S.xtex[t];
+    if (x.x == LAMBDA_SPECTRUM_SCALAR) return s4(scalar_texture(S, x.y, h.p));
 #pragma unroll
-    for (int j = 0; j < kHero; j++) r.v[j] = spectrum_texture(S, t, h, lam.v[j]);
+    for (int j = 0; j < kHero; j++) r.v[j] = spectrum_texture_tabulated(S, x, h, lam.v[j]);
     return r;
 }
 
@@ -637,7 +643,7 @@ LAM_FN V3 instance_sample(const DeviceScene &S, int inst, int obj, V3 p, Pcg &rn
 // of d} at wavelength lambda, for a ray of direction d hitting a surface with
 // normal n (facing the ray).  thick[] holds the layer thicknesses (nm) in
 // material order; front selects the stack orientation (material.h:264-268).
-LAM_FN float thinfilm_reflectance(const DeviceScene &S, int4 mt, bool front, V3 n, V3 d, float lambda,
+LAM_OUTLINE float thinfilm_reflectance(const DeviceScene &S, int4 mt, bool front, V3 n, V3 d, float lambda,
                                   const float *thick) {
     const int layers = mt.y, n_off = mt.z;
     const int j_off = front ? 0 : layers + 1;
