@@ -280,6 +280,28 @@ LAM_FN bool test_object_ranked(const DeviceScene &S, int k, float4 a, float4 b, 
     return true;
 }
 
+// test_object_ranked with the mode chosen at run time.
+LAM_FN bool test_object_mode(const DeviceScene &S, bool any, int k, float4 a, float4 b, float4 c, V3 o, V3 d,
+                             float &best, int inst, int &hit_inst, int &hit_obj) {
+    int type = as_int(b.w);
+    float limit = any ? best : nextafterf(best, INFINITY);
+    float t, al, be;
+    if (type == LAMBDA_OBJECT_SPHERE) {
+        if (!test_sphere(a, o, d, limit, t)) return false;
+    } else if (type == LAMBDA_OBJECT_TRI) {
+        if (!test_tri(v3(a), v3(b), v3(c), o, d, limit, t, al, be)) return false;
+    } else {
+        float4 n = S.obj[4 * size_t(k) + 3];
+        if (!test_quad(v3(a), v3(b), v3(c), v3(n), o, d, limit, t, al, be)) return false;
+    }
+    if (!any && t == best && hit_obj >= 0 && ref_rank(S, inst, k) > ref_rank(S, hit_inst, hit_obj))
+        return false;
+    best = t;
+    hit_inst = inst;
+    hit_obj = k;
+    return true;
+}
+
 // Entries that wait for a visit: a wide node (code >= 0) or an instance
 // (code = -1 - instance), with the entry distance of its box.  The top
 // kRing entries of each lane live in LDS (entry j at slot j mod kRing,
@@ -287,10 +309,21 @@ LAM_FN bool test_object_ranked(const DeviceScene &S, int k, float4 a, float4 b, 
 // entries are moved to a global area laid out [entry][thread] when the ring
 // is full and brought back when the stack shrinks to them.  The host sizes
 // the global area from the exact worst-case depth of the tree.
+//
+// In device code the ring pointers carry the LDS address space: with plain
+// (generic) pointers the compiler turns the pop's two candidate reads (ring
+// or global area) into one flat load through a selected pointer, which is
+// slower than an LDS read and waits on both memory counters.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define LAM_LDS __attribute__((address_space(3)))
+#else
+#define LAM_LDS
+#endif
+
 template <int kRing>
 struct TraversalStack {
-    int *lcode;          // LDS, kRing * stride
-    float *ldist;        // LDS, kRing * stride
+    LAM_LDS int *lcode;     // LDS, kRing * stride
+    LAM_LDS float *ldist;   // LDS, kRing * stride
     int stride;
     int2 *spill;         // global, entry j of this thread at spill[j * spill_stride]
     size_t spill_stride;
@@ -441,16 +474,197 @@ LAM_FN bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_ins
     }
 }
 
+// One lane's occlusion ray (when `occl`) and then its closest-hit ray (when
+// `closest`) in one loop.  A lane moves on to its closest-hit ray as soon
+// as its occlusion ray is resolved, so in a wave the lanes without an
+// occlusion ray traverse for their closest hits while the others finish
+// theirs, instead of the whole wave running a separate any-hit traversal
+// for a few lanes.  Each ray gets exactly the answer of traverse<true> and
+// traverse<false>: the visiting order, the entry-distance pruning and the
+// ranked tie rule are the same, only the mode is chosen at run time.
+//   occlusion ray: (so, sd), light distance s_dist, light (skip_inst,
+//     skip_obj); blocked = true when something lies before the light.
+//   closest-hit ray: origin and direction read from *co, *cd when the
+//     occlusion ray is done (or at once); (hit_inst, hit_obj, best) as
+//     traverse<false> returns them, hit_inst = hit_obj = -1 on a miss.
+template <int kRing>
+LAM_FN void traverse_pair(const DeviceScene &S, bool occl, V3 so, V3 sd, float s_dist, int skip_inst, int skip_obj,
+                          bool &blocked, bool closest, const float4 *co, const float4 *cd, float &best,
+                          int &hit_inst, int &hit_obj, TraversalStack<kRing> &stk) {
+    blocked = false;
+    hit_inst = hit_obj = -1;
+    best = kMax;
+    if (!occl && !closest) return;
+    bool any = occl;
+    V3 o, d;
+    float lim;
+    if (any) {
+        o = so;
+        d = sd;
+        lim = s_dist;
+    } else {
+        o = v3(*co);
+        d = v3(*cd);
+        lim = kMax;
+    }
+    V3 inv = v3(1 / d.x, 1 / d.y, 1 / d.z);
+    int hi = -1, ho = -1;
+    bool in_blas = false;
+    int blas_base = 0;
+    int inst = -1;
+    V3 ro = o, rd = d, ri = inv;
+    int cur = 0;
+    stk.sp = stk.lo = 0;
+    while (true) {
+        if (cur < 0) {
+            inst = -1 - cur;
+            local_ray(S, inst, o, d, ro, rd);
+            ri = v3(1 / rd.x, 1 / rd.y, 1 / rd.z);
+            in_blas = true;
+            blas_base = stk.sp;
+            cur = S.inst_root[inst];
+        }
+        const float4 *w = S.wide + 8 * size_t(cur);
+        const float4 lx = w[0], hx = w[1], ly = w[2], hy = w[3], lz = w[4], hz = w[5];
+        const float4 cw = w[6], nw = w[7];
+        const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, HX[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
+        const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
+        const int CH[4] = {as_int(cw.x), as_int(cw.y), as_int(cw.z), as_int(cw.w)};
+        const int CN[4] = {as_int(nw.x), as_int(nw.y), as_int(nw.z), as_int(nw.w)};
+        float key[4];
+        int code[4];
+        // box tests of the four slots; the hit object leaves are noted in
+        // `leaves` and their objects tested afterwards in one loop, so that
+        // lanes whose leaves sit in different slots test them together
+        unsigned leaves = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            float tn;
+            bool hit = CN[j] >= 0 && slab(LX[j], LY[j], LZ[j], HX[j], HY[j], HZ[j], ro, ri, tn) && !(tn > lim);
+            key[j] = INFINITY;
+            code[j] = 0;
+            if (!hit) continue;
+            if (CN[j] == 0) {
+                key[j] = tn;
+                code[j] = CH[j];
+            } else if (!in_blas) {
+                key[j] = tn;
+                code[j] = -1 - CH[j];
+            } else {
+                leaves |= 1u << j;
+            }
+        }
+        bool done = false;   // the current ray is resolved
+        // objects two at a time, both records requested before either is
+        // tested (a lone last object is read twice)
+        int k = 0, end = 0;
+        while (true) {
+            if (k >= end) {
+                if (leaves == 0) break;
+                const int j = __builtin_ctz(leaves);
+                leaves &= leaves - 1;
+                k = j == 0 ? CH[0] : j == 1 ? CH[1] : j == 2 ? CH[2] : CH[3];
+                end = k + (j == 0 ? CN[0] : j == 1 ? CN[1] : j == 2 ? CN[2] : CN[3]);
+            }
+            const bool two = k + 1 < end;
+            const float4 *r0 = S.obj + 4 * size_t(k);
+            const float4 *r1 = two ? r0 + 4 : r0;
+            const float4 a0 = r0[0], b0 = r0[1], c0 = r0[2];
+            const float4 a1 = r1[0], b1 = r1[1], c1 = r1[2];
+            if (!(any && inst == skip_inst && k == skip_obj) &&
+                test_object_mode(S, any, k, a0, b0, c0, ro, rd, lim, inst, hi, ho) && any) {
+                done = true;
+                break;
+            }
+            if (two && !(any && inst == skip_inst && k + 1 == skip_obj) &&
+                test_object_mode(S, any, k + 1, a1, b1, c1, ro, rd, lim, inst, hi, ho) && any) {
+                done = true;
+                break;
+            }
+            k += 2;
+        }
+        if (done) {
+            blocked = true;
+        } else {
+            sort4(key, code);
+            int m = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) m += !(key[j] > lim) ? 1 : 0;
+            if (m > 0) {
+                for (int j = m - 1; j >= 1; j--) stk.push(code[j], key[j]);
+                cur = code[0];
+                continue;
+            }
+            bool next = false;
+            while (!next) {
+                if (in_blas && stk.sp == blas_base) {
+                    in_blas = false;
+                    ro = o;
+                    rd = d;
+                    ri = inv;
+                    continue;
+                }
+                if (stk.sp == 0) break;
+                int c;
+                float t;
+                stk.pop(c, t);
+                if (!(t > lim)) {
+                    cur = c;
+                    next = true;
+                }
+            }
+            if (next) continue;
+        }
+        // the current ray is resolved
+        if (any && closest) {
+            any = false;
+            hi = ho = -1;
+            o = v3(*co);
+            d = v3(*cd);
+            lim = kMax;
+            inv = v3(1 / d.x, 1 / d.y, 1 / d.z);
+            in_blas = false;
+            inst = -1;
+            ro = o;
+            rd = d;
+            ri = inv;
+            cur = 0;
+            stk.sp = stk.lo = 0;
+            continue;
+        }
+        if (!any) {
+            hit_inst = hi;
+            hit_obj = ho;
+            best = lim;
+        }
+        return;
+    }
+}
+
 // --------------------------------------------------------------- textures
 
-LAM_OUTLINE float scalar_texture(const DeviceScene &S, int t, V3 p) {
-    float4 x = S.stex[t];
+//
+// The out-of-line functions take the tables they read as pointer arguments,
+// never the scene by reference: a reference would make the kernel keep its
+// scene record in scratch memory, and every table pointer read back from
+// there loses its global address space, so the traversal's loads would be
+// issued as flat loads (which also make every LDS stack access wait for
+// them).
+
+LAM_OUTLINE float scalar_texture_at(const float4 *stex, const int *perm, const float4 *worley_points,
+                                    const int4 *worley_box, const int4 *worley_dim, int t, V3 p) {
+    float4 x = stex[t];
     int type = as_int(x.x);
     if (type == LAMBDA_SCALAR_CONSTANT) return x.y;
-    if (type == LAMBDA_SCALAR_PERLIN) return x.y + (x.z - x.y) * perlin(S.perm, p * x.w);
-    const int4 box = S.worley_box[t], dim = S.worley_dim[t];
-    WorleyTable tab{dim.x > 0 ? S.worley_points + box.w : nullptr, box.x, box.y, box.z, dim.x, dim.y, dim.z};
+    if (type == LAMBDA_SCALAR_PERLIN) return x.y + (x.z - x.y) * perlin(perm, p * x.w);
+    const int4 box = worley_box[t], dim = worley_dim[t];
+    WorleyTable tab{dim.x > 0 ? worley_points + box.w : nullptr, box.x, box.y, box.z, dim.x, dim.y, dim.z};
     return x.y + (x.z - x.y) * worley(p * x.w, tab);
+}
+
+LAM_FN float scalar_texture(const DeviceScene &S, int t, V3 p) {
+    return scalar_texture_at(S.stex, S.perm, S.worley_points, S.worley_box, S.worley_dim, t, p);
 }
 
 // Constant or checker spectrum texture x at one wavelength
@@ -643,13 +857,13 @@ LAM_FN V3 instance_sample(const DeviceScene &S, int inst, int obj, V3 p, Pcg &rn
 // of d} at wavelength lambda, for a ray of direction d hitting a surface with
 // normal n (facing the ray).  thick[] holds the layer thicknesses (nm) in
 // material order; front selects the stack orientation (material.h:264-268).
-LAM_OUTLINE float thinfilm_reflectance(const DeviceScene &S, int4 mt, bool front, V3 n, V3 d, float lambda,
-                                  const float *thick) {
+LAM_OUTLINE float thinfilm_reflectance_at(const float2 *cspectra, const int *props, int4 mt, bool front, V3 n, V3 d,
+                                          float lambda, const float *thick) {
     const int layers = mt.y, n_off = mt.z;
     const int j_off = front ? 0 : layers + 1;
     const int j_sign = front ? 1 : -1;
     auto index_of = [&](int j) {
-        float2 v = dense_c(S.cspectra + size_t(S.props[n_off + j_off + j_sign * j]) * kBins, lambda);
+        float2 v = dense_c(cspectra + size_t(props[n_off + j_off + j_sign * j]) * kBins, lambda);
         return cff(v.x, v.y);
     };
     const CFF one = cff(1.0f);
@@ -686,6 +900,11 @@ LAM_OUTLINE float thinfilm_reflectance(const DeviceScene &S, int4 mt, bool front
     const FF rs = cffabs2(es_far - ys) / cffabs2(es_far + ys);
     const FF rp = cffabs2(ep_far - yp) / cffabs2(ep_far + yp);
     return ffloat((rs + rp) * 0.5f);
+}
+
+LAM_FN float thinfilm_reflectance(const DeviceScene &S, int4 mt, bool front, V3 n, V3 d, float lambda,
+                                  const float *thick) {
+    return thinfilm_reflectance_at(S.cspectra, S.props, mt, front, n, d, lambda, thick);
 }
 
 constexpr int kMaxLayers = 8;

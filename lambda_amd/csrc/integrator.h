@@ -151,8 +151,8 @@ constexpr int kNoAllocMaxDevices = 64;
 
 __device__ __forceinline__ Stack make_stack(int *s_code, float *s_dist, int2 *spill) {
     Stack s;
-    s.lcode = s_code + threadIdx.x;
-    s.ldist = s_dist + threadIdx.x;
+    s.lcode = (LAM_LDS int *)(s_code + threadIdx.x);
+    s.ldist = (LAM_LDS float *)(s_dist + threadIdx.x);
     s.stride = kBlock;
     const size_t threads = size_t(gridDim.x) * blockDim.x;
     s.spill = spill ? spill + (size_t(blockIdx.x) * blockDim.x + threadIdx.x) : nullptr;

@@ -26,34 +26,50 @@ __global__ void __launch_bounds__(kBlock, kWaves) k_persist(DeviceScene S, Frame
     Stack stk = make_stack(s_code, s_dist, spill);
     unsigned n_closest = 0, n_shadow = 0;
     if (F.cost) F.unit_t0[s] = wall_clock64();
+    // Each iteration traces the occlusion ray of the previous vertex (if
+    // any) and then the closest hit of the path's next ray, in one
+    // traversal loop; a path that ended with an occlusion ray pending skips
+    // its closest hit for one iteration and retires after the occlusion
+    // ray.  Radiance is summed in the same order as the reference's trace
+    // loop: the occlusion ray's contribution lands before the next vertex's.
+    bool pend = false;   // an occlusion ray of the previous vertex waits
+    bool alive = true;   // the path has a closest-hit ray to trace
+    ShadowRay sh{};
     while (true) {
-        n_closest++;
         {
-            float4 o = P.o[s], d = P.d[s];
-            float best = kMax;
-            int hi = -1, ho = -1;
-            stk.sp = stk.lo = 0;
-            trace_ray<false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk);
-            P.hit[s] = make_int2(hi, ho);
-        }
-        bool push = false;
-        ShadowRay sh;
-        const bool ended = shade_slot(S, F, P, s, push, sh);
-        if (push) {
-            n_shadow++;
-            float best = sh.o.w;
-            int hi = -1, ho = -1;
-            stk.sp = stk.lo = 0;
-            if (!trace_ray<true>(S, v3(sh.o), v3(sh.d), best, hi, ho, sh.i.y, sh.i.z, stk)) {
-                float4 L = P.L[s];
-                L.x += sh.c.x;
-                L.y += sh.c.y;
-                L.z += sh.c.z;
-                L.w += sh.c.w;
-                P.L[s] = L;
+            bool blocked;
+            float best;
+            int hi, ho;
+            traverse_pair<kRing>(S, pend, v3(sh.o), v3(sh.d), sh.o.w, sh.i.y, sh.i.z, blocked, alive, P.o + s,
+                                 P.d + s, best, hi, ho, stk);
+            if (pend) {
+                n_shadow++;
+                if (!blocked) {
+                    float4 L = P.L[s];
+                    L.x += sh.c.x;
+                    L.y += sh.c.y;
+                    L.z += sh.c.z;
+                    L.w += sh.c.w;
+                    P.L[s] = L;
+                }
+                pend = false;
+            }
+            if (alive) {
+                n_closest++;
+                P.hit[s] = make_int2(hi, ho);
             }
         }
-        if (!ended) continue;
+        if (alive) {
+            bool push = false;
+            const bool ended = shade_slot(S, F, P, s, push, sh);
+            pend = push;
+            if (!ended) continue;
+            if (pend) {
+                alive = false;
+                continue;
+            }
+        }
+        alive = true;
         if (retire_sample(S, F, P, s)) continue;
         if (F.cost) {
             // device-clock time of the unit (its pixel's samples in this launch)

@@ -77,8 +77,8 @@ int main(int argc, char **argv) {
     std::vector<int2> spill(size_t(std::max(need, 1)));
     auto make = [&]() {
         TraversalStack<RING> s;
-        s.lcode = lcode;
-        s.ldist = ldist;
+        s.lcode = (decltype(s.lcode))lcode;
+        s.ldist = (decltype(s.ldist))ldist;
         s.stride = 1;
         s.spill = spill.data();
         s.spill_stride = 1;
