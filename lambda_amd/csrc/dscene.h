@@ -409,7 +409,12 @@ LAM_FN bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_ins
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             float tn;
-            bool hit = CN[j] >= 0 && slab(LX[j], LY[j], LZ[j], HX[j], HY[j], HZ[j], ro, ri, tn) && !(tn > best);
+            // all four boxes are tested, empty slots too (their result is
+            // masked): with the test behind the slot check the compiler
+            // issues the box loads inside the branch, a second round trip
+            // to memory per node
+            const bool in = slab(LX[j], LY[j], LZ[j], HX[j], HY[j], HZ[j], ro, ri, tn);
+            bool hit = in & (CN[j] >= 0) & !(tn > best);
             key[j] = INFINITY;
             code[j] = 0;
             if (!hit) continue;
@@ -541,7 +546,12 @@ LAM_FN void traverse_pair(const DeviceScene &S, bool occl, V3 so, V3 sd, float s
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             float tn;
-            bool hit = CN[j] >= 0 && slab(LX[j], LY[j], LZ[j], HX[j], HY[j], HZ[j], ro, ri, tn) && !(tn > lim);
+            // all four boxes are tested, empty slots too (their result is
+            // masked): with the test behind the slot check the compiler
+            // issues the box loads inside the branch, a second round trip
+            // to memory per node
+            const bool in = slab(LX[j], LY[j], LZ[j], HX[j], HY[j], HZ[j], ro, ri, tn);
+            bool hit = in & (CN[j] >= 0) & !(tn > lim);
             key[j] = INFINITY;
             code[j] = 0;
             if (!hit) continue;
@@ -641,6 +651,137 @@ LAM_FN void traverse_pair(const DeviceScene &S, bool occl, V3 so, V3 sd, float s
         return;
     }
 }
+
+// Resumable form of traverse_pair's loop for kernels that interleave
+// traversal with other work: begin_*() sets a ray up, each step() visits one
+// wide node (and tests the objects of its hit leaves) and returns true once
+// the ray is resolved.  Same visiting order, pruning and tie rule as
+// traverse<kAny>, so the results are the same.
+struct RayWalk {
+    V3 o, d, inv;     // world ray
+    V3 ro, rd, ri;    // ray of the current level
+    float lim;        // closest hit so far, or the light distance
+    int hi, ho;       // closest hit so far (closest-hit mode)
+    int inst, blas_base, cur;
+    int skip_inst, skip_obj;
+    bool any, in_blas, blocked;
+
+    LAM_FN void begin(bool any_hit, V3 o_, V3 d_, float lim_, int skip_i, int skip_o) {
+        any = any_hit;
+        o = o_;
+        d = d_;
+        inv = v3(1 / d.x, 1 / d.y, 1 / d.z);
+        ro = o;
+        rd = d;
+        ri = inv;
+        lim = lim_;
+        hi = ho = -1;
+        inst = -1;
+        blas_base = 0;
+        cur = 0;
+        skip_inst = skip_i;
+        skip_obj = skip_o;
+        in_blas = false;
+        blocked = false;
+    }
+
+    template <int kRing>
+    LAM_FN bool step(const DeviceScene &S, TraversalStack<kRing> &stk) {
+        if (cur < 0) {
+            inst = -1 - cur;
+            local_ray(S, inst, o, d, ro, rd);
+            ri = v3(1 / rd.x, 1 / rd.y, 1 / rd.z);
+            in_blas = true;
+            blas_base = stk.sp;
+            cur = S.inst_root[inst];
+        }
+        const float4 *w = S.wide + 8 * size_t(cur);
+        const float4 lx = w[0], hx = w[1], ly = w[2], hy = w[3], lz = w[4], hz = w[5];
+        const float4 cw = w[6], nw = w[7];
+        const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, HX[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
+        const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
+        const int CH[4] = {as_int(cw.x), as_int(cw.y), as_int(cw.z), as_int(cw.w)};
+        const int CN[4] = {as_int(nw.x), as_int(nw.y), as_int(nw.z), as_int(nw.w)};
+        float key[4];
+        int code[4];
+        unsigned leaves = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            float tn;
+            // all four boxes are tested, empty slots too (their result is
+            // masked): with the test behind the slot check the compiler
+            // issues the box loads inside the branch, a second round trip
+            // to memory per node
+            const bool in = slab(LX[j], LY[j], LZ[j], HX[j], HY[j], HZ[j], ro, ri, tn);
+            bool hit = in & (CN[j] >= 0) & !(tn > lim);
+            key[j] = INFINITY;
+            code[j] = 0;
+            if (!hit) continue;
+            if (CN[j] == 0) {
+                key[j] = tn;
+                code[j] = CH[j];
+            } else if (!in_blas) {
+                key[j] = tn;
+                code[j] = -1 - CH[j];
+            } else {
+                leaves |= 1u << j;
+            }
+        }
+        int k = 0, end = 0;
+        while (true) {
+            if (k >= end) {
+                if (leaves == 0) break;
+                const int j = __builtin_ctz(leaves);
+                leaves &= leaves - 1;
+                k = j == 0 ? CH[0] : j == 1 ? CH[1] : j == 2 ? CH[2] : CH[3];
+                end = k + (j == 0 ? CN[0] : j == 1 ? CN[1] : j == 2 ? CN[2] : CN[3]);
+            }
+            const bool two = k + 1 < end;
+            const float4 *r0 = S.obj + 4 * size_t(k);
+            const float4 *r1 = two ? r0 + 4 : r0;
+            const float4 a0 = r0[0], b0 = r0[1], c0 = r0[2];
+            const float4 a1 = r1[0], b1 = r1[1], c1 = r1[2];
+            if (!(any && inst == skip_inst && k == skip_obj) &&
+                test_object_mode(S, any, k, a0, b0, c0, ro, rd, lim, inst, hi, ho) && any) {
+                blocked = true;
+                return true;
+            }
+            if (two && !(any && inst == skip_inst && k + 1 == skip_obj) &&
+                test_object_mode(S, any, k + 1, a1, b1, c1, ro, rd, lim, inst, hi, ho) && any) {
+                blocked = true;
+                return true;
+            }
+            k += 2;
+        }
+        sort4(key, code);
+        int m = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) m += !(key[j] > lim) ? 1 : 0;
+        if (m > 0) {
+            for (int j = m - 1; j >= 1; j--) stk.push(code[j], key[j]);
+            cur = code[0];
+            return false;
+        }
+        while (true) {
+            if (in_blas && stk.sp == blas_base) {
+                in_blas = false;
+                ro = o;
+                rd = d;
+                ri = inv;
+                continue;
+            }
+            if (stk.sp == 0) return true;
+            int c;
+            float t;
+            stk.pop(c, t);
+            if (!(t > lim)) {
+                cur = c;
+                return false;
+            }
+        }
+    }
+};
 
 // --------------------------------------------------------------- textures
 
