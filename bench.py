@@ -424,6 +424,9 @@ def main() -> int:
                     help="reference streams: time the first N samples of every pixel, then run the slowest tiles "
                          "first (0 automatic: N = 4; -1 off)")
     ap.add_argument("--graph-passes", type=int, default=0, help="capture the bounce loop in a hipGraph of this many passes")
+    ap.add_argument("--occupancy", type=int, default=0,
+                    help="with --no-autotune: compiled waves per SIMD of the persistent kernel (2, 3 or 4; 0: default)")
+    ap.add_argument("--tile", type=int, default=0, help="with --no-autotune: tile side of the cost order (0: default)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the library's default schedule instead of timing short renders first")
     ap.add_argument("--tune-cache", default=os.environ.get("LAMBDA_AUTOTUNE_CACHE", ""),
@@ -523,6 +526,10 @@ def main() -> int:
     tuned = None
     sched = {"path_slots": args.path_slots, "tail_paths": args.tail_paths, "cost_order": args.cost_order,
              "graph_passes": args.graph_passes}
+    if args.occupancy:
+        sched["occupancy"] = args.occupancy
+    if args.tile:
+        sched["tile"] = args.tile
     if not args.no_autotune:
         from lambda_amd import autotune
         t0 = time.perf_counter()

@@ -198,7 +198,7 @@ typedef struct lambda_render_params {
     int32_t tile;          /* side of the square pixel tiles of the visiting
                               order (4, 8 or 16; 0 = 8) */
     int32_t occupancy;     /* persistent kernel variant: waves per SIMD its
-                              registers are capped for (2, 3 or 4; 0 = 3) */
+                              registers are capped for (2, 3 or 4; 0 = 4) */
 } lambda_render_params;
 
 /*

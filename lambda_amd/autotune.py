@@ -28,7 +28,7 @@ CACHE = os.environ.get("LAMBDA_AUTOTUNE_CACHE", os.path.join(os.path.expanduser(
 
 def candidates(npix: int, rng_mode: int) -> list[dict]:
     """The search space: one dict of render-parameter overrides each."""
-    base = {"tail_paths": 0, "path_slots": 0, "cost_order": 4, "tile": 8, "occupancy": 3}
+    base = {"tail_paths": 0, "path_slots": 0, "cost_order": 4, "tile": 8, "occupancy": 4}
     c = [
         base,                                       # persistent, resident slots, slowest 8x8 tiles first
         dict(base, cost_order=8),                   # ... timed over the first 8 samples
@@ -36,7 +36,7 @@ def candidates(npix: int, rng_mode: int) -> list[dict]:
         dict(base, tile=4),                         # 4x4 tiles
         dict(base, tile=16),                        # 16x16 tiles
         dict(base, occupancy=2),                    # kernel variant for 2 waves per SIMD
-        dict(base, occupancy=4),                    # ... for 4
+        dict(base, occupancy=3),                    # ... for 3
         {"tail_paths": 65536, "path_slots": npix, "cost_order": -1, "tile": 8, "occupancy": 3},  # passes + tail
         {"tail_paths": -1, "path_slots": npix, "cost_order": -1, "graph_passes": 8, "tile": 8, "occupancy": 3},
     ]

@@ -47,6 +47,8 @@ struct DeviceScene {
     const int4 *worley_dim;       // [scalar textures] {nx, ny, nz, 0}; nx = 0: no table
     const int *tlas_rank;   // [instances] order of the instance's leaf in the reference's DFS
     const int *obj_rank;    // [objects] order within its object-level BVH in that DFS
+    const int4 *mat_tex;    // [materials] xtex record of the material's texture (Lambertian, mirror, emissive)
+    const float *tex_avg;   // [spectrum textures] mean over wavelength (spectrum_texture_average)
     float4 cam_pos, cam_corner, cam_h, cam_v;
     float total_power;
     int num_lights, bg_tex;
@@ -62,6 +64,20 @@ struct Hit {
     float u, v;
     bool front;
 };
+
+// Pins a loaded value's load where this is called: a load whose value is
+// used only inside a branch is moved into that branch by the compiler, where
+// it costs a memory round trip of its own.  Calling this after a group of
+// loads keeps them in flight together (one round trip for the group).
+LAM_FN void keep(float a, float b = 0, float c = 0, float d = 0) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d));
+#else
+    (void)a, (void)b, (void)c, (void)d;
+#endif
+}
+LAM_FN void keep(int a) { keep(as_float(a)); }
+LAM_FN void keep(float4 v) { keep(v.x, v.y, v.z, v.w); }
 
 LAM_FN float dense(const float *s, float lambda) {
     // 1 nm table, linear interpolation (math/spectrum.h:170-188) in float.
@@ -138,11 +154,10 @@ LAM_FN bool test_tri(V3 c, V3 h, V3 v, V3 o, V3 d, float best, float &t_out, flo
     return true;
 }
 
-// Full intersection record of object k for a local-space ray, as the
-// reference's intersect* fills it (scene/object.h:301-400).
-LAM_FN bool object_record(const DeviceScene &S, int k, V3 o, V3 d, float best, Hit &h) {
-    float4 a = S.obj[4 * size_t(k)];
-    float4 b = S.obj[4 * size_t(k) + 1];
+// object_record from the object's records already loaded: R = obj[4k ..
+// 4k + 3], SH = its shading record.
+LAM_FN bool object_record_rec(const float4 (&R)[4], const float4 (&SH)[4], int k, V3 o, V3 d, float best, Hit &h) {
+    const float4 a = R[0], b = R[1];
     int type = as_int(b.w);
     float t, al = 0, be = 0;
     if (type == LAMBDA_OBJECT_SPHERE) {
@@ -154,13 +169,11 @@ LAM_FN bool object_record(const DeviceScene &S, int k, V3 o, V3 d, float best, H
         h.v = acosf(-h.n.y) / kPi;
         h.front = dot(d, h.n) < 0;
     } else {
-        float4 c = S.obj[4 * size_t(k) + 2];
-        float4 nn = S.obj[4 * size_t(k) + 3];
+        const float4 c = R[2], nn = R[3];
         V3 nrm = v3(nn);
         if (type == LAMBDA_OBJECT_TRI) {
             if (!test_tri(v3(a), v3(b), v3(c), o, d, best, t, al, be)) return false;
-            const float4 *sh = S.obj_s + 4 * size_t(k);
-            float4 s0 = sh[0], s1 = sh[1], s2 = sh[2], s3 = sh[3];
+            const float4 s0 = SH[0], s1 = SH[1], s2 = SH[2], s3 = SH[3];
             V3 n0 = v3(s0.x, s0.y, s0.z), n1 = v3(s0.w, s1.x, s1.y), n2 = v3(s1.z, s1.w, s2.x);
             float w = 1 - al - be;
             h.t = t;
@@ -184,6 +197,23 @@ LAM_FN bool object_record(const DeviceScene &S, int k, V3 o, V3 d, float best, H
     return true;
 }
 
+LAM_FN void load_object(const DeviceScene &S, int k, float4 (&R)[4], float4 (&SH)[4]) {
+    const float4 *r = S.obj + 4 * size_t(k), *sh = S.obj_s + 4 * size_t(k);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        R[i] = r[i];
+        SH[i] = sh[i];
+    }
+}
+
+// Full intersection record of object k for a local-space ray, as the
+// reference's intersect* fills it (scene/object.h:301-400).
+LAM_FN bool object_record(const DeviceScene &S, int k, V3 o, V3 d, float best, Hit &h) {
+    float4 R[4], SH[4];
+    load_object(S, k, R, SH);
+    return object_record_rec(R, SH, k, o, d, best, h);
+}
+
 LAM_FN void local_ray(const DeviceScene &S, int inst, V3 o, V3 d, V3 &lo, V3 &ld) {
     const float4 *m = S.inst + 8 * size_t(inst);
     lo = xform(m + 4, o, 1);
@@ -192,14 +222,20 @@ LAM_FN void local_ray(const DeviceScene &S, int inst, V3 o, V3 d, V3 &lo, V3 &ld
 
 // World-space record of (instance, object) for a world ray: local test, then
 // point by M and normal by the transposed inverse (scene/scene.h:101-104).
+// Every record of (instance, object) is requested at once: one round trip
+// to memory instead of one per dependent step.
 LAM_FN bool hit_record(const DeviceScene &S, int inst, int obj, V3 o, V3 d, Hit &h) {
-    V3 lo, ld;
-    local_ray(S, inst, o, d, lo, ld);
-    if (!object_record(S, obj, lo, ld, kMax, h)) return false;
-    const float4 *m = S.inst + 8 * size_t(inst);
+    const float4 *mp = S.inst + 8 * size_t(inst);
+    float4 M[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) M[i] = mp[i];
+    float4 R[4], SH[4];
+    load_object(S, obj, R, SH);
+    const V3 lo = xform(M + 4, o, 1), ld = xform(M + 4, d, 0);
+    if (!object_record_rec(R, SH, obj, lo, ld, kMax, h)) return false;
     h.inst = inst;
-    h.p = xform(m, h.p, 1);
-    h.n = normalize(xform_t(m + 4, h.n, 0));
+    h.p = xform(M, h.p, 1);
+    h.n = normalize(xform_t(M + 4, h.n, 0));
     return true;
 }
 
@@ -836,13 +872,17 @@ LAM_FN bool spectrum_texture_is_scalar(const DeviceScene &S, int t) {
 // The four wavelengths of a vertex; a scalar texture is evaluated once (it
 // does not depend on the wavelength), so only one copy of its code is
 // inlined per call site.
-LAM_FN S4 spectrum_texture4(const DeviceScene &S, int t, const Hit &h, const S4 &lam) {
+// x: the texture's xtex record.
+LAM_FN S4 spectrum_texture4_rec(const DeviceScene &S, int4 x, const Hit &h, const S4 &lam) {
     S4 r;
-    const int4 x = S.xtex[t];
     if (x.x == LAMBDA_SPECTRUM_SCALAR) return s4(scalar_texture(S, x.y, h.p));
 #pragma unroll
     for (int j = 0; j < kHero; j++) r.v[j] = spectrum_texture_tabulated(S, x, h, lam.v[j]);
     return r;
+}
+
+LAM_FN S4 spectrum_texture4(const DeviceScene &S, int t, const Hit &h, const S4 &lam) {
+    return spectrum_texture4_rec(S, S.xtex[t], h, lam);
 }
 
 // Environment lookup by direction: equirectangular (u, v), point = direction

@@ -135,7 +135,9 @@ __device__ __forceinline__ Pcg load_rng(const Paths &P, int s) {
 constexpr int kRing = 16;
 // Waves per SIMD the persistent kernel is compiled for (its registers are
 // capped so that this many fit; measured best on gfx950).
-constexpr int kPersistWaves = 3;
+constexpr int kPersistWaves = 4;
+// Waves per SIMD of the serving and table-guided persistent kernels.
+constexpr int kPersistOtherWaves = 3;
 // Automatic schedule: live paths below which a render switches to the
 // persistent schedule (above any slot count: persistent from the first pass
 // on).  Slots default to the number of persistent threads resident at once.
@@ -246,7 +248,8 @@ constexpr float kGuideFracMax = 0.95f;
 template <bool kGuided = false>
 __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F, const Paths &P, int s,
                                            bool &push_shadow, ShadowRay &sh, const float *gout = nullptr,
-                                           const float *gfeat = nullptr, const GuideTableDev *table = nullptr) {
+                                           const float *gfeat = nullptr, const GuideTableDev *table = nullptr,
+                                           const int2 *hit_now = nullptr) {
     push_shadow = false;
     {
         float4 o4 = P.o[s], d4 = P.d[s];
@@ -258,7 +261,8 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                 P.grec[(size_t(s) * F.grec_stride + info.z - 1) * 3 + 2].w = P.L[s].x;
             info.w &= ~kFlagGuidePending;
         }
-        int2 hit = P.hit[s];
+        // the caller may hand over the hit it has just found (and stored)
+        int2 hit = hit_now ? *hit_now : P.hit[s];
         S4 lam = s4(P.lam[s]);
         S4 beta = s4(P.beta[s]);
         S4 L = s4(P.L[s]);
@@ -281,21 +285,27 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
             for (int k = 0; k < kHero; k++) L.v[k] += c.v[k];
             finished = true;
         } else {
+            // the material indices are requested with the hit's records
+            const int inst_m = S.inst_i[hit.x].w, obj_m = S.obj_mat[hit.y];
             Hit h;
-            if (!hit_record(S, hit.x, hit.y, o, d, h)) {
+            const bool rec = hit_record(S, hit.x, hit.y, o, d, h);
+            keep(inst_m);
+            keep(obj_m);
+            if (!rec) {
                 atomicOr(&P.ctr[kCtrError], 1);
                 finished = true;
             } else {
-                const int mat = material_of(S, hit.x, hit.y);
+                const int mat = inst_m >= 0 ? inst_m : obj_m;   // material_of
                 const int4 mt = S.mat[mat];
+                const int4 mtex = S.mat_tex[mat];
                 if (mt.x == LAMBDA_MATERIAL_EMISSIVE) {
                     // Emitter hit (render/renderer.h:119-131).
                     float w = 1;
                     if (!specular && total > 0) {
-                        float lp = instance_area(S, hit.x, hit.y) * spectrum_texture_average(S, mt.y);
+                        float lp = instance_area(S, hit.x, hit.y) * S.tex_avg[mt.y];
                         w = power_heuristic(prev_pdf, instance_pdf(S, hit.x, hit.y, o, d) * lp / total);
                     }
-                    S4 e = spectrum_texture4(S, mt.y, h, lam);
+                    S4 e = spectrum_texture4_rec(S, mtex, h, lam);
                     S4 c = beta * e * w;
 #pragma unroll
                     for (int k = 0; k < kHero; k++) L.v[k] += c.v[k];
@@ -303,7 +313,7 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                 } else {
                     const bool lambertian = mt.x == LAMBDA_MATERIAL_LAMBERTIAN;
                     S4 albedo = s4(0);
-                    if (lambertian) albedo = lambertian_albedo(S, mt, h, lam);
+                    if (lambertian) albedo = spectrum_texture4_rec(S, mtex, h, lam);
                     if constexpr (kGuided) {
                         if (table && lambertian && bounce < F.guide_bounces)
                             gout = table->rows + guide_table_entry(*table, h.p, h.n) * kLobeOut;
@@ -378,7 +388,7 @@ __device__ __forceinline__ bool shade_slot(const DeviceScene &S, const Frame &F,
                         att = dot(h.n, nd) <= 0 ? s4(0) : albedo / kPi;
                     } else if (mt.x == LAMBDA_MATERIAL_MIRROR) {
                         nd = reflected(d, h.n);
-                        att = spectrum_texture4(S, mt.y, h, lam);
+                        att = spectrum_texture4_rec(S, mtex, h, lam);
                     } else if (mt.x == LAMBDA_MATERIAL_DIELECTRIC) {
                         float factor = powf(1 + dot(d, h.n), 5.0f);
                         const float *s0 = S.spectra + size_t(mt.y) * kBins;

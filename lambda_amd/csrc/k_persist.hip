@@ -1,4 +1,4 @@
-// The persistent per-thread schedule, default variant (3 waves per SIMD).
+// The persistent per-thread schedule, default variant (4 waves per SIMD).
 #include "launch.h"
 #include "persist_kernel.h"
 

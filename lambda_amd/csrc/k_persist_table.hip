@@ -10,7 +10,7 @@ namespace {
 
 // k_persist with guided vertices: shading looks up the table entry of each
 // guided vertex (Lambertian, bounce index below F.guide_bounces).
-__global__ void __launch_bounds__(kBlock, kPersistWaves) k_persist_table(DeviceScene S, Frame F, Paths P, int n_in,
+__global__ void __launch_bounds__(kBlock, kPersistOtherWaves) k_persist_table(DeviceScene S, Frame F, Paths P, int n_in,
                                                                          int2 *spill, GuideTableDev T) {
     __shared__ int s_code[kRing * kBlock];
     __shared__ float s_dist[kRing * kBlock];

@@ -5,7 +5,7 @@
 namespace lam {
 namespace {
 
-__global__ void __launch_bounds__(kBlock, kPersistWaves) k_serve(const DeviceScene *scenes, const ServeJob *jobs,
+__global__ void __launch_bounds__(kBlock, kPersistOtherWaves) k_serve(const DeviceScene *scenes, const ServeJob *jobs,
                                                                   int njobs, long long total_units, Paths P,
                                                                   ServeCtl *ctl, int *done, unsigned long long *finish,
                                                                   int2 *spill) {

@@ -172,8 +172,8 @@ struct lambda_hip_scene {
     float box[6] = {0, 0, 0, 0, 0, 0};  // scene bounds (instance-level root)
     DevBuf<float4> worley_points;
     DevBuf<int4> worley_box, worley_dim;
-    DevBuf<int4> inst_i, mat, xtex;
-    DevBuf<float> spectra, light_pow, cie;
+    DevBuf<int4> inst_i, mat, xtex, mat_tex;
+    DevBuf<float> spectra, light_pow, cie, tex_avg;
     DevBuf<float2> cspec;
     DevBuf<int2> light_io;
     DeviceScene ds{};
@@ -233,6 +233,39 @@ void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g, int max_depth
         std::memcpy(&sc, v.spectrum_texture_scale + i, 4);
         xt[i] = make_int4(v.spectrum_texture_type[i], v.spectrum_texture_index[2 * i], v.spectrum_texture_index[2 * i + 1], sc);
     }
+    // per material, the xtex record of its texture (the materials whose
+    // first parameter is a spectrum texture), so that shading reads it with
+    // the material record
+    std::vector<int4> mtx(size_t(v.num_materials), make_int4(0, 0, 0, 0));
+    for (int i = 0; i < v.num_materials; i++) {
+        const int t = v.material_type[i];
+        if (t == LAMBDA_MATERIAL_LAMBERTIAN || t == LAMBDA_MATERIAL_MIRROR || t == LAMBDA_MATERIAL_EMISSIVE)
+            mtx[size_t(i)] = xt[size_t(v.material_params[3 * i])];
+    }
+    // mean of each spectrum texture over the 471 bins, summed in float in
+    // bin order as the device formula (spectrum_texture_average) does
+    std::vector<float> tavg(size_t(v.num_spectrum_textures));
+    {
+        auto spectrum_mean = [&](int k) {
+            const float *p = v.spectra + size_t(k) * kBins;
+            float sum = 0;
+            for (int i = 0; i < kBins; i++) sum += p[i];
+            return sum / kBins;
+        };
+        for (int i = 0; i < v.num_spectrum_textures; i++) {
+            const int4 x = xt[size_t(i)];
+            if (x.x == LAMBDA_SPECTRUM_CONSTANT) {
+                tavg[size_t(i)] = spectrum_mean(x.y);
+            } else if (x.x == LAMBDA_SPECTRUM_CHECKER) {
+                tavg[size_t(i)] = float(0.5) * (spectrum_mean(x.y) + spectrum_mean(x.z));
+            } else {
+                const float4 q = st[size_t(x.y)];
+                int qt;
+                std::memcpy(&qt, &q.x, 4);
+                tavg[size_t(i)] = qt == LAMBDA_SCALAR_CONSTANT ? q.y : (q.y + q.z) / 2;
+            }
+        }
+    }
     std::vector<float2> cs(size_t(v.num_complex_spectra) * kBins);
     for (size_t i = 0; i < cs.size(); i++)
         cs[i] = make_float2(float(v.complex_spectra[2 * i]), float(v.complex_spectra[2 * i + 1]));
@@ -252,6 +285,8 @@ void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g, int max_depth
     g.mat.upload(mt);
     g.stex.upload(st);
     g.xtex.upload(xt);
+    g.mat_tex.upload(mtx);
+    g.tex_avg.upload(tavg);
     g.spectra.upload(v.spectra, size_t(v.num_spectra) * kBins);
     if (v.num_complex_spectra) g.cspec.upload(cs);
     if (v.num_material_properties) g.props.upload(v.material_properties, size_t(v.num_material_properties));
@@ -316,6 +351,8 @@ void upload_scene(const lambda_scene_view &v, lambda_hip_scene &g, int max_depth
     d.mat = g.mat.p;
     d.stex = g.stex.p;
     d.xtex = g.xtex.p;
+    d.mat_tex = g.mat_tex.p;
+    d.tex_avg = g.tex_avg.p;
     d.spectra = g.spectra.p;
     d.cspectra = g.cspec.p;
     d.props = g.props.p;
@@ -381,7 +418,7 @@ void validate_params(const lambda_render_params &p) {
     if (p.tile != 0 && p.tile != 4 && p.tile != 8 && p.tile != 16)
         throw Failure{LAMBDA_EINVAL, "invalid tile (4, 8 or 16; 0 = 8)"};
     if (p.occupancy != 0 && (p.occupancy < 2 || p.occupancy > 4))
-        throw Failure{LAMBDA_EINVAL, "invalid occupancy (2, 3 or 4 waves per SIMD; 0 = 3)"};
+        throw Failure{LAMBDA_EINVAL, "invalid occupancy (2, 3 or 4 waves per SIMD; 0 = 4)"};
     if ((long long)p.width * p.height > (1LL << 28)) throw Failure{LAMBDA_EINVAL, "image too large"};
     // work-unit counters are 32-bit and may run past the last unit by the
     // number of slots (at most 2^28)

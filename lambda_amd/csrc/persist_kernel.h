@@ -36,6 +36,7 @@ __global__ void __launch_bounds__(kBlock, kWaves) k_persist(DeviceScene S, Frame
     bool alive = true;   // the path has a closest-hit ray to trace
     ShadowRay sh{};
     while (true) {
+        int2 hit = make_int2(-1, -1);
         {
             bool blocked;
             float best;
@@ -56,12 +57,13 @@ __global__ void __launch_bounds__(kBlock, kWaves) k_persist(DeviceScene S, Frame
             }
             if (alive) {
                 n_closest++;
-                P.hit[s] = make_int2(hi, ho);
+                hit = make_int2(hi, ho);
+                P.hit[s] = hit;
             }
         }
         if (alive) {
             bool push = false;
-            const bool ended = shade_slot(S, F, P, s, push, sh);
+            const bool ended = shade_slot(S, F, P, s, push, sh, nullptr, nullptr, nullptr, &hit);
             pend = push;
             if (!ended) continue;
             if (pend) {
