@@ -136,8 +136,10 @@ constexpr int kRing = 16;
 // Waves per SIMD the persistent kernel is compiled for (its registers are
 // capped so that this many fit; measured best on gfx950).
 constexpr int kPersistWaves = 4;
-// Waves per SIMD of the serving and table-guided persistent kernels.
-constexpr int kPersistOtherWaves = 3;
+// Waves per SIMD of the table-guided persistent kernel and of the serving
+// kernel (measured on gfx950: 4 and 3 best).
+constexpr int kPersistTableWaves = 4;
+constexpr int kServeWaves = 3;
 // Automatic schedule: live paths below which a render switches to the
 // persistent schedule (above any slot count: persistent from the first pass
 // on).  Slots default to the number of persistent threads resident at once.

@@ -10,7 +10,7 @@ namespace {
 
 // k_persist with guided vertices: shading looks up the table entry of each
 // guided vertex (Lambertian, bounce index below F.guide_bounces).
-__global__ void __launch_bounds__(kBlock, kPersistOtherWaves) k_persist_table(DeviceScene S, Frame F, Paths P, int n_in,
+__global__ void __launch_bounds__(kBlock, kPersistTableWaves) k_persist_table(DeviceScene S, Frame F, Paths P, int n_in,
                                                                          int2 *spill, GuideTableDev T) {
     __shared__ int s_code[kRing * kBlock];
     __shared__ float s_dist[kRing * kBlock];
@@ -20,39 +20,53 @@ __global__ void __launch_bounds__(kBlock, kPersistOtherWaves) k_persist_table(De
     const int s = t < n_in ? P.q_in[t] : P.q_out[t - n_in];
     Stack stk = make_stack(s_code, s_dist, spill);
     unsigned n_closest = 0, n_shadow = 0;
+    // the loop of k_persist: occlusion ray and next closest hit in one
+    // traversal, radiance summed in the reference's order
+    bool pend = false;   // an occlusion ray of the previous vertex waits
+    bool alive = true;   // the path has a closest-hit ray to trace
+    ShadowRay sh{};
     while (true) {
-        n_closest++;
+        int2 hit = make_int2(-1, -1);
         {
-            float4 o = P.o[s], d = P.d[s];
-            float best = kMax;
-            int hi = -1, ho = -1;
-            stk.sp = stk.lo = 0;
-            trace_ray<false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk);
-            P.hit[s] = make_int2(hi, ho);
-            // a record left by an earlier sample in this slot is void until
-            // shading writes this vertex's
-            const int bounce = P.info[s].z;
-            if (bounce < F.grec_stride && bounce < F.guide_bounces)
-                P.grec[(size_t(s) * F.grec_stride + bounce) * 3 + 2].w = kRecNone;
-        }
-        bool push = false;
-        ShadowRay sh;
-        const bool ended = shade_slot<true>(S, F, P, s, push, sh, nullptr, nullptr, &T);
-        if (push) {
-            n_shadow++;
-            float best = sh.o.w;
-            int hi = -1, ho = -1;
-            stk.sp = stk.lo = 0;
-            if (!trace_ray<true>(S, v3(sh.o), v3(sh.d), best, hi, ho, sh.i.y, sh.i.z, stk)) {
-                float4 L = P.L[s];
-                L.x += sh.c.x;
-                L.y += sh.c.y;
-                L.z += sh.c.z;
-                L.w += sh.c.w;
-                P.L[s] = L;
+            bool blocked;
+            float best;
+            int hi, ho;
+            traverse_pair<kRing>(S, pend, v3(sh.o), v3(sh.d), sh.o.w, sh.i.y, sh.i.z, blocked, alive, P.o + s,
+                                 P.d + s, best, hi, ho, stk);
+            if (pend) {
+                n_shadow++;
+                if (!blocked) {
+                    float4 L = P.L[s];
+                    L.x += sh.c.x;
+                    L.y += sh.c.y;
+                    L.z += sh.c.z;
+                    L.w += sh.c.w;
+                    P.L[s] = L;
+                }
+                pend = false;
+            }
+            if (alive) {
+                n_closest++;
+                hit = make_int2(hi, ho);
+                P.hit[s] = hit;
+                // a record left by an earlier sample in this slot is void
+                // until shading writes this vertex's
+                const int bounce = P.info[s].z;
+                if (bounce < F.grec_stride && bounce < F.guide_bounces)
+                    P.grec[(size_t(s) * F.grec_stride + bounce) * 3 + 2].w = kRecNone;
             }
         }
-        if (!ended) continue;
+        if (alive) {
+            bool push = false;
+            const bool ended = shade_slot<true>(S, F, P, s, push, sh, nullptr, nullptr, &T, &hit);
+            pend = push;
+            if (!ended) continue;
+            if (pend) {
+                alive = false;
+                continue;
+            }
+        }
+        alive = true;
         if (retire_sample(S, F, P, s)) continue;
         const int w = atomicAdd(&P.ctr[kCtrWork], 1);
         if (w >= work_units(F)) break;

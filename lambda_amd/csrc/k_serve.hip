@@ -5,7 +5,7 @@
 namespace lam {
 namespace {
 
-__global__ void __launch_bounds__(kBlock, kPersistOtherWaves) k_serve(const DeviceScene *scenes, const ServeJob *jobs,
+__global__ void __launch_bounds__(kBlock, kServeWaves) k_serve(const DeviceScene *scenes, const ServeJob *jobs,
                                                                   int njobs, long long total_units, Paths P,
                                                                   ServeCtl *ctl, int *done, unsigned long long *finish,
                                                                   int2 *spill) {
@@ -34,34 +34,47 @@ __global__ void __launch_bounds__(kBlock, kPersistOtherWaves) k_serve(const Devi
             camera_sample(S, F, Pj, s, pixel, 0, rng);
             store_rng(Pj, s, rng);
         }
+        // the pixel's sample chain, as k_persist runs it
+        bool pend = false;
+        bool alive = true;
+        ShadowRay sh{};
         while (true) {
-            n_closest++;
+            int2 hit = make_int2(-1, -1);
             {
-                float4 o = Pj.o[s], d = Pj.d[s];
-                float best = kMax;
-                int hi = -1, ho = -1;
-                stk.sp = stk.lo = 0;
-                trace_ray<false>(S, v3(o), v3(d), best, hi, ho, -1, -2, stk);
-                Pj.hit[s] = make_int2(hi, ho);
-            }
-            bool push = false;
-            ShadowRay sh;
-            const bool ended = shade_slot(S, F, Pj, s, push, sh);
-            if (push) {
-                n_shadow++;
-                float best = sh.o.w;
-                int hi = -1, ho = -1;
-                stk.sp = stk.lo = 0;
-                if (!trace_ray<true>(S, v3(sh.o), v3(sh.d), best, hi, ho, sh.i.y, sh.i.z, stk)) {
-                    float4 L = Pj.L[s];
-                    L.x += sh.c.x;
-                    L.y += sh.c.y;
-                    L.z += sh.c.z;
-                    L.w += sh.c.w;
-                    Pj.L[s] = L;
+                bool blocked;
+                float best;
+                int hi, ho;
+                traverse_pair<kRing>(S, pend, v3(sh.o), v3(sh.d), sh.o.w, sh.i.y, sh.i.z, blocked, alive, Pj.o + s,
+                                     Pj.d + s, best, hi, ho, stk);
+                if (pend) {
+                    n_shadow++;
+                    if (!blocked) {
+                        float4 L = Pj.L[s];
+                        L.x += sh.c.x;
+                        L.y += sh.c.y;
+                        L.z += sh.c.z;
+                        L.w += sh.c.w;
+                        Pj.L[s] = L;
+                    }
+                    pend = false;
+                }
+                if (alive) {
+                    n_closest++;
+                    hit = make_int2(hi, ho);
+                    Pj.hit[s] = hit;
                 }
             }
-            if (!ended) continue;
+            if (alive) {
+                bool push = false;
+                const bool ended = shade_slot(S, F, Pj, s, push, sh, nullptr, nullptr, nullptr, &hit);
+                pend = push;
+                if (!ended) continue;
+                if (pend) {
+                    alive = false;
+                    continue;
+                }
+            }
+            alive = true;
             if (retire_sample(S, F, Pj, s)) continue;
             break;
         }
