@@ -290,33 +290,10 @@ LAM_FN uint64_t ref_rank(const DeviceScene &S, int inst, int obj) {
     return (uint64_t(uint32_t(S.tlas_rank[inst])) << 32) | uint64_t(uint32_t(S.obj_rank[obj]));
 }
 
-// Test of object k, whose record starts {a, b, c}, against the (local) ray.
-// A closer hit always wins; a hit at exactly the current distance wins when
-// it comes first in the reference's order (closest-hit mode only).
-template <bool kAny>
-LAM_FN bool test_object_ranked(const DeviceScene &S, int k, float4 a, float4 b, float4 c, V3 o, V3 d, float &best,
-                               int inst, int &hit_inst, int &hit_obj) {
-    int type = as_int(b.w);
-    // accept t <= best here, decide ties below
-    float limit = kAny ? best : nextafterf(best, INFINITY);
-    float t, al, be;
-    if (type == LAMBDA_OBJECT_SPHERE) {
-        if (!test_sphere(a, o, d, limit, t)) return false;
-    } else if (type == LAMBDA_OBJECT_TRI) {
-        if (!test_tri(v3(a), v3(b), v3(c), o, d, limit, t, al, be)) return false;
-    } else {
-        float4 n = S.obj[4 * size_t(k) + 3];
-        if (!test_quad(v3(a), v3(b), v3(c), v3(n), o, d, limit, t, al, be)) return false;
-    }
-    if (!kAny && t == best && hit_obj >= 0 && ref_rank(S, inst, k) > ref_rank(S, hit_inst, hit_obj))
-        return false;
-    best = t;
-    hit_inst = inst;
-    hit_obj = k;
-    return true;
-}
-
-// test_object_ranked with the mode chosen at run time.
+// Test of object k, whose record starts {a, b, c}, against the (local) ray;
+// any = true: occlusion mode (any hit within best counts), else closest-hit
+// mode: a closer hit always wins; a hit at exactly the current distance wins
+// when it comes first in the reference's order.
 LAM_FN bool test_object_mode(const DeviceScene &S, bool any, int k, float4 a, float4 b, float4 c, V3 o, V3 d,
                              float &best, int inst, int &hit_inst, int &hit_obj) {
     int type = as_int(b.w);
@@ -414,106 +391,7 @@ LAM_FN void sort4(float (&t)[4], int (&c)[4]) {
 //   the light itself; returns true on the first blocking hit.
 template <bool kAny, int kRing>
 LAM_FN bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_inst, int &hit_obj, int skip_inst,
-                     int skip_obj, TraversalStack<kRing> &stk) {
-    const V3 inv = v3(1 / d.x, 1 / d.y, 1 / d.z);
-    bool found = false;
-    bool in_blas = false;
-    int blas_base = 0;
-    int inst = -1;
-    V3 ro = o, rd = d, ri = inv;   // ray of the current level
-    int cur = 0;                   // wide node (>= 0) or instance (< 0) to visit
-    while (true) {
-        if (cur < 0) {
-            // enter an instance: its object-level BVH in local space
-            inst = -1 - cur;
-            local_ray(S, inst, o, d, ro, rd);
-            ri = v3(1 / rd.x, 1 / rd.y, 1 / rd.z);
-            in_blas = true;
-            blas_base = stk.sp;
-            cur = S.inst_root[inst];
-        }
-        const float4 *w = S.wide + 8 * size_t(cur);
-        const float4 lx = w[0], hx = w[1], ly = w[2], hy = w[3], lz = w[4], hz = w[5];
-        const float4 cw = w[6], nw = w[7];
-        const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, HX[4] = {hx.x, hx.y, hx.z, hx.w};
-        const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
-        const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
-        const int CH[4] = {as_int(cw.x), as_int(cw.y), as_int(cw.z), as_int(cw.w)};
-        const int CN[4] = {as_int(nw.x), as_int(nw.y), as_int(nw.z), as_int(nw.w)};
-        float key[4];
-        int code[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            float tn;
-            // all four boxes are tested, empty slots too (their result is
-            // masked): with the test behind the slot check the compiler
-            // issues the box loads inside the branch, a second round trip
-            // to memory per node
-            const bool in = slab(LX[j], LY[j], LZ[j], HX[j], HY[j], HZ[j], ro, ri, tn);
-            bool hit = in & (CN[j] >= 0) & !(tn > best);
-            key[j] = INFINITY;
-            code[j] = 0;
-            if (!hit) continue;
-            if (CN[j] == 0) {
-                key[j] = tn;
-                code[j] = CH[j];
-            } else if (!in_blas) {
-                key[j] = tn;
-                code[j] = -1 - CH[j];
-            } else {
-                // objects two at a time, both records requested before
-                // either is tested (a lone last object is read twice)
-                const int end = CH[j] + CN[j];
-                for (int k = CH[j]; k < end; k += 2) {
-                    const bool two = k + 1 < end;
-                    const float4 *r0 = S.obj + 4 * size_t(k);
-                    const float4 *r1 = two ? r0 + 4 : r0;
-                    const float4 a0 = r0[0], b0 = r0[1], c0 = r0[2];
-                    const float4 a1 = r1[0], b1 = r1[1], c1 = r1[2];
-                    if (!(kAny && inst == skip_inst && k == skip_obj) &&
-                        test_object_ranked<kAny>(S, k, a0, b0, c0, ro, rd, best, inst, hit_inst, hit_obj)) {
-                        if (kAny) return true;
-                        found = true;
-                    }
-                    if (two && !(kAny && inst == skip_inst && k + 1 == skip_obj) &&
-                        test_object_ranked<kAny>(S, k + 1, a1, b1, c1, ro, rd, best, inst, hit_inst, hit_obj)) {
-                        if (kAny) return true;
-                        found = true;
-                    }
-                }
-            }
-        }
-        sort4(key, code);
-        // entries beyond a hit found at this node are dropped
-        int m = 0;
-#pragma unroll
-        for (int j = 0; j < 4; j++) m += !(key[j] > best) ? 1 : 0;
-        if (m > 0) {
-            for (int j = m - 1; j >= 1; j--) stk.push(code[j], key[j]);
-            cur = code[0];
-            continue;
-        }
-        // pop the next entry still worth visiting
-        bool next = false;
-        while (!next) {
-            if (in_blas && stk.sp == blas_base) {
-                in_blas = false;
-                ro = o;
-                rd = d;
-                ri = inv;
-                continue;
-            }
-            if (stk.sp == 0) return found;
-            int c;
-            float t;
-            stk.pop(c, t);
-            if (!(t > best)) {
-                cur = c;
-                next = true;
-            }
-        }
-    }
-}
+                     int skip_obj, TraversalStack<kRing> &stk);
 
 // One lane's occlusion ray (when `occl`) and then its closest-hit ray (when
 // `closest`) in one loop.  A lane moves on to its closest-hit ray as soon
@@ -531,162 +409,7 @@ LAM_FN bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_ins
 template <int kRing>
 LAM_FN void traverse_pair(const DeviceScene &S, bool occl, V3 so, V3 sd, float s_dist, int skip_inst, int skip_obj,
                           bool &blocked, bool closest, const float4 *co, const float4 *cd, float &best,
-                          int &hit_inst, int &hit_obj, TraversalStack<kRing> &stk) {
-    blocked = false;
-    hit_inst = hit_obj = -1;
-    best = kMax;
-    if (!occl && !closest) return;
-    bool any = occl;
-    V3 o, d;
-    float lim;
-    if (any) {
-        o = so;
-        d = sd;
-        lim = s_dist;
-    } else {
-        o = v3(*co);
-        d = v3(*cd);
-        lim = kMax;
-    }
-    V3 inv = v3(1 / d.x, 1 / d.y, 1 / d.z);
-    int hi = -1, ho = -1;
-    bool in_blas = false;
-    int blas_base = 0;
-    int inst = -1;
-    V3 ro = o, rd = d, ri = inv;
-    int cur = 0;
-    stk.sp = stk.lo = 0;
-    while (true) {
-        if (cur < 0) {
-            inst = -1 - cur;
-            local_ray(S, inst, o, d, ro, rd);
-            ri = v3(1 / rd.x, 1 / rd.y, 1 / rd.z);
-            in_blas = true;
-            blas_base = stk.sp;
-            cur = S.inst_root[inst];
-        }
-        const float4 *w = S.wide + 8 * size_t(cur);
-        const float4 lx = w[0], hx = w[1], ly = w[2], hy = w[3], lz = w[4], hz = w[5];
-        const float4 cw = w[6], nw = w[7];
-        const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, HX[4] = {hx.x, hx.y, hx.z, hx.w};
-        const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
-        const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
-        const int CH[4] = {as_int(cw.x), as_int(cw.y), as_int(cw.z), as_int(cw.w)};
-        const int CN[4] = {as_int(nw.x), as_int(nw.y), as_int(nw.z), as_int(nw.w)};
-        float key[4];
-        int code[4];
-        // box tests of the four slots; the hit object leaves are noted in
-        // `leaves` and their objects tested afterwards in one loop, so that
-        // lanes whose leaves sit in different slots test them together
-        unsigned leaves = 0;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            float tn;
-            // all four boxes are tested, empty slots too (their result is
-            // masked): with the test behind the slot check the compiler
-            // issues the box loads inside the branch, a second round trip
-            // to memory per node
-            const bool in = slab(LX[j], LY[j], LZ[j], HX[j], HY[j], HZ[j], ro, ri, tn);
-            bool hit = in & (CN[j] >= 0) & !(tn > lim);
-            key[j] = INFINITY;
-            code[j] = 0;
-            if (!hit) continue;
-            if (CN[j] == 0) {
-                key[j] = tn;
-                code[j] = CH[j];
-            } else if (!in_blas) {
-                key[j] = tn;
-                code[j] = -1 - CH[j];
-            } else {
-                leaves |= 1u << j;
-            }
-        }
-        bool done = false;   // the current ray is resolved
-        // objects two at a time, both records requested before either is
-        // tested (a lone last object is read twice)
-        int k = 0, end = 0;
-        while (true) {
-            if (k >= end) {
-                if (leaves == 0) break;
-                const int j = __builtin_ctz(leaves);
-                leaves &= leaves - 1;
-                k = j == 0 ? CH[0] : j == 1 ? CH[1] : j == 2 ? CH[2] : CH[3];
-                end = k + (j == 0 ? CN[0] : j == 1 ? CN[1] : j == 2 ? CN[2] : CN[3]);
-            }
-            const bool two = k + 1 < end;
-            const float4 *r0 = S.obj + 4 * size_t(k);
-            const float4 *r1 = two ? r0 + 4 : r0;
-            const float4 a0 = r0[0], b0 = r0[1], c0 = r0[2];
-            const float4 a1 = r1[0], b1 = r1[1], c1 = r1[2];
-            if (!(any && inst == skip_inst && k == skip_obj) &&
-                test_object_mode(S, any, k, a0, b0, c0, ro, rd, lim, inst, hi, ho) && any) {
-                done = true;
-                break;
-            }
-            if (two && !(any && inst == skip_inst && k + 1 == skip_obj) &&
-                test_object_mode(S, any, k + 1, a1, b1, c1, ro, rd, lim, inst, hi, ho) && any) {
-                done = true;
-                break;
-            }
-            k += 2;
-        }
-        if (done) {
-            blocked = true;
-        } else {
-            sort4(key, code);
-            int m = 0;
-#pragma unroll
-            for (int j = 0; j < 4; j++) m += !(key[j] > lim) ? 1 : 0;
-            if (m > 0) {
-                for (int j = m - 1; j >= 1; j--) stk.push(code[j], key[j]);
-                cur = code[0];
-                continue;
-            }
-            bool next = false;
-            while (!next) {
-                if (in_blas && stk.sp == blas_base) {
-                    in_blas = false;
-                    ro = o;
-                    rd = d;
-                    ri = inv;
-                    continue;
-                }
-                if (stk.sp == 0) break;
-                int c;
-                float t;
-                stk.pop(c, t);
-                if (!(t > lim)) {
-                    cur = c;
-                    next = true;
-                }
-            }
-            if (next) continue;
-        }
-        // the current ray is resolved
-        if (any && closest) {
-            any = false;
-            hi = ho = -1;
-            o = v3(*co);
-            d = v3(*cd);
-            lim = kMax;
-            inv = v3(1 / d.x, 1 / d.y, 1 / d.z);
-            in_blas = false;
-            inst = -1;
-            ro = o;
-            rd = d;
-            ri = inv;
-            cur = 0;
-            stk.sp = stk.lo = 0;
-            continue;
-        }
-        if (!any) {
-            hit_inst = hi;
-            hit_obj = ho;
-            best = lim;
-        }
-        return;
-    }
-}
+                          int &hit_inst, int &hit_obj, TraversalStack<kRing> &stk);
 
 // Resumable form of traverse_pair's loop for kernels that interleave
 // traversal with other work: begin_*() sets a ray up, each step() visits one
@@ -818,6 +541,50 @@ struct RayWalk {
         }
     }
 };
+
+template <int kRing>
+LAM_FN void traverse_pair(const DeviceScene &S, bool occl, V3 so, V3 sd, float s_dist, int skip_inst, int skip_obj,
+                          bool &blocked, bool closest, const float4 *co, const float4 *cd, float &best,
+                          int &hit_inst, int &hit_obj, TraversalStack<kRing> &stk) {
+    blocked = false;
+    hit_inst = hit_obj = -1;
+    best = kMax;
+    if (!occl && !closest) return;
+    RayWalk w;
+    if (occl) w.begin(true, so, sd, s_dist, skip_inst, skip_obj);
+    else w.begin(false, v3(*co), v3(*cd), kMax, -1, -2);
+    stk.sp = stk.lo = 0;
+    while (true) {
+        if (!w.template step<kRing>(S, stk)) continue;
+        if (w.any) {
+            blocked = w.blocked;
+            if (!closest) return;
+            w.begin(false, v3(*co), v3(*cd), kMax, -1, -2);
+            stk.sp = stk.lo = 0;
+            continue;
+        }
+        hit_inst = w.hi;
+        hit_obj = w.ho;
+        best = w.lim;
+        return;
+    }
+}
+
+// traverse<kAny> through RayWalk: one ray, stepped until it is resolved.
+template <bool kAny, int kRing>
+LAM_FN bool traverse(const DeviceScene &S, V3 o, V3 d, float &best, int &hit_inst, int &hit_obj, int skip_inst,
+                     int skip_obj, TraversalStack<kRing> &stk) {
+    RayWalk w;
+    w.begin(kAny, o, d, best, skip_inst, skip_obj);
+    while (!w.template step<kRing>(S, stk)) {
+    }
+    if (kAny) return w.blocked;
+    if (w.hi < 0) return false;
+    best = w.lim;
+    hit_inst = w.hi;
+    hit_obj = w.ho;
+    return true;
+}
 
 // --------------------------------------------------------------- textures
 
