@@ -564,13 +564,18 @@ def main() -> int:
     thr = None
     if mode == lambda_amd.LAMBDA_RNG_REFERENCE and args.throughput_steps > 0:
         thr_img = np.zeros_like(img)
-        dev.render(rng_mode=lambda_amd.LAMBDA_RNG_THROUGHPUT, out=thr_img)
+        thr_sched = {}
+        if not args.no_autotune:
+            from lambda_amd import autotune
+            kw = {"cache": args.tune_cache} if args.tune_cache else {}
+            thr_sched = dict(autotune.tune(dev, trial_spp=16, rng_mode=lambda_amd.LAMBDA_RNG_THROUGHPUT, **kw)["best"])
+        dev.render(rng_mode=lambda_amd.LAMBDA_RNG_THROUGHPUT, out=thr_img, **thr_sched)
         t0 = time.perf_counter()
         for _ in range(args.throughput_steps):
-            dev.render(rng_mode=lambda_amd.LAMBDA_RNG_THROUGHPUT, out=thr_img)
+            dev.render(rng_mode=lambda_amd.LAMBDA_RNG_THROUGHPUT, out=thr_img, **thr_sched)
         t_thr = (time.perf_counter() - t0) / args.throughput_steps
         thr = {"value": paths_per_step / t_thr, "unit": "paths/s", "ms_per_step": t_thr * 1e3,
-               "steps": args.throughput_steps}
+               "steps": args.throughput_steps, "schedule": thr_sched}
         log("throughput streams: %.4g paths/s" % thr["value"])
 
     # Algorithmic bytes per closest-hit ray, counted by the oracle on a

@@ -199,6 +199,11 @@ typedef struct lambda_render_params {
                               order (4, 8 or 16; 0 = 8) */
     int32_t occupancy;     /* persistent kernel variant: waves per SIMD its
                               registers are capped for (2, 3 or 4; 0 = 4) */
+    int32_t vote;          /* persistent schedule: 0 = each lane traverses and
+                              shades in turn; 1 = the wave alternates traversal
+                              steps and shading, shading when at least half
+                              of its lanes wait (3 or 4 waves per SIMD; same
+                              image, faster with throughput streams) */
 } lambda_render_params;
 
 /*

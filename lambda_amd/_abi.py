@@ -49,6 +49,7 @@ class RenderParams(C.Structure):
         ("seed", C.c_uint64), ("rng_mode", C.c_int32), ("path_slots", C.c_int32),
         ("tail_paths", C.c_int32), ("cost_order", C.c_int32), ("graph_passes", C.c_int32),
         ("tile", C.c_int32), ("occupancy", C.c_int32),
+        ("vote", C.c_int32),
     ]
 
 

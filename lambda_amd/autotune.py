@@ -43,6 +43,12 @@ def candidates(npix: int, rng_mode: int) -> list[dict]:
     if rng_mode != LAMBDA_RNG_REFERENCE:
         c = [dict(x, cost_order=0) for x in c if x["cost_order"] in (4, -1)]
         c.append(dict(base, cost_order=0, path_slots=1 << 20))
+        # the voting schedule (wave alternates traversal steps and shading);
+        # only with independent streams: with per-pixel sample chains a
+        # short trial can favour it while the full frame, bound by its
+        # slowest chains, is slower
+        c.append(dict(base, cost_order=0, vote=1))
+        c.append(dict(base, cost_order=0, vote=1, occupancy=3))
     return c
 
 

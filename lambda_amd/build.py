@@ -30,7 +30,7 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
 HOST_SOURCES = ["scene_model.cpp", "parser.cpp", "host_api.cpp"]
-HIP_SOURCES = ["lambda_hip.hip", "k_passes.hip", "k_shade.hip", "k_persist.hip", "k_persist_occ.hip", "k_persist_guided.hip",
+HIP_SOURCES = ["lambda_hip.hip", "k_passes.hip", "k_shade.hip", "k_persist.hip", "k_persist_occ.hip", "k_persist_vote.hip", "k_persist_guided.hip",
                "k_persist_table.hip", "k_serve.hip", "k_query.hip", "guide_net.hip", "lbvh.hip"]
 
 # No -march: the host front end must round exactly like the reference's

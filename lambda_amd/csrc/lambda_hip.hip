@@ -419,6 +419,8 @@ void validate_params(const lambda_render_params &p) {
         throw Failure{LAMBDA_EINVAL, "invalid tile (4, 8 or 16; 0 = 8)"};
     if (p.occupancy != 0 && (p.occupancy < 2 || p.occupancy > 4))
         throw Failure{LAMBDA_EINVAL, "invalid occupancy (2, 3 or 4 waves per SIMD; 0 = 4)"};
+    if (p.vote != 0 && p.vote != 1) throw Failure{LAMBDA_EINVAL, "invalid vote (0 or 1)"};
+    if (p.vote && p.occupancy == 2) throw Failure{LAMBDA_EINVAL, "the voting schedule runs at 3 or 4 waves per SIMD"};
     if ((long long)p.width * p.height > (1LL << 28)) throw Failure{LAMBDA_EINVAL, "image too large"};
     // work-unit counters are 32-bit and may run past the last unit by the
     // number of slots (at most 2^28)
@@ -456,6 +458,10 @@ long long resident_threads(int blocks) {
 }
 
 long long resident_persist_threads(int occupancy = 0) { return resident_threads(persist_blocks_per_cu(occupancy)); }
+
+// The persistent-kernel variant code of a render: waves per SIMD, plus
+// kPersistVoteBit for the voting schedule.
+int persist_variant(const lambda_render_params &p) { return p.occupancy | (p.vote ? kPersistVoteBit : 0); }
 
 Frame make_frame(const lambda_render_params &p) {
     Frame F{};  // every optional pointer (cost, train) starts null
@@ -884,7 +890,7 @@ void render_impl(lambda_hip_scene &g, const lambda_render_params &p, float *out_
     // Slots: paths in flight.  Reference mode: at most one per pixel (a slot
     // owns a pixel for all its samples).
     const long long units = F.mode == LAMBDA_RNG_REFERENCE ? F.npix : F.total_work;
-    long long slots = p.path_slots > 0 ? p.path_slots : resident_persist_threads(p.occupancy);
+    long long slots = p.path_slots > 0 ? p.path_slots : resident_persist_threads(persist_variant(p));
     slots = std::max<long long>(1, std::min<long long>(slots, units));
     if (slots > (1LL << 28)) throw Failure{LAMBDA_EINVAL, "too many path slots"};
     // Below this many live paths the rest of the render runs in one
@@ -922,7 +928,7 @@ void render_with(Renderer &R, const Frame &F, const lambda_render_params &p, lon
         R.P.order = R.B.order.p;
         R.tile = tile;
     }
-    R.occupancy = p.occupancy;
+    R.occupancy = persist_variant(p);
     // reference streams, persistent schedule, more pixels than slots: the
     // two-launch cost order (cost_order samples first; 0 = automatic: 4, the
     // best measured on the 1080p instanced-mesh frame against 1, 2 and 8)

@@ -30,6 +30,13 @@ int persist_blocks_per_cu(int occupancy);
 void launch_persist_occ(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, int n_in, int2 *spill,
                         int occupancy);
 int persist_occ_blocks_per_cu(int occupancy);
+// The voting variant of the persistent kernel (3 or 4 waves per SIMD).
+void launch_persist_vote(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, int n_in,
+                         int2 *spill, int occupancy);
+int persist_vote_blocks_per_cu(int occupancy);
+// Bit of a Renderer's persistent-kernel variant code that selects the
+// voting schedule (the low bits: waves per SIMD, 0 = default).
+constexpr int kPersistVoteBit = 0x100;
 void launch_persist_guided(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, int n_in, int2 *spill, GuideDev G);
 int persist_guided_blocks_per_cu();
 void launch_persist_table(dim3 grid, dim3 block, hipStream_t st, DeviceScene S, Frame F, Paths P, int n_in, int2 *spill,

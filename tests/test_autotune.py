@@ -28,6 +28,9 @@ def test_candidates_cover_the_schedules():
     # tile sizes and the compiled occupancy variants
     assert {c["tile"] for c in ref} == {4, 8, 16}
     assert {c["occupancy"] for c in ref} == {2, 3, 4}
+    # the voting schedule is tried with independent streams only
+    assert not any(c.get("vote") for c in ref)
+    assert {c["occupancy"] for c in thr if c.get("vote")} == {3, 4}
 
 
 def test_key_depends_on_scene_and_frame_and_cache_is_used(tmp_path):

@@ -267,6 +267,32 @@ def test_persistent_tail_matches_bounce_passes(name, devices):
     assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
 
 
+@pytest.mark.parametrize("name", ["box", "film", "mesh"])
+def test_voting_schedule_matches_per_lane_schedule(name, devices):
+    """The voting persistent schedule (the wave alternates traversal steps
+    and shading) runs every lane's steps in the same order as the per-lane
+    schedule: with reference streams the images agree bit for bit, at 4 and
+    3 waves per SIMD, with one launch or the two-launch cost order and with
+    fewer slots than pixels."""
+    a = devices[name].render(tail_paths=-1, cost_order=-1)
+    for kw in (dict(vote=1), dict(vote=1, occupancy=3), dict(vote=1, cost_order=-1),
+               dict(vote=1, path_slots=777, cost_order=2)):
+        b = devices[name].render(**kw)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), kw
+    # throughput streams: only the order of the FP64 framebuffer additions differs
+    kw = dict(rng_mode=lambda_amd.LAMBDA_RNG_THROUGHPUT, path_slots=4096)
+    c = devices[name].render(**kw)
+    d = devices[name].render(vote=1, **kw)
+    np.testing.assert_allclose(d, c, rtol=1e-6, atol=1e-7)
+
+
+def test_voting_schedule_rejects_two_waves(devices):
+    with pytest.raises(lambda_amd.LambdaError):
+        devices["box"].render(vote=1, occupancy=2)
+    with pytest.raises(lambda_amd.LambdaError):
+        devices["box"].render(vote=2)
+
+
 def test_persistent_tail_throughput_mode(devices):
     """Throughput streams: each work item has its own stream, so only the
     order of the FP64 framebuffer additions differs between schedules."""
