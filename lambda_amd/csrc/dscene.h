@@ -837,7 +837,9 @@ LAM_OUTLINE float thinfilm_reflectance_at(const float2 *cspectra, const int *pro
         // cos and sin of the complex phase, both divided by cosh(Im delta)
         FF sa, ca;
         ffsincos(delta.re, sa, ca);
-        const FF th = fftanh(delta.im);
+        // a lossless layer has a real phase: tanh(0) = 0 exactly, so its
+        // exp series is skipped
+        const FF th = delta.im.hi == 0 && delta.im.lo == 0 ? ff(0.0f) : fftanh(delta.im);
         const CFF cd = CFF{ca, -(sa * th)};
         const CFF isd = CFF{-(ca * th), sa};  // i * sin(delta) / cosh
         const CFF es = nj * cj, ep = cj / nj;
