@@ -58,6 +58,7 @@ __device__ __forceinline__ void gemm_rows16(const __bf16 *A, int lda, const __bf
                                             f32x4 (&acc)[4]) {
     const int l = threadIdx.x & 63;
     const int r = l & 15, q = l >> 4;
+#pragma unroll
     for (int kk = 0; kk < K; kk += 32) {
         const bf16x8 a = load8(A + r * lda + kk + 8 * q);
 #pragma unroll

@@ -105,6 +105,9 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
     __syncthreads();
 
     int cur = 0;
+    // unrolled: each layer's shapes are constants, so the products' loops
+    // unroll and their weight loads are issued together
+#pragma unroll
     for (int layer = kGuideLayers - 1; layer >= 0; layer--) {
         const int K = guide_layer_in(layer);
         const __bf16 *Hprev = layer == 0 ? sX : sH[layer - 1];
