@@ -105,6 +105,14 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
     __syncthreads();
 
     int cur = 0;
+    // The weight-gradient tiles and bias sums of every layer stay in
+    // registers until the backward pass ends and are added to `grads` then:
+    // a barrier waits for the block's outstanding memory operations, so
+    // atomics issued per layer would put their latency into every layer's
+    // step of the chain.
+    constexpr int kMaxTiles = kGuideIn / 16;   // tiles per wave of the widest layer
+    f32x4 dw[kGuideLayers][kMaxTiles];
+    float db[kGuideLayers];
     // unrolled: each layer's shapes are constants, so the products' loops
     // unroll and their weight loads are issued together
 #pragma unroll
@@ -113,11 +121,12 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
         const __bf16 *Hprev = layer == 0 ? sX : sH[layer - 1];
         const int ldp = layer == 0 ? kLdX : kLdH;
         const __bf16 *dZ = sD[cur];
-        float *gW = grads + guide_w_offset(layer);
-        float *gb = grads + guide_b_offset(layer);
-        // dW_l[K x 64] = H_prev^T dZ over the block's 64 rows: tiles (kt, nt)
+        // dW_l[K x 64] = H_prev^T dZ over the block's 64 rows: tiles (kt, nt),
+        // tile w + 4 i for this wave
         const int ktiles = K / 16;
-        for (int t = w; t < ktiles * 4; t += kGuideWaves) {
+#pragma unroll
+        for (int i = 0; i < ktiles; i++) {
+            const int t = w + kGuideWaves * i;
             const int kt = t >> 2, nt = t & 3;
             f32x4 acc = {0, 0, 0, 0};
 #pragma unroll
@@ -128,17 +137,14 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
                 const bf16x8 b = load8_strided(dZ + (rr + 8 * q) * kLdH + nt * 16 + r16, kLdH);
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
             }
-#pragma unroll
-            for (int rr = 0; rr < 4; rr++) {
-                const int k = kt * 16 + 4 * q + rr, c = nt * 16 + r16;
-                atomicAdd(gW + k * 64 + c, acc[rr]);
-            }
+            dw[layer][i] = acc;
         }
         // db_l = column sums of dZ
+        db[layer] = 0;
         if (threadIdx.x < 64) {
-            float s = 0;
-            for (int r = 0; r < kGuideRowsPerBlock; r++) s += float(dZ[r * kLdH + threadIdx.x]);
-            atomicAdd(gb + threadIdx.x, s);
+            float sum = 0;
+            for (int r = 0; r < kGuideRowsPerBlock; r++) sum += float(dZ[r * kLdH + threadIdx.x]);
+            db[layer] = sum;
         }
         if (layer > 0) {
             // dH_prev[16 x 64] = dZ[16 x 64] W_l^T for this wave's rows; the
@@ -160,8 +166,24 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
                 }
             }
         }
-        __syncthreads();
+        if (layer > 0) __syncthreads();
         cur ^= 1;
+    }
+#pragma unroll
+    for (int layer = 0; layer < kGuideLayers; layer++) {
+        float *gW = grads + guide_w_offset(layer);
+        const int ktiles = guide_layer_in(layer) / 16;
+#pragma unroll
+        for (int i = 0; i < ktiles; i++) {
+            const int t = w + kGuideWaves * i;
+            const int kt = t >> 2, nt = t & 3;
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int k = kt * 16 + 4 * q + rr, c = nt * 16 + r16;
+                atomicAdd(gW + k * 64 + c, dw[layer][i][rr]);
+            }
+        }
+        if (threadIdx.x < 64) atomicAdd(grads + guide_b_offset(layer) + threadIdx.x, db[layer]);
     }
 }
 
