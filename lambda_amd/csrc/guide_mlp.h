@@ -26,6 +26,9 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kGuideRowsPerWave = 16;
 constexpr int kGuideWaves = 4;
 constexpr int kGuideRowsPerBlock = kGuideRowsPerWave * kGuideWaves;  // 64
+// Gradient kernel grid: at most this many blocks (two per compute unit at
+// 77 KB of LDS each on 256 compute units), each over several 64-row chunks.
+constexpr int kGradientBlocks = 512;
 // LDS row strides (elements), padded by 8 to spread banks.
 constexpr int kLdX = kGuideIn + 8;
 constexpr int kLdH = kGuideHidden + 8;

@@ -41,7 +41,12 @@ __global__ void __launch_bounds__(kFwdThreads, 3) k_guide_forward(GuideDev G, co
 // gradient (four lanes per row), then backward through the five layers.  Weight
 // gradients dW_l = H_{l-1}^T dZ_l sum over the block's rows on the MFMA
 // (16 x 16 tiles shared among the block's waves).
-__global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float *samples, int n, float *grads,
+// kChunks: blocks loop over several 64-row chunks (batches above
+// kGradientBlocks chunks); otherwise each block takes one chunk.  The loop
+// keeps the gradient registers live through the forward pass as well, which
+// costs a small batch more than the atomics it saves.
+template <bool kChunks>
+__global__ void __launch_bounds__(256, 2) k_guide_gradient(GuideDev G, const float *samples, int n, float *grads,
                                                         float *loss, const float *c_cap) {
     __shared__ __attribute__((aligned(16))) __bf16 sX[kGuideRowsPerBlock * kLdX];
     __shared__ __attribute__((aligned(16))) __bf16 sH[4][kGuideRowsPerBlock * kLdH];   // H1..H4
@@ -49,8 +54,25 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
     __shared__ __attribute__((aligned(16))) float sO[kGuideRowsPerBlock * kLobeOut];  // raw outputs (40 used)
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int r16 = l & 15, q = l >> 4;
-    const int row0 = blockIdx.x * kGuideRowsPerBlock;
     const int wr = w * kGuideRowsPerWave;  // this wave's first row in the block
+    // The weight-gradient tiles and bias sums of every layer stay in
+    // registers over all of the block's 64-row chunks and are added to
+    // `grads` at the end, one atomic per parameter per block (the atomics'
+    // throughput bounds large batches); a barrier also waits for the
+    // block's outstanding memory operations, so atomics issued per layer
+    // would put their latency into every step of the chain.
+    constexpr int kMaxTiles = kGuideIn / 16;   // tiles per wave of the widest layer
+    f32x4 dw[kGuideLayers][kMaxTiles];
+    float db[kGuideLayers];
+#pragma unroll
+    for (int layer = 0; layer < kGuideLayers; layer++) {
+#pragma unroll
+        for (int i = 0; i < kMaxTiles; i++) dw[layer][i] = f32x4{0, 0, 0, 0};
+        db[layer] = 0;
+    }
+    const int nchunks = (n + kGuideRowsPerBlock - 1) / kGuideRowsPerBlock;
+    for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const int row0 = chunk * kGuideRowsPerBlock;
 
     // forward
     encode_rows(samples, kGuideSample, row0 + wr, n, sX + wr * kLdX);
@@ -105,14 +127,6 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
     __syncthreads();
 
     int cur = 0;
-    // The weight-gradient tiles and bias sums of every layer stay in
-    // registers until the backward pass ends and are added to `grads` then:
-    // a barrier waits for the block's outstanding memory operations, so
-    // atomics issued per layer would put their latency into every layer's
-    // step of the chain.
-    constexpr int kMaxTiles = kGuideIn / 16;   // tiles per wave of the widest layer
-    f32x4 dw[kGuideLayers][kMaxTiles];
-    float db[kGuideLayers];
     // unrolled: each layer's shapes are constants, so the products' loops
     // unroll and their weight loads are issued together
 #pragma unroll
@@ -128,7 +142,7 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
         for (int i = 0; i < ktiles; i++) {
             const int t = w + kGuideWaves * i;
             const int kt = t >> 2, nt = t & 3;
-            f32x4 acc = {0, 0, 0, 0};
+            f32x4 acc = dw[layer][i];
 #pragma unroll
             for (int rr = 0; rr < kGuideRowsPerBlock; rr += 32) {
                 // A = H_prev^T: A[i][rows] = H_prev[rows][kt*16 + i]
@@ -140,11 +154,10 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
             dw[layer][i] = acc;
         }
         // db_l = column sums of dZ
-        db[layer] = 0;
         if (threadIdx.x < 64) {
             float sum = 0;
             for (int r = 0; r < kGuideRowsPerBlock; r++) sum += float(dZ[r * kLdH + threadIdx.x]);
-            db[layer] = sum;
+            db[layer] += sum;
         }
         if (layer > 0) {
             // dH_prev[16 x 64] = dZ[16 x 64] W_l^T for this wave's rows; the
@@ -166,9 +179,11 @@ __global__ void __launch_bounds__(256) k_guide_gradient(GuideDev G, const float 
                 }
             }
         }
-        if (layer > 0) __syncthreads();
+        __syncthreads();
         cur ^= 1;
     }
+    if constexpr (!kChunks) break;
+    }  // chunks
 #pragma unroll
     for (int layer = 0; layer < kGuideLayers; layer++) {
         float *gW = grads + guide_w_offset(layer);
@@ -335,8 +350,12 @@ void GuideNet::forward_dev(const float *feat, const int *n_dev, int bound, float
 // at *c_cap when given).
 void GuideNet::gradient(const float *samples, int n, hipStream_t st, const float *c_cap) {
     if (n <= 0) return;
-    k_guide_gradient<<<(n + kGuideRowsPerBlock - 1) / kGuideRowsPerBlock, 256, 0, st>>>(dev(), samples, n,
-                                                                                        grads.p, loss.p, c_cap);
+    // one 64-row chunk per block, or kGradientBlocks blocks over several
+    const int chunks = (n + kGuideRowsPerBlock - 1) / kGuideRowsPerBlock;
+    if (chunks > kGradientBlocks)
+        k_guide_gradient<true><<<kGradientBlocks, 256, 0, st>>>(dev(), samples, n, grads.p, loss.p, c_cap);
+    else
+        k_guide_gradient<false><<<chunks, 256, 0, st>>>(dev(), samples, n, grads.p, loss.p, c_cap);
     HIP_OK(hipGetLastError());
 }
 

@@ -165,6 +165,24 @@ def test_gradient_is_additive_over_samples(guide):
     np.testing.assert_allclose(g, g1 + g2, rtol=1e-4, atol=1e-4 * np.abs(g).max())
 
 
+def test_gradient_over_several_chunks_per_block(guide):
+    """Batches above 512 x 64 rows run several 64-row chunks per block with
+    the gradient held in registers across them: the gradient of such a
+    batch (ragged tail included) equals the sum of the gradients of parts
+    that take one chunk per block."""
+    rng = np.random.default_rng(13)
+    guide.set_params(random_params(rng).astype(np.float32))
+    n = 2 * 512 * 64 + 37
+    s = random_samples(rng, n).astype(np.float32)
+    g, loss = guide.gradient(s)
+    cuts = (0, 30000, 60000, n)
+    parts = [guide.gradient(s[a:b]) for a, b in zip(cuts[:-1], cuts[1:])]
+    gsum = sum(p[0].astype(np.float64) for p in parts)
+    lsum = sum(p[1] for p in parts)
+    assert abs(loss - lsum) <= 1e-4 * abs(lsum)
+    np.testing.assert_allclose(g, gsum, rtol=1e-4, atol=1e-4 * np.abs(gsum).max())
+
+
 def target_samples(rng, n, kappa=20.0):
     """Directions from a vMF whose mean depends on the position feature."""
     s = np.zeros((n, 10), np.float32)
