@@ -105,6 +105,20 @@ def test_row_band_render_matches_oracle(full, spp):
     assert err < 1e-4
 
 
+def test_schedules_bit_identical_at_full_scale(full):
+    """With traversals that spill past the LDS ring: the default persistent
+    kernel, its 3- and 2-wave variants, the voting schedule and the
+    two-launch cost order render the frame (4 spp, reference streams) bit
+    for bit alike."""
+    s, dev = full
+    kw = dict(samples=4, sqrt_samples=2)
+    a = dev.render(cost_order=-1, **kw)
+    for extra in (dict(occupancy=3, cost_order=-1), dict(occupancy=2, cost_order=-1), dict(vote=1, cost_order=-1),
+                  dict(cost_order=2), dict(vote=1, occupancy=3, cost_order=2)):
+        b = dev.render(**extra, **kw)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), extra
+
+
 def test_gpu_bvh_on_full_scale_scene(full):
     """The BVHs of the 1.31M-triangle object group and the 65 instances
     built on the GPU: hit distances bit-identical to the host-built
