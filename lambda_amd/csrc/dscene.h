@@ -222,8 +222,8 @@ LAM_FN void local_ray(const DeviceScene &S, int inst, V3 o, V3 d, V3 &lo, V3 &ld
 
 // World-space record of (instance, object) for a world ray: local test, then
 // point by M and normal by the transposed inverse (scene/scene.h:101-104).
-// Every record of (instance, object) is requested at once: one round trip
-// to memory instead of one per dependent step.
+// The transform and the object's intersection and shading records are read
+// before the test, so that none of their loads waits on the test's branches.
 LAM_FN bool hit_record(const DeviceScene &S, int inst, int obj, V3 o, V3 d, Hit &h) {
     const float4 *mp = S.inst + 8 * size_t(inst);
     float4 M[8];
