@@ -29,6 +29,13 @@ constexpr int kGuideRowsPerBlock = kGuideRowsPerWave * kGuideWaves;  // 64
 // Gradient kernel grid: at most this many blocks (two per compute unit at
 // 77 KB of LDS each on 256 compute units), each over several 64-row chunks.
 constexpr int kGradientBlocks = 512;
+// A gradient block's row of sums: the weight-gradient tiles as its 256
+// threads hold them (kGradSlots float4 each: six tiles of the input layer,
+// four of each other layer), the 5 x 64 bias sums, the loss, padding.
+constexpr int kGradSlots = kGuideIn / 16 + 4 * (kGuideLayers - 1);
+constexpr int kGradRow4 = kGradSlots * 256 + (kGuideLayers * 64 + 1 + 3) / 4;
+// slot of tile i of layer l
+__host__ __device__ constexpr int grad_slot(int l) { return l == 0 ? 0 : kGuideIn / 16 + 4 * (l - 1); }
 // LDS row strides (elements), padded by 8 to spread banks.
 constexpr int kLdX = kGuideIn + 8;
 constexpr int kLdH = kGuideHidden + 8;

@@ -12,6 +12,7 @@ struct GuideNet {
     DevBuf<float> params, grads, m, v, cap;
     DevBuf<__bf16> w_nk, w_kn;
     DevBuf<float> loss;       // summed loss of the last gradient or training call
+    DevBuf<float> part;       // per-block gradient sums: kGradientBlocks rows of kGradRow4 float4
     int step = 0;
     float lr = 1e-3f;
 

@@ -35,8 +35,9 @@ __global__ void __launch_bounds__(kFwdThreads, 3) k_guide_forward(GuideDev G, co
 }
 
 
-// Gradient of sum_i -c_i log q(w_i) over samples [0, n) into grads (fp32,
-// parameter layout, accumulated with atomics) and the loss into *loss.
+// Gradient of sum_i -c_i log q(w_i) over samples [0, n), as per-block sums
+// in `part` (one row of kGradRow4 float4 per block, layout below) that
+// k_guide_reduce adds to grads (fp32, parameter layout) and *loss.
 // Each block takes 64 rows: forward with stored activations, the loss
 // gradient (four lanes per row), then backward through the five layers.  Weight
 // gradients dW_l = H_{l-1}^T dZ_l sum over the block's rows on the MFMA
@@ -44,10 +45,14 @@ __global__ void __launch_bounds__(kFwdThreads, 3) k_guide_forward(GuideDev G, co
 // kChunks: blocks loop over several 64-row chunks (batches above
 // kGradientBlocks chunks); otherwise each block takes one chunk.  The loop
 // keeps the gradient registers live through the forward pass as well, which
-// costs a small batch more than the atomics it saves.
+// costs a small batch more than the rows it saves.
+// Per-block sums instead of atomics: the blocks of a launch reach their
+// 22,848 additions together, and same-address atomics from every block
+// (and one loss atomic per wave, which the next weight loads waited
+// behind) bounded the kernel.
 template <bool kChunks>
-__global__ void __launch_bounds__(256, 2) k_guide_gradient(GuideDev G, const float *samples, int n, float *grads,
-                                                        float *loss, const float *c_cap) {
+__global__ void __launch_bounds__(256, 2) k_guide_gradient(GuideDev G, const float *samples, int n, float4 *part,
+                                                        const float *c_cap) {
     __shared__ __attribute__((aligned(16))) __bf16 sX[kGuideRowsPerBlock * kLdX];
     __shared__ __attribute__((aligned(16))) __bf16 sH[4][kGuideRowsPerBlock * kLdH];   // H1..H4
     __shared__ __attribute__((aligned(16))) __bf16 sD[2][kGuideRowsPerBlock * kLdH];   // dZ ping-pong
@@ -55,15 +60,12 @@ __global__ void __launch_bounds__(256, 2) k_guide_gradient(GuideDev G, const flo
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int r16 = l & 15, q = l >> 4;
     const int wr = w * kGuideRowsPerWave;  // this wave's first row in the block
-    // The weight-gradient tiles and bias sums of every layer stay in
-    // registers over all of the block's 64-row chunks and are added to
-    // `grads` at the end, one atomic per parameter per block (the atomics'
-    // throughput bounds large batches); a barrier also waits for the
-    // block's outstanding memory operations, so atomics issued per layer
-    // would put their latency into every step of the chain.
+    // The weight-gradient tiles, bias sums and loss stay in registers over
+    // all of the block's 64-row chunks and are written once at the end.
     constexpr int kMaxTiles = kGuideIn / 16;   // tiles per wave of the widest layer
     f32x4 dw[kGuideLayers][kMaxTiles];
-    float db[kGuideLayers];
+    float db[kGuideLayers];   // column l of dZ over this wave's 16 rows
+    float lacc = 0;           // loss of this lane's rows
 #pragma unroll
     for (int layer = 0; layer < kGuideLayers; layer++) {
 #pragma unroll
@@ -120,9 +122,7 @@ __global__ void __launch_bounds__(256, 2) k_guide_gradient(GuideDev G, const flo
         for (int j = 0; j < 10; j++) dz[10 * part + j] = __bf16(valid ? g[j] : 0.0f);
 #pragma unroll
         for (int j = 0; j < 6; j++) dz[kLobeOut + 6 * part + j] = __bf16(0.0f);
-        float lsum = valid && part == 0 ? -c * logf(q) : 0.0f;
-        for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
-        if (l == 0) atomicAdd(loss, lsum);
+        if (valid && part == 0) lacc += -c * logf(q);
     }
     __syncthreads();
 
@@ -153,10 +153,11 @@ __global__ void __launch_bounds__(256, 2) k_guide_gradient(GuideDev G, const flo
             }
             dw[layer][i] = acc;
         }
-        // db_l = column sums of dZ
-        if (threadIdx.x < 64) {
+        // db_l: column l of dZ over this wave's 16 rows
+        {
             float sum = 0;
-            for (int r = 0; r < kGuideRowsPerBlock; r++) sum += float(dZ[r * kLdH + threadIdx.x]);
+#pragma unroll
+            for (int r = 0; r < kGuideRowsPerWave; r++) sum += float(dZ[(wr + r) * kLdH + l]);
             db[layer] += sum;
         }
         if (layer > 0) {
@@ -184,21 +185,88 @@ __global__ void __launch_bounds__(256, 2) k_guide_gradient(GuideDev G, const flo
     }
     if constexpr (!kChunks) break;
     }  // chunks
+    // this block's row of `part`: float4 slot j of thread x at j * 256 + x
+    // (each store instruction writes 1 KB in one piece), then the bias
+    // columns and the loss summed over the four waves through LDS (sO is
+    // free once the last barrier has passed)
+    float4 *row = part + size_t(blockIdx.x) * kGradRow4;
 #pragma unroll
     for (int layer = 0; layer < kGuideLayers; layer++) {
-        float *gW = grads + guide_w_offset(layer);
         const int ktiles = guide_layer_in(layer) / 16;
 #pragma unroll
         for (int i = 0; i < ktiles; i++) {
-            const int t = w + kGuideWaves * i;
-            const int kt = t >> 2, nt = t & 3;
+            const f32x4 v = dw[layer][i];
+            row[(grad_slot(layer) + i) * 256 + threadIdx.x] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+        sO[(layer * kGuideWaves + w) * 64 + l] = db[layer];
+    }
+    for (int off = 32; off > 0; off >>= 1) lacc += __shfl_xor(lacc, off);
+    if (l == 0) sO[kGuideLayers * kGuideWaves * 64 + w] = lacc;
+    __syncthreads();
+    float *tail = reinterpret_cast<float *>(row + kGradSlots * 256);
+    for (int e = threadIdx.x; e < kGuideLayers * 64; e += blockDim.x) {
+        const float *s4 = sO + (e >> 6) * kGuideWaves * 64 + (e & 63);
+        tail[e] = ((s4[0] + s4[64]) + s4[128]) + s4[192];
+    }
+    if (threadIdx.x == 0) {
+        const float *s4 = sO + kGuideLayers * kGuideWaves * 64;
+        tail[kGuideLayers * 64] = ((s4[0] + s4[1]) + s4[2]) + s4[3];
+    }
+}
+
+// grads += the sum of the nb rows of `part` (and *loss += their loss).  A
+// block takes 64 float4 columns and 64 rows (four groups of 16); its sums
+// are added with one atomic per float per block.
+__global__ void __launch_bounds__(256) k_guide_reduce(const float4 *part, int nb, float *grads, float *loss) {
+    __shared__ float4 red[4][64];
+    const int x = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int col = blockIdx.x * 64 + x;
+    const int b0 = blockIdx.y * 64 + g * 16;
+    float4 s = make_float4(0, 0, 0, 0);
+    if (col < kGradRow4) {
 #pragma unroll
-            for (int rr = 0; rr < 4; rr++) {
-                const int k = kt * 16 + 4 * q + rr, c = nt * 16 + r16;
-                atomicAdd(gW + k * 64 + c, dw[layer][i][rr]);
+        for (int k = 0; k < 16; k++) {
+            if (b0 + k < nb) {
+                const float4 v = part[size_t(b0 + k) * kGradRow4 + col];
+                s.x += v.x;
+                s.y += v.y;
+                s.z += v.z;
+                s.w += v.w;
             }
         }
-        if (threadIdx.x < 64) atomicAdd(grads + guide_b_offset(layer) + threadIdx.x, db[layer]);
+    }
+    red[g][x] = s;
+    __syncthreads();
+    if (g != 0 || col >= kGradRow4) return;
+#pragma unroll
+    for (int k = 1; k < 4; k++) {
+        s.x += red[k][x].x;
+        s.y += red[k][x].y;
+        s.z += red[k][x].z;
+        s.w += red[k][x].w;
+    }
+    const float v[4] = {s.x, s.y, s.z, s.w};
+    if (col < kGradSlots * 256) {
+        // weight-gradient slot j of thread t: tile (layer, i) of wave t / 64,
+        // rows 4 q .. 4 q + 3 of column r16 (the MFMA's accumulator layout)
+        const int j = col >> 8, t = col & 255;
+        int layer = 0, i = j;
+        if (j >= kGuideIn / 16) {
+            layer = 1 + (j - kGuideIn / 16) / 4;
+            i = (j - kGuideIn / 16) % 4;
+        }
+        const int w = t >> 6, q = (t & 63) >> 4, r16 = t & 15;
+        const int tile = w + kGuideWaves * i, kt = tile >> 2, nt = tile & 3;
+        float *gW = grads + guide_w_offset(layer);
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) atomicAdd(gW + (kt * 16 + 4 * q + rr) * 64 + nt * 16 + r16, v[rr]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int e = (col - kGradSlots * 256) * 4 + k;  // bias index layer * 64 + c, then the loss
+            if (e < kGuideLayers * 64) atomicAdd(grads + guide_b_offset(e >> 6) + (e & 63), v[k]);
+            else if (e == kGuideLayers * 64) atomicAdd(loss, v[k]);
+        }
     }
 }
 
@@ -293,6 +361,7 @@ void GuideNet::alloc() {
     w_kn.alloc(kGuideWeightsBf);
     loss.alloc(1);
     cap.alloc(1);
+    part.alloc(size_t(kGradientBlocks) * kGradRow4 * 4);
 }
 
 // He-uniform weights from a seeded PCG32, zero biases; the output layer
@@ -352,10 +421,14 @@ void GuideNet::gradient(const float *samples, int n, hipStream_t st, const float
     if (n <= 0) return;
     // one 64-row chunk per block, or kGradientBlocks blocks over several
     const int chunks = (n + kGuideRowsPerBlock - 1) / kGuideRowsPerBlock;
+    const int blocks = std::min(chunks, kGradientBlocks);
+    float4 *rows = reinterpret_cast<float4 *>(part.p);
     if (chunks > kGradientBlocks)
-        k_guide_gradient<true><<<kGradientBlocks, 256, 0, st>>>(dev(), samples, n, grads.p, loss.p, c_cap);
+        k_guide_gradient<true><<<blocks, 256, 0, st>>>(dev(), samples, n, rows, c_cap);
     else
-        k_guide_gradient<false><<<chunks, 256, 0, st>>>(dev(), samples, n, grads.p, loss.p, c_cap);
+        k_guide_gradient<false><<<blocks, 256, 0, st>>>(dev(), samples, n, rows, c_cap);
+    HIP_OK(hipGetLastError());
+    k_guide_reduce<<<dim3((kGradRow4 + 63) / 64, (blocks + 63) / 64), 256, 0, st>>>(rows, blocks, grads.p, loss.p);
     HIP_OK(hipGetLastError());
 }
 
