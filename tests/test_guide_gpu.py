@@ -134,11 +134,14 @@ def test_training_batch_is_one_adam_step_of_the_oracle_gradient():
     assert err < 5e-3
 
 
-def test_gradient_matches_oracle(guide):
+# 1000 rows: 16 blocks, one row group of the reduction; 6437 rows: 101
+# blocks, a full group of 64 rows and a ragged one of 37, ragged last chunk
+@pytest.mark.parametrize("n", [1000, 64 * 100 + 37])
+def test_gradient_matches_oracle(guide, n):
     rng = np.random.default_rng(11)
     p = random_params(rng).astype(np.float32)
     guide.set_params(p)
-    s = random_samples(rng, 1000).astype(np.float32)
+    s = random_samples(rng, n).astype(np.float32)
     g, loss = guide.gradient(s)
     gref, lref = go.gradient(p.astype(np.float64), s.astype(np.float64), bf=True)
     print("loss gpu %.6g oracle %.6g" % (loss, lref), flush=True)
